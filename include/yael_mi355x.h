@@ -1,0 +1,107 @@
+/*
+ * yael_mi355x.h -- public C-ABI of libyael_mi355x.so (yael_amd/_lib/).
+ *
+ * Part 1 is yael's own API for the k-means / exhaustive-kNN path: same
+ * names, signatures, argument order (base before queries), layout
+ * (column-major d x n fvecs = row-major n x d floats) and error behaviour
+ * (abort on misuse, kmeans returns -1 on abandon).  Each declaration cites
+ * the reference declaration it replaces.  Thread-count arguments are
+ * accepted and ignored: the GPU grid replaces yael's OpenMP slices.
+ *
+ * Part 2 are status-returning twins of part 1 (same arguments) for callers
+ * that want an error code instead of an abort.
+ *
+ * Part 3 is the device-level layer (device pointers, the library's HIP
+ * stream), used by the benchmark and by callers that keep data resident in
+ * HBM.  It is declared in yael_amd/csrc/yk.h, which this header includes.
+ */
+#ifndef YAEL_MI355X_H_INCLUDED
+#define YAEL_MI355X_H_INCLUDED
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- part 1: yael's API ---------------- */
+
+/* kmeans flags, kmeans.h:11-18 */
+#define KMEANS_QUIET 0x10000
+#define KMEANS_INIT_BERKELEY 0x20000 /* not served on the GPU: error */
+#define KMEANS_NORMALIZE_CENTS 0x40000
+#define KMEANS_INIT_RANDOM 0x80000
+#define KMEANS_INIT_USER 0x100000
+#define KMEANS_L1 0x200000   /* not served on the GPU: error */
+#define KMEANS_CHI2 0x400000 /* not served on the GPU: error */
+
+/* replaces kmeans.h:41-44 (kmeans.c:332-447) */
+float kmeans(int d, int n, int k, int niter, const float *v, int flags, long seed, int redo,
+             float *centroids, float *dis, int *assign, int *nassign);
+
+/* replaces kmeans.h:49-51, 56-59, 63-66 (kmeans.c:452-504) */
+float *clustering_kmeans(int n, int d, const float *points, int k, int nb_iter_max,
+                         double normalize);
+float *clustering_kmeans_assign(int n, int d, const float *points, int k, int nb_iter_max,
+                                double normalize, int **clust_assign_out);
+float *clustering_kmeans_assign_with_score(int n, int d, const float *points, int k,
+                                           int nb_iter_max, double normalize, int n_thread,
+                                           double *score_out, int **clust_assign_out);
+
+/* replaces nn.h:41-45 (nn.c:451-525); distance_type 2/12 (L2) or 16 (dot) */
+void knn_full(int distance_type, int nq, int nb, int d, int k, const float *b, const float *q,
+              const float *b_weights, int *assign, float *dis);
+
+/* replaces nn.h:49-54 (nn.c:679-699) */
+void knn_full_thread(int distance_type, int nq, int nb, int d, int k, const float *b,
+                     const float *q, const float *b_weights, int *assign, float *dis,
+                     int n_thread);
+
+/* replace nn.h:60-83 (nn.c:608-632, 704-726) */
+double nn(int n, int nb, int d, const float *b, const float *v, int *assign);
+double nn_thread(int n, int nb, int d, const float *b, const float *v, int *assign,
+                 int n_thread);
+float *knn(int n, int nb, int d, int k, const float *b, const float *v, int *assign);
+float *knn_thread(int nq, int nb, int d, int k, const float *b, const float *v, int *assign,
+                  int n_thread);
+
+/* replaces nn.h:102-104 (nn.c:528-580) */
+void knn_reorder_shortlist(int n, int nb, int d, int k, const float *b, const float *v,
+                           int *idx, float *dis);
+
+/* replaces nn.h:141-143 (nn.c:92-97) */
+void compute_cross_distances(int d, int na, int nb, const float *a, const float *b,
+                             float *dist2);
+/* replaces nn.h:151-154 (nn.c:100-129) */
+void compute_cross_distances_nonpacked(int d, int na, int nb, const float *a, int lda,
+                                       const float *b, int ldb, float *dist2, int ldd);
+/* replaces nn.h:156-159 (nn.c:777-792) */
+void compute_cross_distances_thread(int d, int na, int nb, const float *a, const float *b,
+                                    float *dist2, int nt);
+/* replaces nn.h:174-176, 180-183, 188-191 (nn.c:280-356, 795-810); types 2/12/16 */
+void compute_cross_distances_alt(int distance_type, int d, int na, int nb, const float *a,
+                                 const float *b, float *dist2);
+void compute_cross_distances_alt_nonpacked(int distance_type, int d, int na, int nb,
+                                           const float *a, int lda, const float *b, int ldb,
+                                           float *dist2, int ldd);
+void compute_cross_distances_alt_thread(int distance_type, int d, int na, int nb,
+                                        const float *a, const float *b, float *dist2, int nt);
+
+/* ---------------- part 2: status-returning twins (0 = ok, < 0 = YK_E*) ---------------- */
+
+int yael_knn_full_checked(int distance_type, int nq, int nb, int d, int k, const float *b,
+                          const float *q, const float *b_weights, int *assign, float *dis);
+int yael_cross_distances_checked(int distance_type, int d, int na, int nb, const float *a,
+                                 int lda, const float *b, int ldb, float *dist2, int ldd);
+int yael_kmeans_checked(int d, int n, int k, int niter, const float *v, int flags, long seed,
+                        int redo, float *centroids, float *dis, int *assign, int *nassign,
+                        float *ret);
+int yael_knn_reorder_shortlist_checked(int n, int nb, int d, int k, const float *b,
+                                       const float *v, int *idx, float *dis);
+
+#ifdef __cplusplus
+}
+#endif
+
+/* ---------------- part 3: device-level layer ---------------- */
+#include "../yael_amd/csrc/yk.h"
+
+#endif

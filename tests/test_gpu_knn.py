@@ -1,0 +1,203 @@
+"""GPU parity of the kNN / cross-distance path (through the C-ABI library).
+
+Against the compiled reference's golden vectors and against the oracle on
+seeded inputs: bit-exact on integer-valued data (S11: all arithmetic exact),
+near-tie rule elsewhere (tests/parity.py).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import parity
+from oracle_bind import Oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    with np.load(os.path.join(GOLD, name + ".npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.fixture(scope="module")
+def ya():
+    import yael_amd
+    yael_amd.init()
+    print("device:", yael_amd.device_info(), flush=True)
+    return yael_amd
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    o = Oracle()
+    o.set_blas(0, 0)
+    return o
+
+
+def mixture(rng, n, d, ncomp, sigma, integer=True):
+    cen = rng.uniform(0, 255, (ncomp, d))
+    x = cen[rng.integers(0, ncomp, n)] + sigma * rng.standard_normal((n, d))
+    x = np.clip(x, 0, 255)
+    if integer:
+        x = np.rint(x)
+    return x.astype(np.float32)
+
+
+KNN_CASES = ["l2_k1", "l2_k10", "dot_k5", "ties_k1", "ties_k12", "w_k1", "w_k7", "nan_k1", "nan_k39",
+             "mix_k1", "mix_k100"]
+
+
+@pytest.mark.parametrize("case", KNN_CASES)
+def test_knn_vs_reference_golden(ya, case):
+    g = load(case)
+    typ, k = int(g["typ"]), int(g["k"])
+    w = g.get("w")
+    print("case", case, flush=True)
+    idx, dis = ya.knn(g["q"], g["b"], k, distance_type=typ, weights=w)
+    exact = case.startswith(("mix", "ties"))
+    parity.check_knn(typ, g["q"], g["b"], k, idx, dis, g["assign"], g["dis"], w=w, exact=exact, label=case)
+
+
+@pytest.mark.parametrize("k", [1, 10, 100])
+@pytest.mark.parametrize("typ", [2, 16])
+def test_knn_integer_mixture_bitexact(ya, oracle, k, typ):
+    rng = np.random.default_rng(100 + k + typ)
+    b = mixture(rng, 20000, 128, 64, 20)
+    q = mixture(rng, 700, 128, 64, 20)
+    print("knn integer mixture", typ, k, flush=True)
+    gi, gd = ya.knn(q, b, k, distance_type=typ)
+    oi, od = oracle.knn_full(typ, b, q, k)
+    parity.check_knn(typ, q, b, k, gi, gd, oi, od, exact=True, label="mix%d" % k)
+
+
+@pytest.mark.parametrize("d", [16, 20, 128, 200, 960])
+@pytest.mark.parametrize("k", [1, 8])
+def test_knn_float_dims(ya, oracle, d, k):
+    rng = np.random.default_rng(d * 7 + k)
+    b = rng.standard_normal((3000, d)).astype(np.float32)
+    q = rng.standard_normal((333, d)).astype(np.float32)
+    print("knn float d", d, "k", k, flush=True)
+    gi, gd = ya.knn(q, b, k)
+    oi, od = oracle.knn_full(2, b, q, k)
+    r = parity.check_knn(2, q, b, k, gi, gd, oi, od, label="d%d" % d)
+    assert r["near_ties"] <= 2
+
+
+def test_knn_split_merge_large_k(ya, oracle):
+    """Few queries, many base rows: the base range is split across workgroups
+    and the partial lists merged; results must not depend on the split."""
+    rng = np.random.default_rng(7)
+    b = mixture(rng, 100000, 64, 128, 15)
+    q = mixture(rng, 37, 64, 128, 15)
+    for k in (1, 64, 250):
+        print("split/merge k", k, flush=True)
+        gi, gd = ya.knn(q, b, k)
+        oi, od = oracle.knn_full(2, b, q, k)
+        parity.check_knn(2, q, b, k, gi, gd, oi, od, exact=True, label="split k%d" % k)
+
+
+def test_knn_edge_shapes(ya, oracle):
+    rng = np.random.default_rng(8)
+    for nb, nq, d, k in [(1, 1, 1, 1), (5, 3, 7, 5), (33, 65, 3, 33), (257, 1, 128, 256), (64, 129, 17, 1)]:
+        b = rng.integers(0, 9, (nb, d)).astype(np.float32)
+        q = rng.integers(0, 9, (nq, d)).astype(np.float32)
+        print("edge", nb, nq, d, k, flush=True)
+        gi, gd = ya.knn(q, b, k)
+        oi, od = oracle.knn_full(2, b, q, k)
+        parity.check_knn(2, q, b, k, gi, gd, oi, od, exact=True, label="edge%d_%d_%d_%d" % (nb, nq, d, k))
+
+
+def test_knn_errors(ya):
+    b = np.zeros((10, 4), np.float32)
+    q = np.zeros((3, 4), np.float32)
+    with pytest.raises(ya.YaelGPUError):
+        ya.knn(q, b, 11)  # k > nb (nn.c:456)
+    with pytest.raises(ya.YaelGPUError):
+        ya.knn(q, b, 2, distance_type=3)  # chi2: no GPU path
+
+
+def test_c_api_knn_full(ya, oracle):
+    """The reference's own symbol, called the way a C caller would."""
+    lib = ya.lib()
+    rng = np.random.default_rng(9)
+    b = rng.integers(0, 256, (5000, 32)).astype(np.float32)
+    q = rng.integers(0, 256, (100, 32)).astype(np.float32)
+    k = 5
+    assign = np.empty((100, k), np.int32)
+    dis = np.empty((100, k), np.float32)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib.knn_full_thread(2, 100, 5000, 32, k, vp(b), vp(q), None, vp(assign), vp(dis), 8)
+    oi, od = oracle.knn_full(2, b, q, k)
+    parity.check_knn(2, q, b, k, assign, dis, oi, od, exact=True, label="c_api")
+    # nn(): sum of the nearest distances, in double
+    vw = np.empty(100, np.int32)
+    s = lib.nn(100, 5000, 32, vp(b), vp(q), vp(vw))
+    assert np.array_equal(vw, oi[:, 0]) and s == od[:, 0].astype(np.float64).sum()
+
+
+@pytest.mark.parametrize("typ", [2, 12, 16])
+def test_cross_distances(ya, oracle, typ):
+    rng = np.random.default_rng(10 + typ)
+    a = rng.integers(0, 256, (301, 40)).astype(np.float32)
+    b = rng.integers(0, 256, (129, 40)).astype(np.float32)
+    g = ya.cross_distances(a, b, typ)
+    o = oracle.cross_distances(a, b, 2 if typ == 12 else typ)
+    assert np.array_equal(g.view(np.int32), o.view(np.int32))
+
+
+def test_cross_distances_golden(ya):
+    for case in ("cross_l2", "cross_dot"):
+        gl = load(case)
+        a, b = gl["a"], gl["b"]
+        out = ya.cross_distances(a, b, int(gl["typ"]))
+        na = (a.astype(np.float64) ** 2).sum(1)
+        nb = (b.astype(np.float64) ** 2).sum(1)
+        bound = 2 * parity.gamma(a.shape[1]) * (na[None, :] + nb[:, None])
+        assert np.all(np.abs(out.astype(np.float64) - gl["dist"]) <= bound), case
+
+
+def test_cross_distances_nonpacked(ya, oracle):
+    lib = ya.lib()
+    rng = np.random.default_rng(11)
+    d, na, nb, lda, ldb, ldd = 24, 50, 30, 31, 40, 64
+    A = rng.integers(0, 50, (na, lda)).astype(np.float32)
+    B = rng.integers(0, 50, (nb, ldb)).astype(np.float32)
+    out = np.full((nb, ldd), -7.0, np.float32)
+    vp = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    lib.compute_cross_distances_nonpacked(d, na, nb, vp(A), lda, vp(B), ldb, vp(out), ldd)
+    ref = oracle.cross_distances(np.ascontiguousarray(A[:, :d]), np.ascontiguousarray(B[:, :d]), 2)
+    assert np.array_equal(out[:, :na], ref)
+    assert np.all(out[:, na:] == -7.0)  # untouched beyond na
+
+
+def test_pair_dist_f64_bitexact(ya, oracle):
+    lib = ya.lib()
+    rng = np.random.default_rng(12)
+    b = rng.standard_normal((400, 33)).astype(np.float32)
+    q = rng.standard_normal((50, 33)).astype(np.float32)
+    idx = rng.integers(-1, 400, (50, 6)).astype(np.int32)
+    ld = lib.yk_ld(33)
+    out = np.empty((50, 6), np.float64)
+    ptrs = {}
+    vp = ctypes.c_void_p
+    for name, arr in (("b", b), ("q", q)):
+        p = vp()
+        ya.check(lib.yk_malloc(ctypes.byref(p), arr.shape[0] * ld * 4))
+        ya.check(lib.yk_upload_vectors(p, arr.ctypes.data_as(vp), arr.shape[0], 33))
+        ptrs[name] = p
+    for name, nbytes in (("i", idx.nbytes), ("o", out.nbytes)):
+        p = vp()
+        ya.check(lib.yk_malloc(ctypes.byref(p), nbytes))
+        ptrs[name] = p
+    ya.check(lib.yk_memcpy_h2d(ptrs["i"], idx.ctypes.data_as(vp), idx.nbytes))
+    ya.check(lib.yk_pair_dist_f64(2, 50, 33, 6, ptrs["q"], ptrs["b"], ptrs["i"], ptrs["o"]))
+    ya.check(lib.yk_memcpy_d2h(out.ctypes.data_as(vp), ptrs["o"], out.nbytes))
+    for p in ptrs.values():
+        ya.check(lib.yk_free(p))
+    ref = oracle.pair_dist_f64(2, b, q, idx)
+    assert np.array_equal(np.isnan(out), np.isnan(ref))
+    assert np.array_equal(out[~np.isnan(out)], ref[~np.isnan(ref)])

@@ -1,0 +1,44 @@
+/* yael_host.h -- internal declarations of the C host layer. */
+#ifndef YAEL_HOST_H_INCLUDED
+#define YAEL_HOST_H_INCLUDED
+
+#include <stddef.h>
+
+/* device buffer roles */
+enum {
+  YH_B = 0, YH_Q, YH_BN, YH_QN, YH_W, YH_OUTI, YH_OUTD,
+  YH_KM_V, YH_KM_QN, YH_KM_C, YH_KM_CN, YH_KM_A, YH_KM_D, YH_KM_CNT, YH_KM_WS,
+  YH_KM_BEST_C,
+  YH_NSLOTS
+};
+
+void yh_lock(void);
+void yh_unlock(void);
+void yh_die(const char *where);
+void *yh_buf(int slot, size_t bytes);
+void yh_release_all(void);
+
+int yh_knn(int type, int nq, int nb, int d, int k, const float *b, const float *q,
+           const float *w, int *assign, float *dis);
+int yh_cross(int type, int d, int na, int nb, const float *a, int lda, const float *b,
+             int ldb, float *dist2, int ldd);
+
+int yh_kmeans(int d, int n, int k, int niter, const float *v, int flags, long seed_in,
+              int redo, float *cent_out, float *dis_out, int *assign_out, int *nassign_out,
+              float *ret);
+
+/* kmeans flags (kmeans.h:11-18) */
+#define KMEANS_QUIET 0x10000
+#define KMEANS_INIT_BERKELEY 0x20000
+#define KMEANS_NORMALIZE_CENTS 0x40000
+#define KMEANS_INIT_RANDOM 0x80000
+#define KMEANS_INIT_USER 0x100000
+#define KMEANS_L1 0x200000
+#define KMEANS_CHI2 0x400000
+
+/* empty-cluster split (kmeans.c:166-209) on host copies of the centroids */
+int yh_reassign_empty(int d, int k, float *cent, long ldc, const int *nassign, unsigned seed);
+/* first k entries of the partial Fisher-Yates of kmeans.c:15-20 */
+void yh_random_init(int n, int k, int *sel, unsigned seed);
+
+#endif

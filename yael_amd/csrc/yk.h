@@ -1,0 +1,112 @@
+/*
+ * yk.h -- C-ABI between the C host layer (the host/ sources) and the HIP layer
+ * (*.hip).  Plain pointers and sizes only.  Every function returns 0 on
+ * success or a negative YK_E* code; yk_last_error() gives the message.
+ *
+ * Device layout ("padded fvec"): a set of n vectors of dimension d is stored
+ * row-major with a row stride ld = yk_ld(d) floats (d rounded up to 16), the
+ * pad columns zero.  Row i, component j sits at x[i*ld + j].  This is yael's
+ * column-major d x n fvec with a padded leading dimension.
+ */
+#ifndef YK_H_INCLUDED
+#define YK_H_INCLUDED
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  YK_OK = 0,
+  YK_EINVAL = -1,   /* bad argument (size, type, k > limit) */
+  YK_ENOMEM = -2,   /* device allocation failed */
+  YK_EHIP = -3,     /* HIP runtime error */
+  YK_ENODEV = -4,   /* no usable GPU */
+  YK_EABANDON = -5  /* k-means gave up after too many empty-cluster splits */
+};
+
+/* Distance types served on the GPU (nn.h:165-171): 2 and 12 squared L2,
+ * 16 dot product (min-k). */
+#define YK_L2 2
+#define YK_DOT 16
+
+/* Largest k the fused selection keeps on chip. */
+#define YK_KMAX 1024
+
+const char *yk_last_error(void);
+/* set the message returned by yk_last_error; returns code */
+int yk_set_error_c(int code, const char *fmt, ...);
+int yk_init(void);
+int yk_ld(int d);
+
+/* device memory and copies (the library's own stream) */
+int yk_malloc(void **p, size_t bytes);
+int yk_free(void *p);
+int yk_upload_vectors(float *dst, const float *src, long n, int d);  /* host n x d -> padded */
+int yk_download_vectors(float *dst, const float *src, long n, int d); /* padded -> host n x d */
+int yk_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int yk_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int yk_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int yk_memset(void *dst, int value, size_t bytes);
+int yk_sync(void);
+
+/* Norm kernel: out_f[i] = float-accumulated ||x_i||^2 (nn.c:108-114) when
+ * out_f != NULL; out_d[i] = double-accumulated sum of float squares
+ * (nn.c:116-120) when out_d != NULL. */
+int yk_norms(const float *x, long n, int d, float *out_f, double *out_d);
+
+/* Fused distance + k-smallest selection, FP32 on MFMA, device pointers.
+ * q: nq padded queries with double norms qn; b: nb padded base vectors with
+ * float norms bn (ignored for type 16); w: optional base weights.
+ * Output row-major nq x k: assign (int32) and dis (float32), sorted.      */
+int yk_knn_f32(int type, long nq, long nb, int d, int k, const float *q,
+               const double *qn, const float *b, const float *bn, const float *w,
+               int *assign, float *dis);
+
+/* Cross distances, dist2[i + ldd*j] for a (na, float norms) x b (nb, double
+ * norms), types 2/12/16 (nn.c:92-129, 266-306). */
+int yk_cross_distances(int type, long na, long nb, int d, const float *a,
+                       const float *an, const float *b, const double *bnd,
+                       float *dist2, long ldd);
+
+/* FP64 pair distances for the error report: out[i*k+r] = exact distance of
+ * query i to base assign[i*k+r] (NaN where assign < 0). */
+int yk_pair_dist_f64(int type, long nq, int d, int k, const float *q,
+                     const float *b, const int *assign, double *out);
+
+/* Shortlist re-rank (nn.c:528-580) in place: idx/dis are n x k device arrays,
+ * b and v padded device vectors. */
+int yk_rerank_shortlist(long n, int d, int k, const float *b, const float *v, int *idx,
+                        float *dis);
+
+/* k-means device steps (kmeans.c:242-310).
+ * update: counts[k], centroids[k x ld] = (float)(FP64 sum / count) in point
+ * order of each cluster; FP64 objective of dis.  ws: workspace from
+ * yk_kmeans_workspace_size. */
+size_t yk_kmeans_workspace_size(long n, int k, int d);
+int yk_kmeans_update(long n, int k, int d, const float *v, const int *assign,
+                     float *cent, int *counts, int normalize, void *ws);
+int yk_sum_f64(const float *x, long n, double *out_host, void *ws);
+
+/* Timing of the library's stream (HIP events recorded on it). */
+int yk_timer_start(void);
+int yk_timer_stop(float *ms);
+/* Per-kernel timing on the library stream: when enabled, every launch of
+ * the fused distance kernel is bracketed by events; totals are returned. */
+int yk_prof_enable(int on);
+int yk_prof_get(int which, double *total_ms, long *count);
+int yk_prof_reset(void);
+
+/* Descriptive strings (device name, build) */
+int yk_device_info(char *buf, int len);
+
+/* Test hook: the chosen launch configuration for a kNN shape. */
+int yk_knn_plan(long nq, long nb, int d, int k, int *nsplit, int *waves,
+                int *nblk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,50 @@
+// yk_dev.h -- internal helpers shared by the HIP translation units.
+#ifndef YK_DEV_H_INCLUDED
+#define YK_DEV_H_INCLUDED
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "yk.h"
+
+int yk_set_error(int code, const char *fmt, ...);
+int yk_hip_fail(hipError_t e, const char *what);
+hipStream_t yk_stream(void);
+int yk_prof_setup(void);
+int yk_prof_begin(int slot);
+void yk_prof_end(int tok);
+
+// profiling slots
+enum { YK_PROF_FUSED = 0, YK_PROF_NORM = 1, YK_PROF_UPDATE = 2, YK_PROF_MERGE = 3,
+       YK_PROF_RERANK = 4, YK_PROF_BATCH = 5 };
+
+#define YK_TRY(x)                 \
+  do {                            \
+    int _rc = (x);                \
+    if (_rc != YK_OK) return _rc; \
+  } while (0)
+
+#define YK_HIP(x)                                    \
+  do {                                               \
+    hipError_t _e = (x);                             \
+    if (_e != hipSuccess) return yk_hip_fail(_e, #x); \
+  } while (0)
+
+// Order-preserving 32-bit image of a float (after -0 -> +0); NaN is never
+// passed in.  Ascending key <=> ascending float.
+__device__ __forceinline__ uint32_t yk_fkey(float v) {
+  uint32_t u = __float_as_uint(v + 0.0f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float yk_fkey_inv(uint32_t k) {
+  uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+  return __uint_as_float(u);
+}
+// (value, index) packed so that unsigned order is lexicographic order.
+__device__ __forceinline__ uint64_t yk_key(float v, int idx) {
+  return ((uint64_t)yk_fkey(v) << 32) | (uint32_t)idx;
+}
+
+#define YK_KEY_NONE 0xffffffffffffffffull
+
+#endif

@@ -1,0 +1,265 @@
+// yk_kmeans.hip -- device side of the k-means iteration (kmeans.c:242-310):
+// cluster histogram, stable cluster-major permutation of the points, FP64
+// segmented centroid sums in point order, scaling by the double reciprocal
+// of the count, optional L2 normalisation, and the FP64 objective.
+//
+// Determinism: the histogram uses integer atomics (order-free); the
+// permutation is a stable LSD radix sort of (cluster, point) pairs, so every
+// cluster sums its points in ascending point order, like the reference's
+// loop (kmeans.c:280-283), only in FP64 instead of FP32; the objective is a
+// fixed-shape tree reduction.  Two runs on the same inputs give the same bits.
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "yk.h"
+#include "yk_dev.h"
+
+#define YK_OBJ_BLOCKS 1024
+
+struct KmWs {
+  int *offsets;    // k + 1
+  int *keys_out;   // n
+  int *perm;       // n
+  int *iota;       // n
+  double *obj;     // YK_OBJ_BLOCKS + 1
+  int *err;        // 1
+  void *sort_tmp;  // rocprim temp
+  size_t sort_bytes;
+};
+
+static size_t yk_align(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static int yk_bits_for(int k) {
+  int b = 1;
+  while (b < 31 && (1L << b) < k) b++;
+  return b;
+}
+
+static size_t yk_sort_tmp_bytes(long n, int k) {
+  size_t bytes = 0;
+  (void)rocprim::radix_sort_pairs((void *)nullptr, bytes, (int *)nullptr, (int *)nullptr,
+                            (int *)nullptr, (int *)nullptr, (size_t)n, 0, yk_bits_for(k));
+  return bytes;
+}
+
+static KmWs yk_ws_layout(void *base, long n, int k) {
+  KmWs w;
+  char *p = (char *)base;
+  w.offsets = (int *)p;
+  p += yk_align(sizeof(int) * (k + 1));
+  w.keys_out = (int *)p;
+  p += yk_align(sizeof(int) * n);
+  w.perm = (int *)p;
+  p += yk_align(sizeof(int) * n);
+  w.iota = (int *)p;
+  p += yk_align(sizeof(int) * n);
+  w.obj = (double *)p;
+  p += yk_align(sizeof(double) * (YK_OBJ_BLOCKS + 1));
+  w.err = (int *)p;
+  p += 256;
+  w.sort_bytes = yk_sort_tmp_bytes(n, k);
+  w.sort_tmp = p;
+  return w;
+}
+
+extern "C" size_t yk_kmeans_workspace_size(long n, int k, int d) {
+  (void)d;
+  return yk_align(sizeof(int) * (k + 1)) + 3 * yk_align(sizeof(int) * n) +
+         yk_align(sizeof(double) * (YK_OBJ_BLOCKS + 1)) + 256 + yk_align(yk_sort_tmp_bytes(n, k));
+}
+
+__global__ void yk_iota_kernel(int *p, long n) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (int)i;
+}
+
+__global__ void yk_zero_i32_kernel(int *p, long n) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void yk_hist_kernel(const int *__restrict__ assign, long n, int k, int *counts,
+                               int *err) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = assign[i];
+  if (c < 0 || c >= k) {
+    atomicOr(err, 1);
+    return;
+  }
+  atomicAdd(&counts[c], 1);
+}
+
+// exclusive scan of counts into offsets[0..k], one workgroup
+__global__ __launch_bounds__(1024) void yk_scan_kernel(const int *__restrict__ counts, int k,
+                                                       int *__restrict__ offsets) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (k + 1023) / 1024;
+  const int lo = t * per, hi = min(k, lo + per);
+  int s = 0;
+  for (int i = lo; i < hi; i++) s += counts[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = t == 0 ? 0 : part[t - 1];
+  for (int i = lo; i < hi; i++) {
+    offsets[i] = run;
+    run += counts[i];
+  }
+  if (t == 1023) offsets[k] = part[1023];
+}
+
+// One wave per cluster: FP64 sums over the cluster's points in ascending
+// point order, then centroid = (float)(sum * (1.0 / count)) (kmeans.c:286-288;
+// count 0 gives 0 * inf = NaN as in the reference).  normalize: L2 norm with
+// float squares summed in double, then c *= 1/nr (vector.c:2016-2025).
+__global__ __launch_bounds__(256) void yk_segsum_kernel(const float *__restrict__ v, int d,
+                                                        int ld, int k,
+                                                        const int *__restrict__ offsets,
+                                                        const int *__restrict__ perm,
+                                                        float *__restrict__ cent,
+                                                        int normalize) {
+  __shared__ float row[4][1024 + 16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long c = (long)blockIdx.x * 4 + wave;
+  if (c >= k) return;
+  const int beg = offsets[c], end = offsets[c + 1];
+  const double inv = 1.0 / (double)(end - beg);
+  for (int j0 = 0; j0 < d; j0 += 128) {
+    const int ja = j0 + 2 * lane;
+    double s0 = 0.0, s1 = 0.0;
+    int p = beg;
+    for (; p + 4 <= end; p += 4) {
+      long i0 = perm[p], i1 = perm[p + 1], i2 = perm[p + 2], i3 = perm[p + 3];
+      if (ja < d) {
+        float2 x0 = *(const float2 *)(v + i0 * ld + ja);
+        float2 x1 = *(const float2 *)(v + i1 * ld + ja);
+        float2 x2 = *(const float2 *)(v + i2 * ld + ja);
+        float2 x3 = *(const float2 *)(v + i3 * ld + ja);
+        s0 += (double)x0.x; s1 += (double)x0.y;
+        s0 += (double)x1.x; s1 += (double)x1.y;
+        s0 += (double)x2.x; s1 += (double)x2.y;
+        s0 += (double)x3.x; s1 += (double)x3.y;
+      }
+    }
+    for (; p < end; p++) {
+      long i0 = perm[p];
+      if (ja < d) {
+        float2 x0 = *(const float2 *)(v + i0 * ld + ja);
+        s0 += (double)x0.x;
+        s1 += (double)x0.y;
+      }
+    }
+    if (ja < d) {
+      float c0 = (float)(s0 * inv), c1 = (float)(s1 * inv);
+      if (normalize) {
+        if (ja < 1024) row[wave][ja] = c0;
+        if (ja + 1 < 1024) row[wave][ja + 1] = c1;
+      }
+      cent[c * ld + ja] = c0;
+      if (ja + 1 < d) cent[c * ld + ja + 1] = c1;
+    }
+  }
+  if (normalize && d <= 1024) {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    double nr = 0.0;
+    if (lane == 0) {
+      for (int j = 0; j < d; j++) {
+        float x = row[wave][j];
+        nr = __dadd_rn(nr, (double)__fmul_rn(x, x));
+      }
+      nr = sqrt(nr);
+    }
+    nr = __shfl(nr, 0);
+    const double sc = 1.0 / nr;
+    for (int j = lane; j < d; j += 64) cent[c * ld + j] = (float)((double)row[wave][j] * sc);
+  }
+}
+
+// FP64 sum of n floats, fixed reduction shape.
+__global__ __launch_bounds__(256) void yk_sum_stage1(const float *__restrict__ x, long n,
+                                                     double *__restrict__ part) {
+  __shared__ double s[256];
+  const long per = (n + gridDim.x - 1) / gridDim.x;
+  const long lo = blockIdx.x * per, hi = min(n, lo + per);
+  double acc = 0.0;
+  for (long i = lo + threadIdx.x; i < hi; i += 256) acc += (double)x[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void yk_sum_stage2(double *part, int nparts) {
+  __shared__ double s[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) acc += part[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[nparts] = s[0];
+}
+
+int yk_sum_f64_async(const float *x, long n, double *part) {
+  hipLaunchKernelGGL(yk_sum_stage1, dim3(YK_OBJ_BLOCKS), dim3(256), 0, yk_stream(), x, n, part);
+  hipLaunchKernelGGL(yk_sum_stage2, dim3(1), dim3(256), 0, yk_stream(), part, YK_OBJ_BLOCKS);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
+extern "C" int yk_sum_f64(const float *x, long n, double *out_host, void *ws) {
+  YK_TRY(yk_init());
+  double *part = (double *)ws;
+  YK_TRY(yk_sum_f64_async(x, n, part));
+  YK_HIP(hipMemcpyAsync(out_host, part + YK_OBJ_BLOCKS, sizeof(double), hipMemcpyDeviceToHost,
+                        yk_stream()));
+  YK_HIP(hipStreamSynchronize(yk_stream()));
+  return YK_OK;
+}
+
+extern "C" int yk_kmeans_update(long n, int k, int d, const float *v, const int *assign,
+                                float *cent, int *counts, int normalize, void *ws) {
+  YK_TRY(yk_init());
+  KmWs w = yk_ws_layout(ws, n, k);
+  hipStream_t st = yk_stream();
+  const int ld = yk_ld(d);
+  int tok = yk_prof_begin(YK_PROF_UPDATE);
+  hipLaunchKernelGGL(yk_zero_i32_kernel, dim3((k + 255) / 256), dim3(256), 0, st, counts, (long)k);
+  hipLaunchKernelGGL(yk_zero_i32_kernel, dim3(1), dim3(64), 0, st, w.err, 1L);
+  hipLaunchKernelGGL(yk_iota_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w.iota,
+                     n);
+  hipLaunchKernelGGL(yk_hist_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, assign,
+                     n, k, counts, w.err);
+  hipLaunchKernelGGL(yk_scan_kernel, dim3(1), dim3(1024), 0, st, counts, k, w.offsets);
+  size_t bytes = w.sort_bytes;
+  YK_HIP(rocprim::radix_sort_pairs(w.sort_tmp, bytes, assign, w.keys_out, w.iota, w.perm,
+                                   (size_t)n, 0, yk_bits_for(k), st));
+  hipLaunchKernelGGL(yk_segsum_kernel, dim3((k + 3) / 4), dim3(256), 0, st, v, d, ld, k,
+                     w.offsets, w.perm, cent, normalize);
+  yk_prof_end(tok);
+  YK_HIP(hipGetLastError());
+  int err = 0;
+  YK_HIP(hipMemcpyAsync(&err, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
+  YK_HIP(hipStreamSynchronize(st));
+  if (err)
+    return yk_set_error(YK_EINVAL,
+                        "a point has no nearest centroid (NaN in the input?), kmeans.c:281");
+  return YK_OK;
+}
+
+// workspace pointer of the objective partials (for callers that keep the
+// objective on the device)
+double *yk_ws_obj(void *ws, long n, int k) { return yk_ws_layout(ws, n, k).obj; }

@@ -1,0 +1,263 @@
+// yk_runtime.hip -- device context of the library: one HIP stream (default
+// priority), error reporting, allocation, padded copies, stream timing.
+#include <hip/hip_runtime.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "yk.h"
+#include "yk_dev.h"
+
+static __thread char g_err[512];
+static hipStream_t g_stream = nullptr;
+static int g_ready = 0;
+static pthread_mutex_t g_init_mu = PTHREAD_MUTEX_INITIALIZER;
+static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
+
+int yk_set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int yk_hip_fail(hipError_t e, const char *what) {
+  return yk_set_error(e == hipErrorOutOfMemory ? YK_ENOMEM : YK_EHIP, "%s: %s", what,
+                      hipGetErrorString(e));
+}
+
+extern "C" const char *yk_last_error(void) { return g_err; }
+
+extern "C" int yk_set_error_c(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+hipStream_t yk_stream(void) { return g_stream; }
+
+extern "C" int yk_init(void) {
+  if (g_ready) return YK_OK;
+  pthread_mutex_lock(&g_init_mu);
+  int rc = YK_OK;
+  if (!g_ready) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1) {
+      rc = yk_set_error(YK_ENODEV, "no GPU visible to HIP (%s)",
+                        e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+    } else if ((e = hipSetDevice(0)) != hipSuccess) {
+      rc = yk_hip_fail(e, "hipSetDevice");
+    } else if ((e = hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking)) != hipSuccess) {
+      rc = yk_hip_fail(e, "hipStreamCreate");
+    } else if ((e = hipEventCreate(&g_t0)) != hipSuccess ||
+               (e = hipEventCreate(&g_t1)) != hipSuccess) {
+      rc = yk_hip_fail(e, "hipEventCreate");
+    } else {
+      rc = yk_prof_setup();
+      if (rc == YK_OK) g_ready = 1;
+    }
+  }
+  pthread_mutex_unlock(&g_init_mu);
+  return rc;
+}
+
+extern "C" int yk_ld(int d) { return (d + 15) & ~15; }
+
+extern "C" int yk_malloc(void **p, size_t bytes) {
+  YK_TRY(yk_init());
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return yk_hip_fail(e, "hipMalloc");
+  return YK_OK;
+}
+
+extern "C" int yk_free(void *p) {
+  if (!p) return YK_OK;
+  hipError_t e = hipFree(p);
+  if (e != hipSuccess) return yk_hip_fail(e, "hipFree");
+  return YK_OK;
+}
+
+extern "C" int yk_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  YK_TRY(yk_init());
+  if (!bytes) return YK_OK;
+  YK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+  YK_HIP(hipStreamSynchronize(g_stream));
+  return YK_OK;
+}
+
+extern "C" int yk_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  YK_TRY(yk_init());
+  if (!bytes) return YK_OK;
+  YK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+  YK_HIP(hipStreamSynchronize(g_stream));
+  return YK_OK;
+}
+
+extern "C" int yk_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  YK_TRY(yk_init());
+  if (!bytes) return YK_OK;
+  YK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+  return YK_OK;
+}
+
+__global__ void yk_fill_u32_kernel(uint32_t *p, uint32_t v, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+// Byte fill done by a kernel (never a memset node), multiple of 4 bytes.
+extern "C" int yk_memset(void *dst, int value, size_t bytes) {
+  YK_TRY(yk_init());
+  if (!bytes) return YK_OK;
+  if (bytes % 4) return yk_set_error(YK_EINVAL, "yk_memset: size %zu not a multiple of 4", bytes);
+  uint32_t b = (uint32_t)(value & 0xff);
+  uint32_t v = b | (b << 8) | (b << 16) | (b << 24);
+  size_t n = bytes / 4;
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(yk_fill_u32_kernel, dim3(blocks), dim3(256), 0, g_stream,
+                     (uint32_t *)dst, v, n);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
+extern "C" int yk_sync(void) {
+  YK_TRY(yk_init());
+  YK_HIP(hipStreamSynchronize(g_stream));
+  return YK_OK;
+}
+
+// host n x d (dense) -> device n x ld (pad columns zeroed)
+extern "C" int yk_upload_vectors(float *dst, const float *src, long n, int d) {
+  YK_TRY(yk_init());
+  if (n <= 0) return YK_OK;
+  int ld = yk_ld(d);
+  if (ld == d) {
+    YK_HIP(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice,
+                          g_stream));
+  } else {
+    YK_TRY(yk_memset(dst, 0, sizeof(float) * (size_t)n * ld));
+    YK_HIP(hipMemcpy2DAsync(dst, sizeof(float) * ld, src, sizeof(float) * d,
+                            sizeof(float) * d, n, hipMemcpyHostToDevice, g_stream));
+  }
+  YK_HIP(hipStreamSynchronize(g_stream));
+  return YK_OK;
+}
+
+extern "C" int yk_download_vectors(float *dst, const float *src, long n, int d) {
+  YK_TRY(yk_init());
+  if (n <= 0) return YK_OK;
+  int ld = yk_ld(d);
+  if (ld == d) {
+    YK_HIP(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)n * d, hipMemcpyDeviceToHost,
+                          g_stream));
+  } else {
+    YK_HIP(hipMemcpy2DAsync(dst, sizeof(float) * d, src, sizeof(float) * ld,
+                            sizeof(float) * d, n, hipMemcpyDeviceToHost, g_stream));
+  }
+  YK_HIP(hipStreamSynchronize(g_stream));
+  return YK_OK;
+}
+
+extern "C" int yk_timer_start(void) {
+  YK_TRY(yk_init());
+  YK_HIP(hipEventRecord(g_t0, g_stream));
+  return YK_OK;
+}
+
+extern "C" int yk_timer_stop(float *ms) {
+  YK_HIP(hipEventRecord(g_t1, g_stream));
+  YK_HIP(hipEventSynchronize(g_t1));
+  YK_HIP(hipEventElapsedTime(ms, g_t0, g_t1));
+  return YK_OK;
+}
+
+extern "C" int yk_device_info(char *buf, int len) {
+  YK_TRY(yk_init());
+  hipDeviceProp_t p;
+  YK_HIP(hipGetDeviceProperties(&p, 0));
+  snprintf(buf, len, "%s gcn=%s cus=%d lds/block=%zu clock_khz=%d mem=%zuMiB", p.name,
+           p.gcnArchName, p.multiProcessorCount, (size_t)p.sharedMemPerBlock, p.clockRate,
+           (size_t)(p.totalGlobalMem >> 20));
+  return YK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-kernel event timing (used by the benchmark to time the dominant kernel
+// on the stream it runs on)
+
+#define YK_PROF_SLOTS 8
+#define YK_PROF_EV 4096
+static int g_prof_on = 0;
+static hipEvent_t g_pev[YK_PROF_EV][2];
+static int g_pev_n = 0;
+static int g_pev_slot[YK_PROF_EV];
+static double g_prof_ms[YK_PROF_SLOTS];
+static long g_prof_cnt[YK_PROF_SLOTS];
+
+int yk_prof_setup(void) {
+  for (int i = 0; i < YK_PROF_EV; i++) {
+    hipError_t e = hipEventCreate(&g_pev[i][0]);
+    if (e == hipSuccess) e = hipEventCreate(&g_pev[i][1]);
+    if (e != hipSuccess) return yk_hip_fail(e, "hipEventCreate(prof)");
+  }
+  return YK_OK;
+}
+
+static void yk_prof_flush(void) {
+  for (int i = 0; i < g_pev_n; i++) {
+    float ms = 0;
+    hipEventSynchronize(g_pev[i][1]);
+    if (hipEventElapsedTime(&ms, g_pev[i][0], g_pev[i][1]) == hipSuccess) {
+      g_prof_ms[g_pev_slot[i]] += ms;
+      g_prof_cnt[g_pev_slot[i]]++;
+    }
+  }
+  g_pev_n = 0;
+}
+
+int yk_prof_begin(int slot) {
+  if (!g_prof_on) return -1;
+  if (g_pev_n == YK_PROF_EV) yk_prof_flush();
+  int i = g_pev_n++;
+  g_pev_slot[i] = slot;
+  hipEventRecord(g_pev[i][0], g_stream);
+  return i;
+}
+
+void yk_prof_end(int tok) {
+  if (tok < 0) return;
+  hipEventRecord(g_pev[tok][1], g_stream);
+}
+
+extern "C" int yk_prof_enable(int on) {
+  YK_TRY(yk_init());
+  g_prof_on = on;
+  return YK_OK;
+}
+
+extern "C" int yk_prof_reset(void) {
+  YK_TRY(yk_init());
+  YK_HIP(hipStreamSynchronize(g_stream));
+  g_pev_n = 0;
+  memset(g_prof_ms, 0, sizeof(g_prof_ms));
+  memset(g_prof_cnt, 0, sizeof(g_prof_cnt));
+  return YK_OK;
+}
+
+extern "C" int yk_prof_get(int which, double *total_ms, long *count) {
+  if (which < 0 || which >= YK_PROF_SLOTS) return yk_set_error(YK_EINVAL, "prof slot");
+  YK_HIP(hipStreamSynchronize(g_stream));
+  yk_prof_flush();
+  *total_ms = g_prof_ms[which];
+  *count = g_prof_cnt[which];
+  return YK_OK;
+}
