@@ -94,6 +94,13 @@ int yael_cross_distances_checked(int distance_type, int d, int na, int nb, const
 int yael_kmeans_checked(int d, int n, int k, int niter, const float *v, int flags, long seed,
                         int redo, float *centroids, float *dis, int *assign, int *nassign,
                         float *ret);
+/* kmeans() on vectors resident in HBM (v_dev: n rows of stride yk_ld(d) floats,
+ * pad columns zero); outputs are host arrays as for kmeans().  No
+ * counterpart in the reference: for callers that keep the training set on
+ * the device across calls. */
+int yael_kmeans_device(int d, int n, int k, int niter, const float *v_dev, int flags, long seed,
+                       int redo, float *centroids, float *dis, int *assign, int *nassign,
+                       float *ret);
 int yael_knn_reorder_shortlist_checked(int n, int nb, int d, int k, const float *b,
                                        const float *v, int *idx, float *dis);
 

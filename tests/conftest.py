@@ -9,7 +9,7 @@ for p in (ROOT, HERE):
 
 # the compiled reference links OpenBLAS: keep it single-threaded inside the
 # OpenMP workers, as yael's README recommends for its BLAS
-os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+os.environ["OPENBLAS_NUM_THREADS"] = "1"
 
 
 def pytest_configure(config):

@@ -91,6 +91,17 @@ def kmeans_cases(ref, rng):
     r, c, dis, asg, nas = ref.kmeans(vd, 60, 6, flags=QUIET, seed=11)
     out["km_split"] = dict(v=vd, k=np.int32(60), niter=np.int32(6), flags=np.int32(QUIET), seed=np.int64(11),
                            redo=np.int32(1), ret=np.float32(r), cent=c, dis=dis, assign=asg, nassign=nas)
+    # one step on integer data with 10 unreachable initial centroids: ten
+    # empty clusters split in that step (all arithmetic before the split exact)
+    cen = rng.uniform(0, 255, (40, 16))
+    vi = np.clip(np.rint(cen[rng.integers(0, 40, 3000)] + 20 * rng.standard_normal((3000, 16))), 0, 255)
+    vi = vi.astype(np.float32)
+    init = np.concatenate([vi[rng.choice(3000, 30, replace=False)],
+                           np.full((10, 16), 10000, np.float32) + np.arange(10, dtype=np.float32)[:, None]])
+    r, c, dis, asg, nas = ref.kmeans(vi, 40, 1, flags=QUIET | INIT_USER, seed=13, init=init)
+    out["km_split_step"] = dict(v=vi, k=np.int32(40), niter=np.int32(1), flags=np.int32(QUIET | INIT_USER),
+                                seed=np.int64(13), redo=np.int32(1), init=init, ret=np.float32(r), cent=c,
+                                dis=dis, assign=asg, nassign=nas)
     # one user-initialised step (the per-iteration parity lever)
     init = v[rng.choice(2000, 20, replace=False)].copy()
     r, c, dis, asg, nas = ref.kmeans(v, 20, 1, flags=QUIET | INIT_USER, seed=4, init=init)
@@ -99,12 +110,13 @@ def kmeans_cases(ref, rng):
                           assign=asg, nassign=nas)
     # a cluster above 46340 points: nassign^2 - 1 wraps in 32-bit int
     vb = np.zeros((100000, 4), np.float32)
-    vb[99000:] = 100 + rng.standard_normal((1000, 4)).astype(np.float32)
+    # quarter-integer values: every sum below is exact in FP32 and FP64
+    vb[99000:] = 100 + np.rint(4 * rng.standard_normal((1000, 4))).astype(np.float32) / 4
     vb[:99000] += rng.integers(0, 2, (99000, 4)).astype(np.float32) * 0.25
     init = np.array([[0, 0, 0, 0], [100, 100, 100, 100], [1000, 1000, 1000, 1000],
                      [2000, 0, 0, 0]], np.float32)
-    r, c, dis, asg, nas = ref.kmeans(vb, 4, 3, flags=QUIET | INIT_USER, seed=21, init=init)
-    out["km_overflow"] = dict(v=vb, k=np.int32(4), niter=np.int32(3), flags=np.int32(QUIET | INIT_USER),
+    r, c, dis, asg, nas = ref.kmeans(vb, 4, 1, flags=QUIET | INIT_USER, seed=21, init=init)
+    out["km_overflow"] = dict(v=vb, k=np.int32(4), niter=np.int32(1), flags=np.int32(QUIET | INIT_USER),
                               seed=np.int64(21), redo=np.int32(1), init=init, ret=np.float32(r), cent=c,
                               dis=dis, assign=asg, nassign=nas)
     # abandon: far more clusters than distinct points

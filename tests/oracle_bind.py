@@ -163,8 +163,20 @@ class Oracle(_Api):
         return out
 
 
+def _openblas_single_thread():
+    """Run OpenBLAS single-threaded inside yael's OpenMP workers (yael's README
+    recommends a sequential BLAS); the environment of a GPU host may ask for
+    more threads, which deadlocks a pthreads OpenBLAS called from OpenMP."""
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    path = openblas_path()
+    if path:
+        lib = ctypes.CDLL(path)
+        lib.scipy_openblas_set_num_threads(1)
+
+
 class Reference(_Api):
     def __init__(self):
+        _openblas_single_thread()
         super().__init__(REF_SO, "")
 
 
@@ -180,6 +192,7 @@ def openblas_entry_points():
     path = openblas_path()
     if path is None:
         return None
+    _openblas_single_thread()
     lib = ctypes.CDLL(path)
     g = ctypes.cast(lib.scipy_sgemm_, ctypes.c_void_p).value
     v = ctypes.cast(lib.scipy_sgemv_, ctypes.c_void_p).value

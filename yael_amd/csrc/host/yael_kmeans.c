@@ -101,14 +101,15 @@ void yh_random_init(int n, int k, int *sel, unsigned seed) {
 typedef struct {
   float *v, *c, *cn, *dis;
   double *qn, *obj;
-  int *assign, *counts;
+  int *assign, *counts, *sel;
   void *ws;
 } yh_km_dev;
 
-static int yh_km_alloc(yh_km_dev *m, long n, int k, int d) {
+static int yh_km_alloc(yh_km_dev *m, long n, int k, int d, int v_on_device) {
   int ld = yk_ld(d);
   long kp = (k + 31) & ~31;
-  m->v = (float *)yh_buf(YH_KM_V, sizeof(float) * (size_t)n * ld);
+  m->v = v_on_device ? (float *)1 : (float *)yh_buf(YH_KM_V, sizeof(float) * (size_t)n * ld);
+  m->sel = (int *)yh_buf(YH_KM_SEL, sizeof(int) * kp);
   m->qn = (double *)yh_buf(YH_KM_QN, sizeof(double) * n);
   m->c = (float *)yh_buf(YH_KM_C, sizeof(float) * (size_t)kp * ld);
   m->cn = (float *)yh_buf(YH_KM_CN, sizeof(float) * kp);
@@ -116,7 +117,8 @@ static int yh_km_alloc(yh_km_dev *m, long n, int k, int d) {
   m->dis = (float *)yh_buf(YH_KM_D, sizeof(float) * n);
   m->counts = (int *)yh_buf(YH_KM_CNT, sizeof(int) * kp + sizeof(double) * 1040);
   m->ws = yh_buf(YH_KM_WS, yk_kmeans_workspace_size(n, k, d));
-  if (!m->v || !m->qn || !m->c || !m->cn || !m->assign || !m->dis || !m->counts || !m->ws)
+  if (!m->v || !m->sel || !m->qn || !m->c || !m->cn || !m->assign || !m->dis || !m->counts ||
+      !m->ws)
     return YK_ENOMEM;
   m->obj = (double *)((char *)m->counts + sizeof(int) * kp);
   return YK_OK;
@@ -168,9 +170,9 @@ static int yh_kmeans_core(int d, int n, int k, int niter, int flags, int verbose
   return YK_OK;
 }
 
-int yh_kmeans(int d, int n, int k, int niter, const float *v, int flags, long seed_in,
-              int redo, float *cent_out, float *dis_out, int *assign_out, int *nassign_out,
-              float *ret) {
+int yh_kmeans(int d, int n, int k, int niter, const float *v, int v_on_device, int flags,
+              long seed_in, int redo, float *cent_out, float *dis_out, int *assign_out,
+              int *nassign_out, float *ret) {
   const int verbose = !(flags & KMEANS_QUIET);
   niter = niter == 0 ? 1000000 : niter;
   const int user = (flags & KMEANS_INIT_USER) != 0;
@@ -186,16 +188,18 @@ int yh_kmeans(int d, int n, int k, int niter, const float *v, int flags, long se
   if (rc != YK_OK) return rc;
   const int ld = yk_ld(d);
   yh_km_dev m;
-  if ((rc = yh_km_alloc(&m, n, k, d)) != YK_OK) return rc;
-  if ((rc = yk_upload_vectors(m.v, v, n, d)) != YK_OK) return rc;
+  if ((rc = yh_km_alloc(&m, n, k, d, v_on_device)) != YK_OK) return rc;
+  if (v_on_device)
+    m.v = (float *)v; /* caller's padded device vectors */
+  else if ((rc = yk_upload_vectors(m.v, v, n, d)) != YK_OK)
+    return rc;
   if ((rc = yk_norms(m.v, n, d, NULL, m.qn)) != YK_OK) return rc;
 
   int *hcounts = (int *)malloc(sizeof(int) * k);
   float *hcent = (float *)malloc(sizeof(float) * (size_t)k * ld);
-  float *hinit = (float *)malloc(sizeof(float) * (size_t)k * d);
   int *sel = (int *)malloc(sizeof(int) * k);
-  if (!hcounts || !hcent || !hinit || !sel) {
-    free(hcounts); free(hcent); free(hinit); free(sel);
+  if (!hcounts || !hcent || !sel) {
+    free(hcounts); free(hcent); free(sel);
     return yk_set_error_c(YK_ENOMEM, "host malloc");
   }
   if (seed_in == 0) seed_in = lrand48(); /* kmeans.c:379-380 */
@@ -206,12 +210,13 @@ int yh_kmeans(int d, int n, int k, int niter, const float *v, int flags, long se
   for (int run = 0; run < redo; run++) {
     if (verbose) printf("<><><><> kmeans / run %d <><><><><>\n", run);
     if (user) {
-      memcpy(hinit, cent_out, sizeof(float) * (size_t)k * d);
+      if ((rc = yk_upload_vectors(m.c, cent_out, k, d)) != YK_OK) break;
     } else {
+      /* the selected points are copied row by row on the device */
       yh_random_init(n, k, sel, (unsigned)rand_r(&seed));
-      for (long i = 0; i < k; i++) memcpy(hinit + i * d, v + (long)sel[i] * d, sizeof(float) * d);
+      if ((rc = yk_memcpy_h2d(m.sel, sel, sizeof(int) * k)) != YK_OK) break;
+      if ((rc = yk_gather_rows(m.v, m.sel, k, d, m.c)) != YK_OK) break;
     }
-    if ((rc = yk_upload_vectors(m.c, hinit, k, d)) != YK_OK) break;
     core = yh_kmeans_core(d, n, k, niter, flags, verbose, &m, (unsigned)rand_r(&seed), hcounts,
                           hcent, &qerr, &iter_tot);
     if (core != YK_OK) break;
@@ -228,7 +233,6 @@ int yh_kmeans(int d, int n, int k, int niter, const float *v, int flags, long se
     printf("Total number of iterations: %ld\n", iter_tot);
   }
   free(sel);
-  free(hinit);
   free(hcent);
   free(hcounts);
   if (rc != YK_OK) return rc;
@@ -246,7 +250,8 @@ float kmeans(int d, int n, int k, int niter, const float *v, int flags, long see
              float *centroids, float *dis, int *assign, int *nassign) {
   float ret = -1;
   yh_lock();
-  int rc = yh_kmeans(d, n, k, niter, v, flags, seed, redo, centroids, dis, assign, nassign, &ret);
+  int rc = yh_kmeans(d, n, k, niter, v, 0, flags, seed, redo, centroids, dis, assign, nassign,
+                     &ret);
   yh_unlock();
   if (rc != YK_OK) yh_die("kmeans");
   return ret;
@@ -298,7 +303,20 @@ int yael_kmeans_checked(int d, int n, int k, int niter, const float *v, int flag
                         int redo, float *centroids, float *dis, int *assign, int *nassign,
                         float *ret) {
   yh_lock();
-  int rc = yh_kmeans(d, n, k, niter, v, flags, seed, redo, centroids, dis, assign, nassign, ret);
+  int rc = yh_kmeans(d, n, k, niter, v, 0, flags, seed, redo, centroids, dis, assign, nassign, ret);
+  yh_unlock();
+  return rc;
+}
+
+/* kmeans() on vectors already resident in HBM: v_dev holds n padded rows
+ * (row stride yk_ld(d) floats, pad columns zero).  Outputs are host arrays
+ * (any may be NULL), as for kmeans(). */
+int yael_kmeans_device(int d, int n, int k, int niter, const float *v_dev, int flags, long seed,
+                       int redo, float *centroids, float *dis, int *assign, int *nassign,
+                       float *ret) {
+  yh_lock();
+  int rc = yh_kmeans(d, n, k, niter, v_dev, 1, flags, seed, redo, centroids, dis, assign, nassign,
+                     ret);
   yh_unlock();
   return rc;
 }

@@ -89,6 +89,8 @@ size_t yk_kmeans_workspace_size(long n, int k, int d);
 int yk_kmeans_update(long n, int k, int d, const float *v, const int *assign,
                      float *cent, int *counts, int normalize, void *ws);
 int yk_sum_f64(const float *x, long n, double *out_host, void *ws);
+/* dst row i = src row idx[i] (padded rows; idx is a device array) */
+int yk_gather_rows(const float *src, const int *idx, long n, int d, float *dst);
 
 /* Timing of the library's stream (HIP events recorded on it). */
 int yk_timer_start(void);

@@ -263,3 +263,23 @@ extern "C" int yk_kmeans_update(long n, int k, int d, const float *v, const int 
 // workspace pointer of the objective partials (for callers that keep the
 // objective on the device)
 double *yk_ws_obj(void *ws, long n, int k) { return yk_ws_layout(ws, n, k).obj; }
+
+// dst[i] = src[idx[i]] for padded rows (initial centroids from selected points)
+__global__ void yk_gather_kernel(const float *__restrict__ src, const int *__restrict__ idx,
+                                 long n, int ld, float *__restrict__ dst) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long row = t / (ld / 4), c4 = t % (ld / 4);
+  if (row >= n) return;
+  ((float4 *)(dst + row * ld))[c4] = ((const float4 *)(src + (long)idx[row] * ld))[c4];
+}
+
+extern "C" int yk_gather_rows(const float *src, const int *idx, long n, int d, float *dst) {
+  YK_TRY(yk_init());
+  if (n <= 0) return YK_OK;
+  const int ld = yk_ld(d);
+  long total = n * (ld / 4);
+  hipLaunchKernelGGL(yk_gather_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     yk_stream(), src, idx, n, ld, dst);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}

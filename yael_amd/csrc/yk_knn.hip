@@ -95,500 +95,6 @@ extern "C" int yk_norms(const float *x, long n, int d, float *out_f, double *out
 }
 
 // ---------------------------------------------------------------------------
-// fused distance + selection
-
-struct FusedArgs {
-  const float *q;
-  const double *qn;
-  long nq;
-  const float *b;
-  const float *bn;  // >= round_up(nb, 32) entries
-  const float *w;   // optional, >= round_up(nb, 32) entries
-  long nb;
-  int ld;
-  int type;
-  long brange;  // base rows per split, multiple of 32
-  int k;
-  int nsplit;
-  int tiles_x;  // query tiles
-  int total;    // query tiles * splits
-  int per_xcd;  // work items per XCD slot
-  int *out_i;
-  float *out_d;
-  uint64_t *part;  // [nq][nsplit][k]
-};
-
-// Work-item index for this workgroup: the hardware hands consecutive block
-// ids to the 8 XCDs in turn; give each XCD a contiguous run of work items
-// (split-major), so workgroups sharing a base range share that XCD's L2.
-__device__ __forceinline__ int yk_xcd_item(int per_xcd) {
-  int id = blockIdx.x;
-  return (id & 7) * per_xcd + (id >> 3);
-}
-
-__device__ __forceinline__ void yk_wave_fence() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
-template <int NB, bool QRES, int MODE>
-__global__ __launch_bounds__(256) void yk_fused_f32(FusedArgs a) {
-  extern __shared__ uint64_t lds_keys[];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  const int r = lane & 31, h = lane >> 5;
-  const int item = yk_xcd_item(a.per_xcd);
-  if (item >= a.total) return;
-  const int tile = item % a.tiles_x, split = item / a.tiles_x;
-  const long qbase = ((long)tile * nwave + wave) * (32 * NB);
-  const long bbeg = (long)split * a.brange;
-  const long bend = min(a.nb, bbeg + a.brange);
-  const int nch = a.ld >> 4;
-  const int ld = a.ld;
-  const bool dot = (a.type == YK_DOT);
-
-  long qrow[NB];
-  double qnd[NB];
-#pragma unroll
-  for (int j = 0; j < NB; j++) {
-    long qi = qbase + j * 32 + r;
-    qrow[j] = qi < a.nq ? qi : a.nq - 1;
-    qnd[j] = dot ? 0.0 : a.qn[qrow[j]];
-  }
-
-  float4 qreg[NB][QRES ? 16 : 1];
-  if (QRES) {
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-#pragma unroll
-      for (int c = 0; c < 8; c++)
-        if (c < nch) {
-          const float *p = a.q + qrow[j] * ld + c * 16 + h * 8;
-          qreg[j][2 * c] = *(const float4 *)p;
-          qreg[j][2 * c + 1] = *(const float4 *)(p + 4);
-        }
-  }
-
-  // selection state
-  float bv[NB];
-  int bi[NB];
-  uint64_t thr[NB];
-  uint64_t *keys = nullptr;
-  int *cnts = nullptr;
-  const int k = a.k;
-#pragma unroll
-  for (int j = 0; j < NB; j++) {
-    bv[j] = 1e30f;
-    bi[j] = -1;
-    thr[j] = YK_KEY_NONE;
-  }
-  if (MODE == 1) {
-    keys = lds_keys + (size_t)wave * (32 * NB) * k;
-    cnts = (int *)(lds_keys + (size_t)nwave * (32 * NB) * k) + wave * (32 * NB);
-    if (lane < 32 * NB) cnts[lane] = 0;
-    yk_wave_fence();
-  }
-
-  for (long bb = bbeg; bb < bend; bb += 32) {
-    const long brow = min(bb + r, a.nb - 1);
-    const float *bp = a.b + brow * ld + h * 8;
-    f32x16 acc[NB];
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-#pragma unroll
-      for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
-
-    if (QRES) {
-      float4 av[16];
-#pragma unroll
-      for (int c = 0; c < 8; c++)
-        if (c < nch) {
-          av[2 * c] = *(const float4 *)(bp + c * 16);
-          av[2 * c + 1] = *(const float4 *)(bp + c * 16 + 4);
-        }
-#pragma unroll
-      for (int c = 0; c < 8; c++)
-        if (c < nch) {
-#pragma unroll
-          for (int j = 0; j < NB; j++) {
-            float4 a0 = av[2 * c], a1 = av[2 * c + 1];
-            float4 b0 = qreg[j][QRES ? 2 * c : 0], b1 = qreg[j][QRES ? 2 * c + 1 : 0];
-            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc[j], 0, 0, 0);
-            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc[j], 0, 0, 0);
-            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc[j], 0, 0, 0);
-            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc[j], 0, 0, 0);
-            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc[j], 0, 0, 0);
-            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc[j], 0, 0, 0);
-            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc[j], 0, 0, 0);
-            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc[j], 0, 0, 0);
-          }
-        }
-    } else {
-      for (int c = 0; c < nch; c++) {
-        float4 a0 = *(const float4 *)(bp + c * 16);
-        float4 a1 = *(const float4 *)(bp + c * 16 + 4);
-#pragma unroll
-        for (int j = 0; j < NB; j++) {
-          const float *qp = a.q + qrow[j] * ld + c * 16 + h * 8;
-          float4 b0 = *(const float4 *)qp, b1 = *(const float4 *)(qp + 4);
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc[j], 0, 0, 0);
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc[j], 0, 0, 0);
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc[j], 0, 0, 0);
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc[j], 0, 0, 0);
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc[j], 0, 0, 0);
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc[j], 0, 0, 0);
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc[j], 0, 0, 0);
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc[j], 0, 0, 0);
-        }
-      }
-    }
-
-    // epilogue: lane holds query column r and base rows 8g + 4h + e
-    float vv[NB][16];
-#pragma unroll
-    for (int g = 0; g < 4; g++) {
-      const long b4 = bb + 8 * g + 4 * h;
-      float4 n4 = dot ? make_float4(0.f, 0.f, 0.f, 0.f) : *(const float4 *)(a.bn + b4);
-      float4 w4 = a.w ? *(const float4 *)(a.w + b4) : make_float4(1.f, 1.f, 1.f, 1.f);
-      float nn[4] = {n4.x, n4.y, n4.z, n4.w};
-      float ww[4] = {w4.x, w4.y, w4.z, w4.w};
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int i = 4 * g + e;
-#pragma unroll
-        for (int j = 0; j < NB; j++) {
-          float v;
-          if (dot)
-            v = acc[j][i];
-          else
-            v = (float)(qnd[j] + (double)nn[e]) - 2.0f * acc[j][i];
-          if (a.w) v *= ww[e];
-          vv[j][i] = v;
-        }
-      }
-    }
-
-    if (MODE == 0) {
-#pragma unroll
-      for (int i = 0; i < 16; i++) {
-        const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
-#pragma unroll
-        for (int j = 0; j < NB; j++)
-          if (bidx < bend && vv[j][i] < bv[j]) {
-            bv[j] = vv[j][i];
-            bi[j] = (int)bidx;
-          }
-      }
-    } else {
-      uint32_t mask[NB];
-      bool any = false;
-#pragma unroll
-      for (int j = 0; j < NB; j++) {
-        mask[j] = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-          const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
-          float v = vv[j][i];
-          if (bidx < bend && v == v && yk_key(v, (int)bidx) < thr[j]) mask[j] |= 1u << i;
-        }
-        any |= mask[j] != 0;
-      }
-      if (__any(any)) {
-        for (int turn = 0; turn < 2; turn++) {
-          if (h == turn) {
-#pragma unroll
-            for (int j = 0; j < NB; j++) {
-              if (!mask[j]) continue;
-              const int ql = j * 32 + r;
-              uint64_t *L = keys + (size_t)ql * k;
-              int cnt = cnts[ql];
-              uint64_t T = thr[j];
-              for (int i = 0; i < 16; i++) {
-                if (!(mask[j] >> i & 1)) continue;
-                const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
-                uint64_t key = yk_key(vv[j][i], (int)bidx);
-                if (key >= T) continue;
-                int pos = cnt < k ? cnt++ : k - 1;
-                while (pos > 0 && L[pos - 1] > key) {
-                  L[pos] = L[pos - 1];
-                  pos--;
-                }
-                L[pos] = key;
-                T = cnt == k ? L[k - 1] : YK_KEY_NONE;
-              }
-              cnts[ql] = cnt;
-            }
-          }
-          yk_wave_fence();
-          // both lanes of each query pick up the new threshold
-#pragma unroll
-          for (int j = 0; j < NB; j++) {
-            const int ql = j * 32 + r;
-            int cnt = cnts[ql];
-            thr[j] = cnt == k ? keys[(size_t)ql * k + k - 1] : YK_KEY_NONE;
-          }
-          yk_wave_fence();
-        }
-      }
-    }
-  }
-
-  // write results
-  if (MODE == 0) {
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      float ov = __shfl_xor(bv[j], 32);
-      int oi = __shfl_xor(bi[j], 32);
-      if (ov < bv[j] || (ov == bv[j] && oi < bi[j])) {
-        bv[j] = ov;
-        bi[j] = oi;
-      }
-      const long qi = qbase + j * 32 + r;
-      if (h == 0 && qi < a.nq) {
-        if (a.nsplit == 1) {
-          a.out_d[qi] = bv[j];
-          a.out_i[qi] = bi[j];
-        } else {
-          a.part[qi * a.nsplit + split] = bi[j] < 0 ? YK_KEY_NONE : yk_key(bv[j], bi[j]);
-        }
-      }
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      const long qi = qbase + j * 32 + r;
-      if (qi >= a.nq) continue;
-      const int ql = j * 32 + r;
-      const int cnt = cnts[ql];
-      const uint64_t *L = keys + (size_t)ql * k;
-      for (int s = h; s < k; s += 2) {
-        uint64_t key = s < cnt ? L[s] : YK_KEY_NONE;
-        if (a.nsplit == 1) {
-          a.out_i[qi * k + s] = key == YK_KEY_NONE ? -1 : (int)(uint32_t)key;
-          a.out_d[qi * k + s] =
-              key == YK_KEY_NONE ? __uint_as_float(0xffffffffu) : yk_fkey_inv((uint32_t)(key >> 32));
-        } else {
-          a.part[(qi * a.nsplit + split) * (long)k + s] = key;
-        }
-      }
-    }
-  }
-}
-
-// k = 1 merge of per-split minima
-__global__ void yk_merge1_kernel(const uint64_t *__restrict__ part, long nq, int nsplit,
-                                 int *out_i, float *out_d) {
-  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (q >= nq) return;
-  uint64_t best = YK_KEY_NONE;
-  for (int s = 0; s < nsplit; s++) {
-    uint64_t v = part[q * nsplit + s];
-    best = v < best ? v : best;
-  }
-  if (best == YK_KEY_NONE) {
-    out_i[q] = -1;
-    out_d[q] = 1e30f;
-  } else {
-    out_i[q] = (int)(uint32_t)best;
-    out_d[q] = yk_fkey_inv((uint32_t)(best >> 32));
-  }
-}
-
-// k > 1 merge: bitonic sort of the nsplit*k keys of one query in LDS.
-__global__ __launch_bounds__(256) void yk_mergek_kernel(const uint64_t *__restrict__ part,
-                                                        long nq, int nsplit, int k, int p2,
-                                                        int *out_i, float *out_d) {
-  extern __shared__ uint64_t s[];
-  const long q = blockIdx.x;
-  const int n = nsplit * k;
-  const uint64_t *src = part + q * (long)n;
-  for (int i = threadIdx.x; i < p2; i += blockDim.x) s[i] = i < n ? src[i] : YK_KEY_NONE;
-  __syncthreads();
-  for (int size = 2; size <= p2; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int t = threadIdx.x; t < p2 / 2; t += blockDim.x) {
-        int lo = 2 * t - (t & (stride - 1));
-        int hi = lo + stride;
-        bool up = (lo & size) == 0;
-        uint64_t x = s[lo], y = s[hi];
-        if ((x > y) == up) {
-          s[lo] = y;
-          s[hi] = x;
-        }
-      }
-      __syncthreads();
-    }
-  }
-  for (int i = threadIdx.x; i < k; i += blockDim.x) {
-    uint64_t key = s[i];
-    out_i[q * k + i] = key == YK_KEY_NONE ? -1 : (int)(uint32_t)key;
-    out_d[q * k + i] =
-        key == YK_KEY_NONE ? __uint_as_float(0xffffffffu) : yk_fkey_inv((uint32_t)(key >> 32));
-  }
-}
-
-// ---------------------------------------------------------------------------
-// planning
-
-struct YkPlan {
-  int nb_blocks;  // NB template value
-  int waves;      // waves per workgroup
-  int nsplit;
-  long brange;
-  int tiles;
-  size_t lds;
-};
-
-static size_t g_lds_max = 0;
-
-static int yk_lds_limit(void) {
-  if (!g_lds_max) {
-    hipDeviceProp_t p;
-    if (hipGetDeviceProperties(&p, 0) != hipSuccess) return 65536;
-    g_lds_max = p.sharedMemPerBlock;
-    if (g_lds_max < 65536) g_lds_max = 65536;
-  }
-  return (int)g_lds_max;
-}
-
-static YkPlan yk_make_plan(long nq, long nb, int d, int k) {
-  YkPlan p;
-  p.nb_blocks = 1;
-  p.waves = 4;
-  const int lds_max = yk_lds_limit();
-  if (k > 1) {
-    size_t per_wave = (size_t)32 * p.nb_blocks * k * 8 + 32 * p.nb_blocks * 4;
-    while (p.waves > 1 && per_wave * p.waves > (size_t)lds_max) p.waves >>= 1;
-    p.lds = per_wave * p.waves;
-  } else {
-    p.lds = 0;
-  }
-  const long qt = 32L * p.nb_blocks * p.waves;
-  p.tiles = (int)((nq + qt - 1) / qt);
-  // enough workgroups for 256 CUs (4 per CU), but at least 512 base rows per
-  // split, and split lists small enough for the merge kernel
-  long want = (1024 + p.tiles - 1) / p.tiles;
-  long maxs = (nb + 511) / 512;
-  if (want > maxs) want = maxs;
-  if (k > 1) {
-    long cap = 4096 / k;
-    if (want > cap) want = cap;
-  }
-  if (want < 1) want = 1;
-  long br = (nb + want - 1) / want;
-  br = (br + 31) & ~31L;
-  p.brange = br;
-  p.nsplit = (int)((nb + br - 1) / br);
-  (void)d;
-  return p;
-}
-
-extern "C" int yk_knn_plan(long nq, long nb, int d, int k, int *nsplit, int *waves,
-                           int *nblk) {
-  YK_TRY(yk_init());
-  YkPlan p = yk_make_plan(nq, nb, d, k);
-  *nsplit = p.nsplit;
-  *waves = p.waves;
-  *nblk = p.nb_blocks;
-  return YK_OK;
-}
-
-// grow-only scratch for split partials
-static void *g_part = nullptr;
-static size_t g_part_bytes = 0;
-
-static int yk_part_buffer(size_t bytes, void **out) {
-  if (bytes > g_part_bytes) {
-    if (g_part) (void)hipFree(g_part);
-    g_part = nullptr;
-    g_part_bytes = 0;
-    hipError_t e = hipMalloc(&g_part, bytes);
-    if (e != hipSuccess) return yk_hip_fail(e, "hipMalloc(partials)");
-    g_part_bytes = bytes;
-  }
-  *out = g_part;
-  return YK_OK;
-}
-
-template <int MODE, bool QRES>
-static int yk_launch_fused(const YkPlan &p, FusedArgs &a) {
-  const int total = p.tiles * p.nsplit;
-  const int per = (total + 7) / 8;
-  a.tiles_x = p.tiles;
-  a.total = total;
-  a.per_xcd = per;
-  auto kern = yk_fused_f32<1, QRES, MODE>;
-  if (p.lds > 65536) {
-    YK_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)p.lds));
-  }
-  int tok = yk_prof_begin(YK_PROF_FUSED);
-  hipLaunchKernelGGL(kern, dim3(per * 8), dim3(64 * p.waves), p.lds, yk_stream(), a);
-  yk_prof_end(tok);
-  YK_HIP(hipGetLastError());
-  return YK_OK;
-}
-
-extern "C" int yk_knn_f32(int type, long nq, long nb, int d, int k, const float *q,
-                          const double *qn, const float *b, const float *bn, const float *w,
-                          int *assign, float *dis) {
-  YK_TRY(yk_init());
-  if (type == 12) type = YK_L2;
-  if (type != YK_L2 && type != YK_DOT)
-    return yk_set_error(YK_EINVAL, "distance type %d has no GPU path (2, 12, 16 only)", type);
-  if (k < 1 || k > nb || k > YK_KMAX)
-    return yk_set_error(YK_EINVAL, "k=%d out of range (nb=%ld, max %d)", k, nb, YK_KMAX);
-  if (nq <= 0) return YK_OK;
-  if (nb > 0x7fffffffL || nq > 0x7fffffffL) return yk_set_error(YK_EINVAL, "sizes exceed int32");
-  YkPlan p = yk_make_plan(nq, nb, d, k);
-  if (k > 1 && p.lds > (size_t)yk_lds_limit())
-    return yk_set_error(YK_EINVAL, "k=%d needs %zu bytes of LDS", k, p.lds);
-  FusedArgs a;
-  a.q = q;
-  a.qn = qn;
-  a.nq = nq;
-  a.b = b;
-  a.bn = bn;
-  a.w = w;
-  a.nb = nb;
-  a.ld = yk_ld(d);
-  a.type = type;
-  a.brange = p.brange;
-  a.k = k;
-  a.nsplit = p.nsplit;
-  a.out_i = assign;
-  a.out_d = dis;
-  a.part = nullptr;
-  if (p.nsplit > 1) {
-    void *buf;
-    YK_TRY(yk_part_buffer(sizeof(uint64_t) * (size_t)nq * p.nsplit * k, &buf));
-    a.part = (uint64_t *)buf;
-  }
-  const bool qres = a.ld <= 128;
-  if (k == 1) {
-    if (qres) YK_TRY((yk_launch_fused<0, true>(p, a)));
-    else YK_TRY((yk_launch_fused<0, false>(p, a)));
-  } else {
-    if (qres) YK_TRY((yk_launch_fused<1, true>(p, a)));
-    else YK_TRY((yk_launch_fused<1, false>(p, a)));
-  }
-  if (p.nsplit > 1) {
-    int tok = yk_prof_begin(YK_PROF_MERGE);
-    if (k == 1) {
-      hipLaunchKernelGGL(yk_merge1_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
-                         yk_stream(), a.part, nq, p.nsplit, assign, dis);
-    } else {
-      int n = p.nsplit * k, p2 = 1;
-      while (p2 < n) p2 <<= 1;
-      hipLaunchKernelGGL(yk_mergek_kernel, dim3((unsigned)nq), dim3(256), sizeof(uint64_t) * p2,
-                         yk_stream(), a.part, nq, p.nsplit, k, p2, assign, dis);
-    }
-    yk_prof_end(tok);
-    YK_HIP(hipGetLastError());
-  }
-  return YK_OK;
-}
-
-// ---------------------------------------------------------------------------
 // cross distances: dist2[i + ldd*j], a = "base" role (float norms, MFMA
 // columns), b = "query" role (double norms, MFMA rows); a wave computes 32
 // rows of b by 64 columns of a, stores are 128-byte rows.
@@ -683,13 +189,12 @@ __global__ void yk_pair_f64_kernel(int type, long nq, int d, int ld, int k,
   }
   const float *x = q + qi * ld, *y = b + (long)j * ld;
   double s = 0.0;
+  // product then sum, each rounded (a fused form would round once, which
+  // differs when (x - y)^2 needs more than 53 bits)
   for (int l = 0; l < d; l++) {
-    if (type == YK_DOT) {
-      s = __fma_rn((double)x[l], (double)y[l], s);
-    } else {
-      double u = (double)x[l] - (double)y[l];
-      s = __fma_rn(u, u, s);
-    }
+    double u = type == YK_DOT ? (double)x[l] : (double)x[l] - (double)y[l];
+    double p = type == YK_DOT ? __dmul_rn(u, (double)y[l]) : __dmul_rn(u, u);
+    s = __dadd_rn(s, p);
   }
   out[t] = s;
 }

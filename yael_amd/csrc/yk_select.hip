@@ -1,0 +1,589 @@
+// yk_select.hip -- fused FP32 distance + k-smallest selection on MFMA.
+//
+// Reference semantics (yael nn.c):
+//   dist(q,b) = float( double(sum_j q_j*q_j) [float products, double sum]
+//                    + double(float-accumulated sum_j b_j*b_j) )  - 2 * q.b
+//   (nn.c:108-126), optionally times w[b] (nn.c:497-500); type 16 is the
+//   plain dot product (nn.c:266-306).  k = 1: strict '<' from (-1, 1e30)
+//   (nn.c:404-440).  k > 1: the k smallest non-NaN values, sorted, padded
+//   with (-1, 0xffffffff) (nn.c:463-525).  Equal values are ordered by base
+//   index (the reference's ascending scan gives the same for k = 1).
+//
+// Every selection works on 64-bit keys (order-preserving float bits << 32 |
+// base index): key order is (value, index) order, so the result does not
+// depend on how the base is split across lanes, waves and workgroups.
+//
+// Scan kernel (gfx950, wave64, v_mfma_f32_32x32x2_f32):
+//   * a workgroup of W waves; each wave owns 32*NB queries (the MFMA
+//     columns, held in registers when d <= 128) and streams its workgroup's
+//     base range in blocks of 32 rows (the MFMA rows) from global memory;
+//   * per 16-wide k-chunk a lane loads 8 contiguous floats (two dwordx4) of
+//     its row: A and B use the same k permutation, each 64-byte row segment
+//     is read once per block;
+//   * the 32x32 accumulator gives each lane one query and 16 base rows, so
+//     the epilogue is lane-local:
+//       MODE 0  running (value, index) minimum           -> k = 1
+//       MODE 2  register top-R of keys (R <= 16)          -> 2 <= k <= 16, and
+//               the threshold sketch for larger k
+//       MODE 3  append every key <= the query's threshold T_q to a per-query
+//               candidate list (global atomics; rare)     -> k > 16
+//   * the base is split over workgroups when the query tiles alone cannot
+//     fill 256 CUs; split results are merged by small sort kernels.
+//
+// k > 16 is exact in two passes: (A) MODE 2 with R = 16 over a strided
+// sample of base blocks gives, per query, 32 * nsplit distinct real keys; their
+// k-th smallest T_q bounds the true k-th smallest key from above.  (B) MODE 3
+// over the whole base collects every key <= T_q, then a per-query sort keeps
+// the k smallest.  A query whose candidates overflow the list gets a tighter
+// T_q (the k-th smallest of the keys it did capture, again an upper bound)
+// and is scanned again.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+
+#include "yk.h"
+#include "yk_dev.h"
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+#define YK_CAND_CAP 4096  // candidate keys kept per query (MODE 3)
+#define YK_REG_R 16       // register top-R of MODE 2
+
+struct ScanArgs {
+  const float *q;
+  const double *qn;
+  long nq;
+  const float *b;
+  const float *bn;  // >= round_up(nb, 32) entries
+  const float *w;   // optional, >= round_up(nb, 32) entries
+  long nb;
+  int ld;
+  int type;
+  long brange;  // base rows per split, multiple of bstep
+  long bstep;   // rows between processed 32-row blocks (32 * sampling stride)
+  int nsplit;
+  int tiles_x;
+  int total;
+  int per_xcd;
+  int *out_i;      // MODE 0, nsplit == 1
+  float *out_d;
+  uint64_t *part;  // MODE 0: [nq][nsplit]; MODE 2: [nq][nsplit][2][R]
+  const uint64_t *thr;  // MODE 3: per-query threshold key (0 = inactive)
+  int *cnt;             // MODE 3: per-query candidate count
+  uint64_t *cand;       // MODE 3: [nq][YK_CAND_CAP]
+};
+
+// The hardware hands consecutive block ids to the 8 XCDs in turn; give each
+// XCD a contiguous run of work items (split-major), so the workgroups that
+// share a base range also share that XCD's L2.
+__device__ __forceinline__ int yk_xcd_item(int per_xcd) {
+  int id = blockIdx.x;
+  return (id & 7) * per_xcd + (id >> 3);
+}
+
+// sorted (ascending) register list: insert x, drop the largest
+template <int R>
+__device__ __forceinline__ void yk_reg_insert(uint64_t (&t)[R], uint64_t x) {
+#pragma unroll
+  for (int i = R - 1; i > 0; i--) {
+    uint64_t lo = t[i - 1], cur = t[i];
+    t[i] = x < lo ? lo : (x < cur ? x : cur);
+  }
+  t[0] = x < t[0] ? x : t[0];
+}
+
+template <int NB, bool QRES, int MODE, int R>
+__global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int r = lane & 31, h = lane >> 5;
+  const int item = yk_xcd_item(a.per_xcd);
+  if (item >= a.total) return;
+  const int tile = item % a.tiles_x, split = item / a.tiles_x;
+  const long qbase = ((long)tile * nwave + wave) * (32 * NB);
+  const long bbeg = (long)split * a.brange;
+  const long bend = min(a.nb, bbeg + a.brange);
+  const int nch = a.ld >> 4;
+  const int ld = a.ld;
+  const bool dot = (a.type == YK_DOT);
+
+  long qrow[NB];
+  bool qok[NB];
+  double qnd[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    long qi = qbase + j * 32 + r;
+    qok[j] = qi < a.nq;
+    qrow[j] = qok[j] ? qi : a.nq - 1;
+    qnd[j] = dot ? 0.0 : a.qn[qrow[j]];
+  }
+
+  // selection state
+  float bv[NB];
+  int bi[NB];
+  uint64_t top[NB][MODE == 2 ? R : 1];
+  uint64_t T[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    bv[j] = 1e30f;
+    bi[j] = -1;
+#pragma unroll
+    for (int s = 0; s < (MODE == 2 ? R : 1); s++) top[j][s] = YK_KEY_NONE;
+    T[j] = MODE == 3 ? a.thr[qrow[j]] : 0;
+  }
+  if (MODE == 3) {
+    bool act = false;
+#pragma unroll
+    for (int j = 0; j < NB; j++) act |= qok[j] && T[j] != 0;
+    if (!__any(act)) return;  // every query of this wave is already final
+  }
+
+  float4 qreg[NB][QRES ? 16 : 1];
+  if (QRES) {
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int c = 0; c < 8; c++)
+        if (c < nch) {
+          const float *p = a.q + qrow[j] * ld + c * 16 + h * 8;
+          qreg[j][2 * c] = *(const float4 *)p;
+          qreg[j][2 * c + 1] = *(const float4 *)(p + 4);
+        }
+  }
+
+  for (long bb = bbeg; bb < bend; bb += a.bstep) {
+    const long brow = min(bb + r, a.nb - 1);
+    const float *bp = a.b + brow * ld + h * 8;
+    f32x16 acc[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
+
+    if (QRES) {
+      float4 av[16];
+#pragma unroll
+      for (int c = 0; c < 8; c++)
+        if (c < nch) {
+          av[2 * c] = *(const float4 *)(bp + c * 16);
+          av[2 * c + 1] = *(const float4 *)(bp + c * 16 + 4);
+        }
+#pragma unroll
+      for (int c = 0; c < 8; c++)
+        if (c < nch) {
+#pragma unroll
+          for (int j = 0; j < NB; j++) {
+            float4 a0 = av[2 * c], a1 = av[2 * c + 1];
+            float4 b0 = qreg[j][QRES ? 2 * c : 0], b1 = qreg[j][QRES ? 2 * c + 1 : 0];
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc[j], 0, 0, 0);
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc[j], 0, 0, 0);
+          }
+        }
+    } else {
+      for (int c = 0; c < nch; c++) {
+        float4 a0 = *(const float4 *)(bp + c * 16);
+        float4 a1 = *(const float4 *)(bp + c * 16 + 4);
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+          const float *qp = a.q + qrow[j] * ld + c * 16 + h * 8;
+          float4 b0 = *(const float4 *)qp, b1 = *(const float4 *)(qp + 4);
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc[j], 0, 0, 0);
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc[j], 0, 0, 0);
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc[j], 0, 0, 0);
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc[j], 0, 0, 0);
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc[j], 0, 0, 0);
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc[j], 0, 0, 0);
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc[j], 0, 0, 0);
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc[j], 0, 0, 0);
+        }
+      }
+    }
+
+    // epilogue: lane holds query column r and base rows 8g + 4h + e
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+      const long b4 = bb + 8 * g + 4 * h;
+      float4 n4 = dot ? make_float4(0.f, 0.f, 0.f, 0.f) : *(const float4 *)(a.bn + b4);
+      float4 w4 = a.w ? *(const float4 *)(a.w + b4) : make_float4(1.f, 1.f, 1.f, 1.f);
+      const float nn[4] = {n4.x, n4.y, n4.z, n4.w};
+      const float ww[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int i = 4 * g + e;
+        const long bidx = b4 + e;
+        const bool bok = bidx < bend;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+          float v;
+          if (dot)
+            v = acc[j][i];
+          else
+            v = (float)(qnd[j] + (double)nn[e]) - 2.0f * acc[j][i];
+          if (a.w) v *= ww[e];
+          if (MODE == 0) {
+            if (bok && v < bv[j]) {  // strict '<' in ascending index order
+              bv[j] = v;
+              bi[j] = (int)bidx;
+            }
+          } else if (MODE == 2) {
+            uint64_t key = (bok && v == v) ? yk_key(v, (int)bidx) : YK_KEY_NONE;
+            if (key < top[j][R - 1]) yk_reg_insert<R>(top[j], key);
+          } else {
+            if (bok && qok[j] && v == v) {
+              uint64_t key = yk_key(v, (int)bidx);
+              if (key <= T[j]) {
+                int slot = atomicAdd(&a.cnt[qrow[j]], 1);
+                if (slot < YK_CAND_CAP) a.cand[qrow[j] * YK_CAND_CAP + slot] = key;
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+
+  if (MODE == 0) {
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      float ov = __shfl_xor(bv[j], 32);
+      int oi = __shfl_xor(bi[j], 32);
+      if (ov < bv[j] || (ov == bv[j] && oi < bi[j])) {
+        bv[j] = ov;
+        bi[j] = oi;
+      }
+      const long qi = qbase + j * 32 + r;
+      if (h == 0 && qok[j]) {
+        if (a.nsplit == 1) {
+          a.out_d[qi] = bv[j];
+          a.out_i[qi] = bi[j];
+        } else {
+          a.part[qi * a.nsplit + split] = bi[j] < 0 ? YK_KEY_NONE : yk_key(bv[j], bi[j]);
+        }
+      }
+    }
+  } else if (MODE == 2) {
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const long qi = qbase + j * 32 + r;
+      if (!qok[j]) continue;
+      uint64_t *dst = a.part + ((qi * a.nsplit + split) * 2 + h) * R;
+#pragma unroll
+      for (int s = 0; s < R; s++) dst[s] = top[j][s];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// small per-query sorts
+
+__device__ __forceinline__ void yk_store_key(uint64_t key, int *oi, float *od) {
+  *oi = key == YK_KEY_NONE ? -1 : (int)(uint32_t)key;
+  *od = key == YK_KEY_NONE ? __uint_as_float(0xffffffffu) : yk_fkey_inv((uint32_t)(key >> 32));
+}
+
+// k = 1 merge of per-split minima (nothing below 1e30 -> (-1, 1e30))
+__global__ void yk_merge1_kernel(const uint64_t *__restrict__ part, long nq, int nsplit,
+                                 int *out_i, float *out_d) {
+  long q = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  uint64_t best = YK_KEY_NONE;
+  for (int s = 0; s < nsplit; s++) {
+    uint64_t v = part[q * nsplit + s];
+    best = v < best ? v : best;
+  }
+  if (best == YK_KEY_NONE) {
+    out_i[q] = -1;
+    out_d[q] = 1e30f;
+  } else {
+    out_i[q] = (int)(uint32_t)best;
+    out_d[q] = yk_fkey_inv((uint32_t)(best >> 32));
+  }
+}
+
+// ascending bitonic sort of s[0..p2) by the block (p2 a power of two)
+__device__ void yk_block_bitonic(uint64_t *s, int p2) {
+  for (int size = 2; size <= p2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < p2 / 2; t += blockDim.x) {
+        int lo = 2 * t - (t & (stride - 1));
+        int hi = lo + stride;
+        bool up = (lo & size) == 0;
+        uint64_t x = s[lo], y = s[hi];
+        if ((x > y) == up) {
+          s[lo] = y;
+          s[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Sort the n keys of each query (src + q*n); then either write the k
+// smallest (out_i/out_d) or, when thr != NULL, the k-th smallest as the
+// query's threshold (YK_KEY_NONE when fewer than k real keys: admit all).
+__global__ __launch_bounds__(256) void yk_sortsel_kernel(const uint64_t *__restrict__ src,
+                                                         int n, int k, int p2, int *out_i,
+                                                         float *out_d, uint64_t *thr) {
+  extern __shared__ uint64_t s[];
+  const long q = blockIdx.x;
+  const uint64_t *p = src + q * (long)n;
+  for (int i = threadIdx.x; i < p2; i += blockDim.x) s[i] = i < n ? p[i] : YK_KEY_NONE;
+  __syncthreads();
+  yk_block_bitonic(s, p2);
+  if (thr) {
+    if (threadIdx.x == 0) thr[q] = k <= p2 ? s[k - 1] : YK_KEY_NONE;
+    return;
+  }
+  for (int i = threadIdx.x; i < k; i += blockDim.x)
+    yk_store_key(i < p2 ? s[i] : YK_KEY_NONE, out_i + q * k + i, out_d + q * k + i);
+}
+
+// After a MODE 3 pass: queries whose list fitted get their k smallest and
+// become inactive (thr = 0); overflowing queries get the k-th smallest
+// captured key as a tighter threshold, their count is reset, *flag is set.
+__global__ __launch_bounds__(256) void yk_select_kernel(const uint64_t *__restrict__ cand,
+                                                        int *cnt, uint64_t *thr, int k,
+                                                        int *out_i, float *out_d, int *flag) {
+  __shared__ uint64_t s[YK_CAND_CAP];
+  const long q = blockIdx.x;
+  if (thr[q] == 0) return;
+  const int n = cnt[q];
+  const int m = n < YK_CAND_CAP ? n : YK_CAND_CAP;
+  int p2 = 1;
+  while (p2 < m) p2 <<= 1;
+  const uint64_t *p = cand + q * (long)YK_CAND_CAP;
+  for (int i = threadIdx.x; i < p2; i += blockDim.x) s[i] = i < m ? p[i] : YK_KEY_NONE;
+  __syncthreads();
+  yk_block_bitonic(s, p2);
+  if (n > YK_CAND_CAP) {
+    if (threadIdx.x == 0) {
+      thr[q] = s[k - 1];
+      cnt[q] = 0;
+      atomicOr(flag, 1);
+    }
+    return;
+  }
+  for (int i = threadIdx.x; i < k; i += blockDim.x)
+    yk_store_key(i < p2 ? s[i] : YK_KEY_NONE, out_i + q * k + i, out_d + q * k + i);
+  __syncthreads();
+  if (threadIdx.x == 0) thr[q] = 0;
+}
+
+__global__ void yk_zero_cnt_kernel(int *cnt, long n) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) cnt[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// planning and launch
+
+struct YkPlan {
+  int nblk;    // query blocks of 32 per wave (NB)
+  int waves;   // waves per workgroup
+  int tiles;   // query tiles
+  int nsplit;  // base splits
+  long brange;
+  long bstep;
+};
+
+// Split the base so that tiles * nsplit reaches ~4 workgroups per CU, with at
+// least min_rows rows per split and at most max_split splits.
+static YkPlan yk_plan(long nq, long nb, long sample, long min_splits, long max_split) {
+  YkPlan p;
+  p.nblk = 1;
+  p.waves = 4;
+  const long qt = 32L * p.nblk * p.waves;
+  p.tiles = (int)((nq + qt - 1) / qt);
+  p.bstep = 32 * sample;
+  const long nblocks = (nb + p.bstep - 1) / p.bstep;  // processed blocks
+  long want = (1024 + p.tiles - 1) / p.tiles;
+  long maxs = (nblocks + 15) / 16;  // >= 16 processed blocks per split
+  if (want > maxs) want = maxs;
+  if (want < min_splits) want = min_splits;
+  if (want > max_split) want = max_split;
+  if (want > nblocks) want = nblocks;
+  if (want < 1) want = 1;
+  long per = (nblocks + want - 1) / want;  // processed blocks per split
+  p.brange = per * p.bstep;
+  p.nsplit = (int)((nb + p.brange - 1) / p.brange);
+  return p;
+}
+
+extern "C" int yk_knn_plan(long nq, long nb, int d, int k, int *nsplit, int *waves, int *nblk) {
+  (void)d;
+  YkPlan p = yk_plan(nq, nb, 1, 1, k > 1 ? 256 : 4096);
+  *nsplit = p.nsplit;
+  *waves = p.waves;
+  *nblk = p.nblk;
+  return YK_OK;
+}
+
+// grow-only device scratch of the selection pipeline
+static void *g_scratch[4];
+static size_t g_scratch_bytes[4];
+
+static int yk_scratch(int slot, size_t bytes, void **out) {
+  if (bytes > g_scratch_bytes[slot]) {
+    if (g_scratch[slot]) (void)hipFree(g_scratch[slot]);
+    g_scratch[slot] = nullptr;
+    g_scratch_bytes[slot] = 0;
+    hipError_t e = hipMalloc(&g_scratch[slot], bytes);
+    if (e != hipSuccess) return yk_hip_fail(e, "hipMalloc(selection scratch)");
+    g_scratch_bytes[slot] = bytes;
+  }
+  *out = g_scratch[slot];
+  return YK_OK;
+}
+
+template <int MODE, int R>
+static int yk_launch_scan(const YkPlan &p, ScanArgs a, bool qres) {
+  a.brange = p.brange;
+  a.bstep = p.bstep;
+  a.nsplit = p.nsplit;
+  a.tiles_x = p.tiles;
+  a.total = p.tiles * p.nsplit;
+  a.per_xcd = (a.total + 7) / 8;
+  dim3 grid(a.per_xcd * 8), block(64 * p.waves);
+  int tok = yk_prof_begin(YK_PROF_FUSED);
+  if (qres)
+    hipLaunchKernelGGL((yk_scan_f32<1, true, MODE, R>), grid, block, 0, yk_stream(), a);
+  else
+    hipLaunchKernelGGL((yk_scan_f32<1, false, MODE, R>), grid, block, 0, yk_stream(), a);
+  yk_prof_end(tok);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
+static int yk_sortsel(const uint64_t *src, long nq, int n, int k, int *oi, float *od,
+                      uint64_t *thr) {
+  int p2 = 1;
+  while (p2 < n) p2 <<= 1;
+  int tok = yk_prof_begin(YK_PROF_MERGE);
+  hipLaunchKernelGGL(yk_sortsel_kernel, dim3((unsigned)nq), dim3(256), sizeof(uint64_t) * p2,
+                     yk_stream(), src, n, k, p2, oi, od, thr);
+  yk_prof_end(tok);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
+// k > 16: sketch threshold, filtered scan, per-query sort (queries in batches
+// that bound the candidate memory).
+static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *dis) {
+  const long QB = 16384;
+  const long nq_all = base.nq;
+  void *vthr, *vcnt, *vcand, *vpart, *vflag;
+  const long qb = nq_all < QB ? nq_all : QB;
+  YK_TRY(yk_scratch(0, sizeof(uint64_t) * qb, &vthr));
+  YK_TRY(yk_scratch(1, sizeof(int) * (qb + 64), &vcnt));
+  YK_TRY(yk_scratch(2, sizeof(uint64_t) * qb * YK_CAND_CAP, &vcand));
+  uint64_t *thr = (uint64_t *)vthr;
+  int *cnt = (int *)vcnt;
+  int *flag = cnt + qb + 32;
+  vflag = flag;
+  // sampling stride of the sketch pass: candidates ~ stride * k must fit
+  long stride = YK_CAND_CAP / (4L * k);
+  if (stride > 8) stride = 8;
+  if (stride < 1) stride = 1;
+  const long nblocks = (base.nb + 31) / 32;
+  while (stride > 1 && (nblocks / stride) * 32 < 4L * k) stride >>= 1;
+  const long min_splits = (2L * k + 31) / 32;  // sketch: 32 keys per split and query
+
+  for (long q0 = 0; q0 < nq_all; q0 += QB) {
+    const long nq = nq_all - q0 < QB ? nq_all - q0 : QB;
+    ScanArgs a = base;
+    a.q = base.q + q0 * base.ld;
+    a.qn = base.qn ? base.qn + q0 : nullptr;
+    a.nq = nq;
+    // (A) sketch: register top-16 per lane over a strided sample
+    YkPlan pa = yk_plan(nq, a.nb, stride, min_splits, 256);
+    YK_TRY(yk_scratch(3, sizeof(uint64_t) * nq * pa.nsplit * 2 * YK_REG_R, &vpart));
+    a.part = (uint64_t *)vpart;
+    YK_TRY((yk_launch_scan<2, YK_REG_R>(pa, a, qres)));
+    YK_TRY(yk_sortsel(a.part, nq, pa.nsplit * 2 * YK_REG_R, k, nullptr, nullptr, thr));
+    // (B) filtered scan + per-query sort, retried for overflowing queries
+    YkPlan pb = yk_plan(nq, a.nb, 1, 1, 4096);
+    a.thr = thr;
+    a.cnt = cnt;
+    a.cand = (uint64_t *)vcand;
+    hipLaunchKernelGGL(yk_zero_cnt_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                       yk_stream(), cnt, nq);
+    hipLaunchKernelGGL(yk_zero_cnt_kernel, dim3(1), dim3(64), 0, yk_stream(), flag, 1L);
+    int tries = 0;
+    for (;;) {
+      YK_TRY((yk_launch_scan<3, 1>(pb, a, qres)));
+      int tok = yk_prof_begin(YK_PROF_MERGE);
+      hipLaunchKernelGGL(yk_select_kernel, dim3((unsigned)nq), dim3(256), 0, yk_stream(), a.cand,
+                         cnt, thr, k, assign + q0 * k, dis + q0 * k, flag);
+      yk_prof_end(tok);
+      YK_HIP(hipGetLastError());
+      int hflag = 0;
+      YK_HIP(hipMemcpyAsync(&hflag, vflag, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
+      YK_HIP(hipStreamSynchronize(yk_stream()));
+      if (!hflag) break;
+      if (++tries == 6)
+        return yk_set_error(YK_EINVAL,
+                            "k=%d: more than %d keys tie below the k-th distance of a query", k,
+                            YK_CAND_CAP);
+      hipLaunchKernelGGL(yk_zero_cnt_kernel, dim3(1), dim3(64), 0, yk_stream(), flag, 1L);
+    }
+  }
+  return YK_OK;
+}
+
+extern "C" int yk_knn_f32(int type, long nq, long nb, int d, int k, const float *q,
+                          const double *qn, const float *b, const float *bn, const float *w,
+                          int *assign, float *dis) {
+  YK_TRY(yk_init());
+  if (type == 12) type = YK_L2;
+  if (type != YK_L2 && type != YK_DOT)
+    return yk_set_error(YK_EINVAL, "distance type %d has no GPU path (2, 12, 16 only)", type);
+  if (k < 1 || k > nb || k > YK_KMAX)
+    return yk_set_error(YK_EINVAL, "k=%d out of range (nb=%ld, max %d)", k, nb, YK_KMAX);
+  if (nq <= 0) return YK_OK;
+  if (nb > 0x7fffffffL || nq > 0x7fffffffL) return yk_set_error(YK_EINVAL, "sizes exceed int32");
+  ScanArgs a = {};
+  a.q = q;
+  a.qn = qn;
+  a.nq = nq;
+  a.b = b;
+  a.bn = bn;
+  a.w = w;
+  a.nb = nb;
+  a.ld = yk_ld(d);
+  a.type = type;
+  a.out_i = assign;
+  a.out_d = dis;
+  const bool qres = a.ld <= 128;
+  if (k == 1) {
+    YkPlan p = yk_plan(nq, nb, 1, 1, 4096);
+    if (p.nsplit > 1) {
+      void *buf;
+      YK_TRY(yk_scratch(3, sizeof(uint64_t) * nq * p.nsplit, &buf));
+      a.part = (uint64_t *)buf;
+    }
+    YK_TRY((yk_launch_scan<0, 1>(p, a, qres)));
+    if (p.nsplit > 1) {
+      int tok = yk_prof_begin(YK_PROF_MERGE);
+      hipLaunchKernelGGL(yk_merge1_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                         yk_stream(), a.part, nq, p.nsplit, assign, dis);
+      yk_prof_end(tok);
+      YK_HIP(hipGetLastError());
+    }
+    return YK_OK;
+  }
+  if (k <= YK_REG_R) {
+    YkPlan p = yk_plan(nq, nb, 1, 1, 256);
+    void *buf;
+    YK_TRY(yk_scratch(3, sizeof(uint64_t) * nq * p.nsplit * 2 * YK_REG_R, &buf));
+    a.part = (uint64_t *)buf;
+    YK_TRY((yk_launch_scan<2, YK_REG_R>(p, a, qres)));
+    return yk_sortsel(a.part, nq, p.nsplit * 2 * YK_REG_R, k, assign, dis, nullptr);
+  }
+  return yk_knn_large_k(a, k, qres, assign, dis);
+}
