@@ -1,14 +1,19 @@
 """Benchmark of yael's exhaustive-kNN / k-means hot path on one MI355X.
 
-Default workload (the headline): exact kNN, nb = 1,000,000 base vectors and
+Workload (the headline): exact kNN, nb = 1,000,000 base vectors and
 nq = 10,000 queries, d = 128, k = 100, squared L2, vectors drawn from a
 1,024-component Gaussian mixture (centres U[0,255]^128, sigma = 20, clipped to
 [0,255], rounded to integers), float32.  One step = one full search of the
-10,000 queries against the resident base (norm kernels + fused distance /
-top-k kernel + split merge), inputs already in HBM.  value = queries/s.
+10,000 queries against the resident base, inputs already in HBM:
+  * FP32 form: norm kernels + fused FP32-MFMA distance / top-k kernels;
+  * precision form (the headline value): norm + BF16-conversion kernels,
+    BF16-MFMA first pass keeping k' = 200 candidates, FP32 re-rank
+    (bit-identical to the FP32 form), certificate, FP32 fallback.
+The line reports the precision form's queries/s; the FP32 form's rate, the
+ratio and the index/distance agreement between the two are in "forms".
 
 Also reported ("extras"): k-means iterations/s for n = 100,000, d = 128,
-k = 1,024, niter = 10 (same mixture, seed 0), device-resident.
+k = 1,024, niter = 10 (same mixture), device-resident.
 
 The CPU baseline is yael's own C path (oracle/_ref: the reference's sources
 compiled with OpenBLAS + OpenMP), timed on the host cores on a bounded sample
@@ -29,8 +34,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 (matrix)
-PEAK_BF16_TFLOPS = 2500.0
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16
 PEAK_HBM_GBS = 8000.0
+
+PROF_FUSED, PROF_NORM, PROF_UPDATE, PROF_MERGE, PROF_RERANK, PROF_BATCH, PROF_BF16 = range(7)
 
 
 def mixture(seed, n, d, ncomp, sigma, integer=True, centres_seed=None):
@@ -50,97 +57,93 @@ def mixture(seed, n, d, ncomp, sigma, integer=True, centres_seed=None):
     return out
 
 
-class Dev:
-    """A padded device copy of an (n, d) float32 array."""
-
-    def __init__(self, ya, x):
-        self.ya, self.lib = ya, ya.lib()
-        self.n, self.d = x.shape
-        self.ld = self.lib.yk_ld(self.d)
-        self.ptr = ctypes.c_void_p()
-        ya.check(self.lib.yk_malloc(ctypes.byref(self.ptr), self.n * self.ld * 4))
-        ya.check(self.lib.yk_upload_vectors(self.ptr, x.ctypes.data_as(ctypes.c_void_p), self.n, self.d))
-
-
-def dalloc(ya, nbytes):
-    p = ctypes.c_void_p()
-    ya.check(ya.lib().yk_malloc(ctypes.byref(p), nbytes))
-    return p
-
-
-def prof_avg_ms(ya, slot):
+def prof(lib, slot):
     tot, cnt = ctypes.c_double(), ctypes.c_long()
-    ya.check(ya.lib().yk_prof_get(slot, ctypes.byref(tot), ctypes.byref(cnt)))
-    return (tot.value / cnt.value if cnt.value else float("nan")), cnt.value
+    lib.yk_prof_get(slot, ctypes.byref(tot), ctypes.byref(cnt))
+    return tot.value, cnt.value
 
 
-def bench_knn(ya, args, base, queries, k):
-    lib = ya.lib()
-    db, dq = Dev(ya, base), Dev(ya, queries)
-    nb, d = base.shape
-    nq = queries.shape[0]
-    bn = dalloc(ya, ((nb + 31) // 32 * 32) * 4)
-    qn = dalloc(ya, nq * 8)
-    oi = dalloc(ya, nq * k * 4)
-    od = dalloc(ya, nq * k * 4)
-
-    def step():
-        ya.check(lib.yk_norms(db.ptr, nb, d, bn, None))
-        ya.check(lib.yk_norms(dq.ptr, nq, d, None, qn))
-        ya.check(lib.yk_knn_f32(2, nq, nb, d, k, dq.ptr, qn, db.ptr, bn, None, oi, od))
-
-    for _ in range(args.warmup):
-        step()
-    ya.check(lib.yk_sync())
-    ya.check(lib.yk_prof_reset())
-    ya.check(lib.yk_prof_enable(1))
-    ya.check(lib.yk_sync())
+def timed(lib, fn, steps, warmup):
+    """Run fn warmup + steps times; wall time per step and per-slot kernel
+    times (HIP events on the library's stream, summed over the steps)."""
+    for _ in range(warmup):
+        fn()
+    lib.yk_sync()
+    lib.yk_prof_reset()
+    lib.yk_prof_enable(1)
+    lib.yk_sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ya.check(lib.yk_sync())
+    for _ in range(steps):
+        fn()
+    lib.yk_sync()
     t1 = time.perf_counter()
-    ya.check(lib.yk_prof_enable(0))
-    fused_ms, nlaunch = prof_avg_ms(ya, 0)
-    norm_ms, _ = prof_avg_ms(ya, 1)
-    merge_ms, _ = prof_avg_ms(ya, 3)
-    idx = np.empty((nq, k), np.int32)
-    dis = np.empty((nq, k), np.float32)
-    ya.check(lib.yk_memcpy_d2h(idx.ctypes.data_as(ctypes.c_void_p), oi, idx.nbytes))
-    ya.check(lib.yk_memcpy_d2h(dis.ctypes.data_as(ctypes.c_void_p), od, dis.nbytes))
-    # FP64 error report of the returned distances (GPU FP64 kernel)
-    o64 = dalloc(ya, nq * k * 8)
-    ya.check(lib.yk_pair_dist_f64(2, nq, d, k, dq.ptr, db.ptr, oi, o64))
-    d64 = np.empty((nq, k), np.float64)
-    ya.check(lib.yk_memcpy_d2h(d64.ctypes.data_as(ctypes.c_void_p), o64, d64.nbytes))
-    rel = np.abs(dis.astype(np.float64) - d64) / np.maximum(np.abs(d64), 1e-30)
-    for p in (db.ptr, dq.ptr, bn, qn, oi, od, o64):
-        lib.yk_free(p)
-    sec = (t1 - t0) / args.steps
-    return dict(sec=sec, fused_ms=fused_ms, norm_ms=norm_ms, merge_ms=merge_ms, nlaunch=nlaunch,
-                idx=idx, dis=dis, max_rel_err_f64=float(np.nanmax(rel)),
-                sorted_ok=bool(np.all(np.diff(dis, axis=1) >= 0)))
+    lib.yk_prof_enable(0)
+    slots = {s: prof(lib, s) for s in range(7)}
+    return (t1 - t0) / steps, slots
 
 
-def bench_kmeans(ya, v, k, niter, seed):
+def bench_knn(ya, dev, args, base, queries, k, kp):
     lib = ya.lib()
-    dv = Dev(ya, v)
+    B, Q = dev.DeviceSet(base), dev.DeviceSet(queries)
+    nq, nb, d = Q.n, B.n, B.d
+    oi, od = dev.DeviceBuffer(nq * k * 4), dev.DeviceBuffer(nq * k * 4)
+    oi16, od16 = dev.DeviceBuffer(nq * k * 4), dev.DeviceBuffer(nq * k * 4)
+    nfb = ctypes.c_int(0)
+
+    def step_fp32():
+        Q.compute_norms()
+        B.compute_norms()
+        ya.check(lib.yk_knn_f32(2, nq, nb, d, k, Q.vec.ptr, Q.dnorm.ptr, B.vec.ptr, B.fnorm.ptr,
+                                None, oi.ptr, od.ptr))
+
+    def step_bf16():
+        Q.compute_norms()
+        B.compute_norms()
+        qh, qst, _ = Q.compute_bf16(2, True)
+        bh, _, bmx = B.compute_bf16(2, False)
+        ya.check(lib.yk_knn_bf16(2, nq, nb, d, k, kp, Q.vec.ptr, Q.dnorm.ptr, qh.ptr, qst.ptr,
+                                 B.vec.ptr, B.fnorm.ptr, bh.ptr, bmx.ptr, oi16.ptr, od16.ptr,
+                                 ctypes.byref(nfb)))
+
+    sec32, sl32 = (0.0, None) if args.only_bf16 else timed(lib, step_fp32, args.steps, args.warmup)
+    if not args.only_bf16:
+        print("# fp32 form: %.3f ms/step" % (sec32 * 1e3), file=sys.stderr, flush=True)
+    sec16, sl16 = timed(lib, step_bf16, args.steps, args.warmup)
+    print("# bf16+rerank form: %.3f ms/step, fallback queries %d" % (sec16 * 1e3, nfb.value),
+          file=sys.stderr, flush=True)
+    i32, d32 = oi.to_host(np.int32, (nq, k)), od.to_host(np.float32, (nq, k))
+    i16, d16 = oi16.to_host(np.int32, (nq, k)), od16.to_host(np.float32, (nq, k))
+    # FP64 error report of the returned distances (GPU FP64 kernel)
+    o64 = dev.DeviceBuffer(nq * k * 8)
+    ya.check(lib.yk_pair_dist_f64(2, nq, d, k, Q.vec.ptr, B.vec.ptr, oi16.ptr, o64.ptr))
+    d64 = o64.to_host(np.float64, (nq, k))
+    rel = np.abs(d16.astype(np.float64) - d64) / np.maximum(np.abs(d64), 1e-30)
+    B.free()
+    Q.free()
+    return dict(sec32=sec32, sl32=sl32, sec16=sec16, sl16=sl16, i32=i32, d32=d32, i16=i16,
+                d16=d16, nfb=nfb.value, max_rel_err_f64=float(np.nanmax(rel)),
+                sorted_ok=bool(np.all(np.diff(d16, axis=1) >= 0)))
+
+
+def bench_kmeans(ya, dev, v, k, niter, seed):
+    lib = ya.lib()
+    V = dev.DeviceSet(v)
     n, d = v.shape
     cent = np.empty((k, d), np.float32)
     ret = ctypes.c_float()
     flags = 0x10000  # quiet
-    # warm-up run (allocations, code objects)
-    ya.check(lib.yael_kmeans_device(d, n, k, 2, dv.ptr, flags, seed, 1,
-                                    cent.ctypes.data_as(ctypes.c_void_p), None, None, None,
-                                    ctypes.byref(ret)))
-    ya.check(lib.yk_sync())
+
+    def run(it):
+        ya.check(lib.yael_kmeans_device(d, n, k, it, V.vec.ptr, flags, seed, 1,
+                                        cent.ctypes.data_as(ctypes.c_void_p), None, None, None,
+                                        ctypes.byref(ret)))
+    run(2)
+    lib.yk_sync()
     t0 = time.perf_counter()
-    ya.check(lib.yael_kmeans_device(d, n, k, niter, dv.ptr, flags, seed, 1,
-                                    cent.ctypes.data_as(ctypes.c_void_p), None, None, None,
-                                    ctypes.byref(ret)))
-    ya.check(lib.yk_sync())
+    run(niter)
+    lib.yk_sync()
     t1 = time.perf_counter()
-    lib.yk_free(dv.ptr)
+    V.free()
     return dict(sec=t1 - t0, iters=niter, obj=float(ret.value))
 
 
@@ -150,8 +153,7 @@ def cpu_baseline_knn(base, queries, k, budget_s=12.0):
     os.environ["OPENBLAS_NUM_THREADS"] = "1"
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_bind import Reference, Oracle, have_ref
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
+    cores = min(len(os.sched_getaffinity(0)), 16)
     if have_ref():
         api, kind = Reference(), "reference"
     else:
@@ -177,8 +179,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--only-bf16", action="store_true", help="profile runs: the precision form only")
     ap.add_argument("--nb", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--kprime", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (got --gpus %d); the multi-GPU "
@@ -186,9 +190,9 @@ def main():
         sys.exit(2)
 
     import yael_amd as ya
+    from yael_amd import device as dev
     ya.init()
-    info = ya.device_info()
-    print("# device:", info, file=sys.stderr, flush=True)
+    print("# device:", ya.device_info(), file=sys.stderr, flush=True)
 
     d, k = 128, 100
     t = time.perf_counter()
@@ -196,49 +200,71 @@ def main():
     queries = mixture(2, args.nq, d, 1024, 20, centres_seed=0)
     print("# data generated in %.1fs" % (time.perf_counter() - t), file=sys.stderr, flush=True)
 
-    r = bench_knn(ya, args, base, queries, k)
+    r = bench_knn(ya, dev, args, base, queries, k, args.kprime)
     nq, nb = args.nq, args.nb
-    qps = nq / r["sec"]
     flops = 2.0 * nq * nb * d
-    achieved = flops / (r["fused_ms"] * 1e-3) / 1e12
-    print("# knn fp32: %.3f ms/step, fused kernel %.3f ms (%d launches), norms %.3f ms, merge %.3f ms"
-          % (r["sec"] * 1e3, r["fused_ms"], r["nlaunch"], r["norm_ms"], r["merge_ms"]),
-          file=sys.stderr, flush=True)
-
+    qps16 = nq / r["sec16"]
+    bf_ms, bf_n = r["sl16"][PROF_BF16]
+    bf_ms_step = bf_ms / args.steps
+    ach16 = flops / (bf_ms_step * 1e-3) / 1e12
     out = {
-        "metric": "exact kNN queries/s, d=128, k=100, nb=1e6, nq=1e4 (FP32 MFMA form)",
-        "value": qps,
+        "metric": "exact kNN queries/s at d=128 (BF16+FP32-rerank form, exact-NN match with FP32)",
+        "value": qps16,
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r["sec"] * 1e3,
+        "ms_per_step": r["sec16"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "bf16 first pass + fp32 re-rank",
         "data": "synthetic: 1,024-component Gaussian mixture, integer-valued float32",
-        "config": {"workload": "knn_full type 2 (squared L2), nb=%d, nq=%d, d=%d, k=%d" % (nb, nq, d, k),
-                   "nb": nb, "nq": nq, "d": d, "k": k},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "yk_fused_f32 (distance GEMM + top-k)",
-                     "avg_launch_ms": r["fused_ms"]},
-        "accuracy": {"max_rel_err_vs_fp64": r["max_rel_err_f64"], "sorted": r["sorted_ok"]},
+        "config": {"workload": "knn_full type 2 (squared L2), nb=%d, nq=%d, d=%d, k=%d, k'=%d"
+                               % (nb, nq, d, k, args.kprime),
+                   "nb": nb, "nq": nq, "d": d, "k": k, "kprime": args.kprime},
+        "roofline": {"bound": "mfma", "achieved": ach16, "peak": PEAK_BF16_TFLOPS,
+                     "unit": "TFLOP/s", "frac": ach16 / PEAK_BF16_TFLOPS, "traffic": None,
+                     "kernel": "yk_scan_bf16 (first pass: sketch + filtered scan)",
+                     "flop_per_step": flops, "kernel_ms_per_step": bf_ms_step,
+                     "launches_per_step": bf_n / args.steps},
+        "accuracy": {"max_rel_err_vs_fp64": r["max_rel_err_f64"], "sorted": r["sorted_ok"],
+                     "fallback_queries": r["nfb"]},
     }
+    print("# bf16 scan kernels: %.3f ms/step (%.1f launches) -> %.1f TFLOP/s"
+          % (bf_ms_step, bf_n / args.steps, ach16), file=sys.stderr, flush=True)
+    if not args.only_bf16:
+        qps32 = nq / r["sec32"]
+        f32_ms, f32_n = r["sl32"][PROF_FUSED]
+        f32_ms_step = f32_ms / args.steps
+        ach32 = flops / (f32_ms_step * 1e-3) / 1e12
+        out["forms"] = {
+            "fp32": {"queries_per_s": qps32, "ms_per_step": r["sec32"] * 1e3,
+                     "roofline": {"bound": "mfma", "achieved": ach32, "peak": PEAK_FP32_TFLOPS,
+                                  "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32_TFLOPS,
+                                  "kernel": "yk_scan_f32", "kernel_ms_per_step": f32_ms_step,
+                                  "launches_per_step": f32_n / args.steps}},
+            "bf16_rerank_over_fp32": qps16 / qps32,
+            "index_agreement_with_fp32": float(np.mean(np.all(r["i16"] == r["i32"], axis=1))),
+            "distances_bit_identical_to_fp32": bool(
+                np.array_equal(r["d16"].view(np.int32), r["d32"].view(np.int32))),
+        }
+        print("# fp32 scan kernels: %.3f ms/step -> %.1f TFLOP/s" % (f32_ms_step, ach32),
+              file=sys.stderr, flush=True)
 
     if not args.no_extras:
         v = mixture(3, 100_000, 128, 1024, 20, centres_seed=0)
-        km = bench_kmeans(ya, v, 1024, 10, 1)
+        km = bench_kmeans(ya, dev, v, 1024, 10, 1)
         out["extras"] = {"kmeans_iter_per_s": km["iters"] / km["sec"],
-                         "kmeans_config": "n=100000 d=128 k=1024 niter=10, device-resident",
+                         "kmeans_config": "n=100000 d=128 k=1024 niter=10, FP32 assign, "
+                                          "device-resident",
                          "kmeans_obj": km["obj"]}
         print("# kmeans: %.2f iter/s" % (km["iters"] / km["sec"]), file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         cb = cpu_baseline_knn(base, queries, k)
-        g_idx = r["idx"][:cb["nq"]]
-        agree = float((np.sort(g_idx, 1) == np.sort(cb["idx"], 1)).mean())
+        g_idx = r["i16"][:cb["nq"]]
+        agree = float(np.mean(np.sort(g_idx, 1) == np.sort(cb["idx"], 1)))
         out["cpu_baseline"] = {"value": cb["value"], "unit": cb["unit"], "cores": cb["cores"],
                                "kind": cb["kind"], "sample": cb["sample"],
                                "index_agreement_with_gpu": agree}

@@ -1,0 +1,82 @@
+"""GPU tests of the precision form (BF16 MFMA first pass + FP32 re-rank +
+certificate + FP32 fallback).  The bar: the same indices and the same
+distance bits as the FP32 form on every query (the re-rank recomputes
+distances with the FP32 form's instruction sequence, and the certificate
+guarantees the FP32 form's k nearest are among the candidates)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import yael_amd
+    from yael_amd import device
+    yael_amd.init()
+    return device
+
+
+def mixture(rng, n, d, ncomp, sigma, integer=True):
+    cen = rng.uniform(0, 255, (ncomp, d))
+    x = np.clip(cen[rng.integers(0, ncomp, n)] + sigma * rng.standard_normal((n, d)), 0, 255)
+    return (np.rint(x) if integer else x).astype(np.float32)
+
+
+def normal_unit(rng, n, d):
+    x = np.abs(rng.standard_normal((n, d))).astype(np.float32)
+    return (x / np.linalg.norm(x, axis=1, keepdims=True)).astype(np.float32)
+
+
+def same_as_fp32(dev, q, b, k, kp, typ=2):
+    Q, B = dev.DeviceSet(q), dev.DeviceSet(b)
+    i32, d32 = dev.knn_fp32(Q, B, k, typ)
+    i16, d16, nfb = dev.knn_bf16(Q, B, k, kp, typ)
+    Q.free()
+    B.free()
+    assert np.array_equal(i16, i32), "indices differ from the FP32 form"
+    assert np.array_equal(d16.view(np.int32), d32.view(np.int32)), "distances differ"
+    return nfb
+
+
+@pytest.mark.parametrize("k,kp", [(1, 16), (10, 64), (100, 200)])
+def test_integer_mixture(dev, k, kp):
+    rng = np.random.default_rng(k)
+    b = mixture(rng, 60000, 128, 256, 20)
+    q = mixture(rng, 900, 128, 256, 20)
+    nfb = same_as_fp32(dev, q, b, k, kp)
+    print("k", k, "fallback queries", nfb, flush=True)
+    assert nfb <= 0.05 * len(q)
+
+
+def test_unit_vectors_d960(dev):
+    rng = np.random.default_rng(5)
+    b = normal_unit(rng, 40000, 960)
+    q = normal_unit(rng, 300, 960)
+    nfb = same_as_fp32(dev, q, b, 10, 64)
+    print("fallback queries", nfb, flush=True)
+
+
+@pytest.mark.parametrize("d", [20, 200])
+def test_float_dims_and_dot(dev, d):
+    rng = np.random.default_rng(d)
+    b = rng.standard_normal((20000, d)).astype(np.float32)
+    q = rng.standard_normal((257, d)).astype(np.float32)
+    same_as_fp32(dev, q, b, 8, 32, typ=2)
+    same_as_fp32(dev, q, b, 8, 32, typ=16)
+
+
+def test_forced_fallback(dev):
+    """k' = k: the certificate can rarely hold, the FP32 fallback answers."""
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal((5000, 64)).astype(np.float32)
+    q = rng.standard_normal((100, 64)).astype(np.float32)
+    nfb = same_as_fp32(dev, q, b, 20, 20)
+    assert nfb > 0
+
+
+def test_small_base(dev):
+    rng = np.random.default_rng(10)
+    b = mixture(rng, 50, 16, 4, 10)
+    q = mixture(rng, 70, 16, 4, 10)
+    same_as_fp32(dev, q, b, 5, 64)

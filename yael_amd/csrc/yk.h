@@ -65,6 +65,23 @@ int yk_knn_f32(int type, long nq, long nb, int d, int k, const float *q,
                const double *qn, const float *b, const float *bn, const float *w,
                int *assign, float *dis);
 
+/* Precision form.  yk_to_bf16 writes the BF16 copy of n padded vectors
+ * (row stride yk_bf16_ld(d, type) uint16; for L2 the squared norm is appended
+ * in three BF16 pieces, queries are stored as -2q) and per-vector statistics
+ * stats[4*i..] = {|x|, |bf(x)|, |x - bf(x)|, |x|^2}; stats_max (4 floats,
+ * optional) receives their maxima over the set.
+ * yk_knn_bf16: BF16 MFMA first pass keeps the kp best candidates per query,
+ * re-ranks them with the FP32 form's arithmetic (bit-identical values) and
+ * certifies the result; uncertified queries are searched with the FP32 form
+ * (*n_fallback of them).  Output as yk_knn_f32. */
+int yk_bf16_ld(int d, int type);
+int yk_to_bf16(const float *x, long n, int d, int type, int is_query, uint16_t *out,
+               float *stats, float *stats_max);
+int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, const float *q,
+                const double *qn, const uint16_t *qh, const float *qstats, const float *b,
+                const float *bn, const uint16_t *bh, const float *bstats_max, int *assign,
+                float *dis, int *n_fallback);
+
 /* Cross distances, dist2[i + ldd*j] for a (na, float norms) x b (nb, double
  * norms), types 2/12/16 (nn.c:92-129, 266-306). */
 int yk_cross_distances(int type, long na, long nb, int d, const float *a,

@@ -1,0 +1,647 @@
+// yk_bf16.hip -- precision form of the exhaustive kNN: a BF16 MFMA first pass
+// selects k' candidates per query, an FP32 re-rank recomputes their
+// distances exactly as the FP32 form does, and a certificate proves that the
+// FP32 form's k nearest are among the candidates (queries that fail it are
+// searched again with the FP32 form).
+//
+// First pass.  Vectors are stored as BF16 with the squared norms appended
+// as three BF16 pieces (hi + mid + lo carries 24 bits), so one MFMA
+// contraction yields the whole expanded distance:
+//     base  row  [ bf(b_0) .. bf(b_d-1) | nb_hi nb_mid nb_lo | 1 1 1 ]
+//     query col  [ bf(-2q_0) .. bf(-2q_d-1) | 1 1 1 | nq_hi nq_mid nq_lo ]
+//     acc = ||q||^2 + ||b||^2 - 2 bf(q).bf(b)        (FP32 accumulation)
+// (type 16: plain bf(q).bf(b)).  The epilogue is a compare and a rare
+// append: v_mfma_f32_32x32x16_bf16 does 16 k-steps per instruction.
+//
+// Re-rank.  Each candidate's distance is recomputed with the FP32 form's own
+// instruction sequence (v_mfma_f32_32x32x2_f32, same k order, same
+// norm/epilogue arithmetic), so re-ranked values are bit-identical to what the
+// FP32 form computes for the same pair.
+//
+// Certificate.  With T = the k'-th smallest first-pass value, every
+// non-candidate has first-pass value >= T, so its exact distance is >=
+// T - E_bf and its FP32-form value >= T - E_bf - E_32.  If the k-th re-ranked
+// value F_k < T - E_bf - E_32 the FP32 form's top k are all candidates.
+//   E_bf  = 2 (|e_q| max|b| + |bf q| max|e_b|)                 (Cauchy-Schwarz
+//           on the BF16 residuals e = x - bf(x))
+//         + gamma (2 |bf q| max|bf b| + |q|^2 + max|b|^2)     (FP32 accumulation)
+//   E_32  = gamma (2 |q| max|b| + |q|^2 + max|b|^2)
+// with gamma = 2 (d' + 8) 2^-23 (d' = padded length, truncating adds allowed).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "yk.h"
+#include "yk_dev.h"
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+#define YK_CAND_CAP 4096
+#define YK_REG_R 16
+
+// ---------------------------------------------------------------------------
+// conversion
+
+__device__ __forceinline__ uint16_t yk_bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // NaN stays NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float yk_bf16_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+extern "C" int yk_bf16_ld(int d, int type) {
+  int extra = (type == YK_DOT) ? 0 : 6;
+  return (d + extra + 15) & ~15;
+}
+
+// One thread per vector: BF16 row (augmented for L2) + error statistics
+// stats[i] = {|x|, |bf(x)|, |x - bf(x)|, norm} (float).
+__global__ void yk_to_bf16_kernel(const float *__restrict__ x, long n, int d, int ld, int ldh,
+                                  int type, int is_query, uint16_t *__restrict__ out,
+                                  float4 *__restrict__ stats) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float *row = x + i * ld;
+  uint16_t *o = out + i * ldh;
+  double s2 = 0, sb = 0, se = 0;
+  const float scale = (type != YK_DOT && is_query) ? -2.0f : 1.0f;
+  for (int j = 0; j < d; j++) {
+    float v = row[j];
+    uint16_t hb = yk_bf16_rne(v * scale);
+    float vb = yk_bf16_f(hb) / scale;  // exact: scale is a power of two
+    o[j] = hb;
+    s2 += (double)v * v;
+    sb += (double)vb * vb;
+    se += ((double)v - vb) * ((double)v - vb);
+  }
+  int j = d;
+  if (type != YK_DOT) {
+    float nf = (float)s2;
+    uint16_t h1 = yk_bf16_rne(nf);
+    float r1 = nf - yk_bf16_f(h1);
+    uint16_t h2 = yk_bf16_rne(r1);
+    float r2 = r1 - yk_bf16_f(h2);
+    uint16_t h3 = yk_bf16_rne(r2);
+    const uint16_t one = 0x3f80;
+    if (is_query) {
+      o[j++] = one; o[j++] = one; o[j++] = one;
+      o[j++] = h1; o[j++] = h2; o[j++] = h3;
+    } else {
+      o[j++] = h1; o[j++] = h2; o[j++] = h3;
+      o[j++] = one; o[j++] = one; o[j++] = one;
+    }
+  }
+  for (; j < ldh; j++) o[j] = 0;
+  stats[i] = make_float4((float)sqrt(s2), (float)sqrt(sb), (float)sqrt(se), (float)s2);
+}
+
+// per-set maxima of the statistics (for the certificate), one workgroup
+__global__ __launch_bounds__(256) void yk_stats_max_kernel(const float4 *__restrict__ st, long n,
+                                                           float4 *out) {
+  __shared__ float4 s[256];
+  float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (long i = threadIdx.x; i < n; i += 256) {
+    float4 v = st[i];
+    m.x = fmaxf(m.x, v.x); m.y = fmaxf(m.y, v.y); m.z = fmaxf(m.z, v.z); m.w = fmaxf(m.w, v.w);
+  }
+  s[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      float4 a = s[threadIdx.x], b = s[threadIdx.x + off];
+      s[threadIdx.x] = make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
+extern "C" int yk_to_bf16(const float *x, long n, int d, int type, int is_query, uint16_t *out,
+                          float *stats, float *stats_max) {
+  YK_TRY(yk_init());
+  if (type == 12) type = YK_L2;
+  if (n <= 0) return YK_OK;
+  const int ld = yk_ld(d), ldh = yk_bf16_ld(d, type);
+  hipLaunchKernelGGL(yk_to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     yk_stream(), x, n, d, ld, ldh, type, is_query, out, (float4 *)stats);
+  if (stats_max)
+    hipLaunchKernelGGL(yk_stats_max_kernel, dim3(1), dim3(256), 0, yk_stream(),
+                       (const float4 *)stats, n, (float4 *)stats_max);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// BF16 scan
+
+struct Bf16Args {
+  const uint16_t *q;  // nq x ldh
+  long nq;
+  const uint16_t *b;  // nb x ldh
+  long nb;
+  int ldh;
+  long brange, bstep;
+  int nsplit, tiles_x, total, per_xcd;
+  uint64_t *part;       // MODE 2: [nq][nsplit][2][R]
+  const uint64_t *thr;  // MODE 3
+  int *cnt;
+  uint64_t *cand;
+};
+
+template <int R>
+__device__ __forceinline__ void yk_reg_insert16(uint64_t (&t)[R], uint64_t x) {
+#pragma unroll
+  for (int i = R - 1; i > 0; i--) {
+    uint64_t lo = t[i - 1], cur = t[i];
+    t[i] = x < lo ? lo : (x < cur ? x : cur);
+  }
+  t[0] = x < t[0] ? x : t[0];
+}
+
+template <int NB, bool QRES, int MODE, int R>
+__global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int r = lane & 31, h = lane >> 5;
+  const int id = blockIdx.x;
+  const int item = (id & 7) * a.per_xcd + (id >> 3);
+  if (item >= a.total) return;
+  const int tile = item % a.tiles_x, split = item / a.tiles_x;
+  const long qbase = ((long)tile * nwave + wave) * (32 * NB);
+  const long bbeg = (long)split * a.brange;
+  const long bend = min(a.nb, bbeg + a.brange);
+  const int nch = a.ldh >> 4;
+  const int ldh = a.ldh;
+
+  long qrow[NB];
+  bool qok[NB];
+  uint64_t top[NB][MODE == 2 ? R : 1];
+  uint64_t T[NB];
+  float Tv[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    long qi = qbase + j * 32 + r;
+    qok[j] = qi < a.nq;
+    qrow[j] = qok[j] ? qi : a.nq - 1;
+#pragma unroll
+    for (int s = 0; s < (MODE == 2 ? R : 1); s++) top[j][s] = YK_KEY_NONE;
+    T[j] = MODE == 3 ? a.thr[qrow[j]] : 0;
+    Tv[j] = (T[j] == 0) ? -INFINITY
+                        : (T[j] == YK_KEY_NONE ? INFINITY : yk_fkey_inv((uint32_t)(T[j] >> 32)));
+  }
+  if (MODE == 3) {
+    bool act = false;
+#pragma unroll
+    for (int j = 0; j < NB; j++) act |= qok[j] && T[j] != 0;
+    if (!__any(act)) return;
+  }
+
+  bf16x8 qreg[NB][QRES ? 16 : 1];
+  if (QRES) {
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int c = 0; c < 16; c++)
+        if (c < nch) qreg[j][c] = *(const bf16x8 *)(a.q + qrow[j] * ldh + c * 16 + h * 8);
+  }
+
+  for (long bb = bbeg; bb < bend; bb += a.bstep) {
+    const long brow = min(bb + r, a.nb - 1);
+    const uint16_t *bp = a.b + brow * ldh + h * 8;
+    f32x16 acc[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
+    if (QRES) {
+      bf16x8 av[16];
+#pragma unroll
+      for (int c = 0; c < 16; c++)
+        if (c < nch) av[c] = *(const bf16x8 *)(bp + c * 16);
+#pragma unroll
+      for (int c = 0; c < 16; c++)
+        if (c < nch) {
+#pragma unroll
+          for (int j = 0; j < NB; j++)
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[c], qreg[j][QRES ? c : 0], acc[j],
+                                                             0, 0, 0);
+        }
+    } else {
+      for (int c = 0; c < nch; c++) {
+        bf16x8 av = *(const bf16x8 *)(bp + c * 16);
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+          bf16x8 bq = *(const bf16x8 *)(a.q + qrow[j] * ldh + c * 16 + h * 8);
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bq, acc[j], 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
+      const bool bok = bidx < bend;
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        const float v = acc[j][i];
+        if (MODE == 2) {
+          uint64_t key = (bok && v == v) ? yk_key(v, (int)bidx) : YK_KEY_NONE;
+          if (key < top[j][R - 1]) yk_reg_insert16<R>(top[j], key);
+        } else {
+          if (v <= Tv[j] && bok && qok[j]) {
+            uint64_t key = yk_key(v, (int)bidx);
+            if (key <= T[j]) {
+              int slot = atomicAdd(&a.cnt[qrow[j]], 1);
+              if (slot < YK_CAND_CAP) a.cand[qrow[j] * YK_CAND_CAP + slot] = key;
+            }
+          }
+        }
+      }
+    }
+  }
+  if (MODE == 2) {
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      if (!qok[j]) continue;
+      const long qi = qbase + j * 32 + r;
+      uint64_t *dst = a.part + ((qi * a.nsplit + split) * 2 + h) * R;
+#pragma unroll
+      for (int s = 0; s < R; s++) dst[s] = top[j][s];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-query sorts (same contracts as in yk_select.hip)
+
+__device__ void yk_bitonic_b(uint64_t *s, int p2) {
+  for (int size = 2; size <= p2; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < p2 / 2; t += blockDim.x) {
+        int lo = 2 * t - (t & (stride - 1)), hi = lo + stride;
+        bool up = (lo & size) == 0;
+        uint64_t x = s[lo], y = s[hi];
+        if ((x > y) == up) {
+          s[lo] = y;
+          s[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+}
+
+// sort n keys per query; write the kp smallest keys (candidates) or the
+// kp-th smallest as threshold
+__global__ __launch_bounds__(256) void yk_bf_sortsel_kernel(const uint64_t *__restrict__ src, int n,
+                                                            int kp, int p2, uint64_t *cand_out,
+                                                            uint64_t *thr) {
+  extern __shared__ uint64_t s[];
+  const long q = blockIdx.x;
+  const uint64_t *p = src + q * (long)n;
+  for (int i = threadIdx.x; i < p2; i += blockDim.x) s[i] = i < n ? p[i] : YK_KEY_NONE;
+  __syncthreads();
+  yk_bitonic_b(s, p2);
+  if (thr) {
+    if (threadIdx.x == 0) thr[q] = kp <= p2 ? s[kp - 1] : YK_KEY_NONE;
+    return;
+  }
+  for (int i = threadIdx.x; i < kp; i += blockDim.x) cand_out[q * kp + i] = i < p2 ? s[i] : YK_KEY_NONE;
+}
+
+// candidates of a filtered pass: fitted -> the kp smallest into cand_out,
+// thr = 0; overflow -> thr = kp-th captured key, cnt = 0, flag set
+__global__ __launch_bounds__(256) void yk_bf_select_kernel(const uint64_t *__restrict__ cand,
+                                                           int *cnt, uint64_t *thr, int kp,
+                                                           uint64_t *cand_out, int *flag) {
+  __shared__ uint64_t s[YK_CAND_CAP];
+  const long q = blockIdx.x;
+  if (thr[q] == 0) return;
+  const int n = cnt[q];
+  const int m = n < YK_CAND_CAP ? n : YK_CAND_CAP;
+  int p2 = 1;
+  while (p2 < m) p2 <<= 1;
+  const uint64_t *p = cand + q * (long)YK_CAND_CAP;
+  for (int i = threadIdx.x; i < p2; i += blockDim.x) s[i] = i < m ? p[i] : YK_KEY_NONE;
+  __syncthreads();
+  yk_bitonic_b(s, p2);
+  if (n > YK_CAND_CAP) {
+    if (threadIdx.x == 0) {
+      thr[q] = s[kp - 1];
+      cnt[q] = 0;
+      atomicOr(flag, 1);
+    }
+    return;
+  }
+  for (int i = threadIdx.x; i < kp; i += blockDim.x) cand_out[q * kp + i] = i < p2 ? s[i] : YK_KEY_NONE;
+  __syncthreads();
+  if (threadIdx.x == 0) thr[q] = 0;
+}
+
+__global__ void yk_bf_zero_kernel(int *p, long n) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 re-rank (bit-identical to the FP32 form) + certificate.
+// One wave per query; MFMA rows = 32 candidates, every column the query.
+
+__global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
+    const float *__restrict__ q, const double *__restrict__ qn, const float *__restrict__ b,
+    const float *__restrict__ bn, long nb, int ld, int type, const uint64_t *__restrict__ cand,
+    int kp, int k, const float4 *__restrict__ qstats, const float4 *__restrict__ bmax, float gam_bf,
+    float gam_32, int *out_i, float *out_d, int *unc_list, int *n_unc) {
+  __shared__ uint64_t ex[YK_KMAX];
+  const long qi = blockIdx.x;
+  const int lane = threadIdx.x, r = lane & 31, h = lane >> 5;
+  const int nch = ld >> 4;
+  const bool dot = type == YK_DOT;
+  const uint64_t *cq = cand + qi * kp;
+  const float *qp = q + qi * ld + h * 8;
+  const double qnd = dot ? 0.0 : qn[qi];
+  for (int t0 = 0; t0 < kp; t0 += 32) {
+    const int ci = t0 + r;
+    const uint64_t key = ci < kp ? cq[ci] : YK_KEY_NONE;
+    const long row = key == YK_KEY_NONE ? 0 : (long)(uint32_t)key;
+    const float *bp = b + row * ld + h * 8;
+    f32x16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[e] = 0.f;
+    for (int c = 0; c < nch; c++) {
+      float4 a0 = *(const float4 *)(bp + c * 16), a1 = *(const float4 *)(bp + c * 16 + 4);
+      float4 b0 = *(const float4 *)(qp + c * 16), b1 = *(const float4 *)(qp + c * 16 + 4);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc, 0, 0, 0);
+    }
+    // column 0 (lanes 0 and 32) holds the 32 candidate rows 8g + 4h + e
+    if (r == 0) {
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int m = 8 * (i >> 2) + 4 * h + (i & 3);
+        const int cidx = t0 + m;
+        if (cidx >= kp) continue;
+        const uint64_t ck = cq[cidx];
+        uint64_t xk = YK_KEY_NONE;
+        if (ck != YK_KEY_NONE) {
+          const int bi = (int)(uint32_t)ck;
+          float v = dot ? acc[i] : (float)(qnd + (double)bn[bi]) - 2.0f * acc[i];
+          if (v == v) xk = yk_key(v, bi);
+        }
+        ex[cidx] = xk;
+      }
+    }
+  }
+  __syncthreads();
+  int p2 = 1;
+  while (p2 < kp) p2 <<= 1;
+  for (int i = kp + lane; i < p2; i += 64) ex[i] = YK_KEY_NONE;
+  __syncthreads();
+  yk_bitonic_b(ex, p2);
+  for (int i = lane; i < k; i += 64) {
+    uint64_t key = ex[i];
+    out_i[qi * k + i] = key == YK_KEY_NONE ? -1 : (int)(uint32_t)key;
+    out_d[qi * k + i] =
+        key == YK_KEY_NONE ? __uint_as_float(0xffffffffu) : yk_fkey_inv((uint32_t)(key >> 32));
+  }
+  if (lane == 0) {
+    // the first pass kept the kp smallest keys; fewer than kp real ones means
+    // every base vector was a candidate
+    const uint64_t last = cq[kp - 1];
+    bool ok;
+    if (last == YK_KEY_NONE || kp >= nb) {
+      ok = true;
+    } else if (ex[k - 1] == YK_KEY_NONE) {
+      ok = false;
+    } else {
+      const float Tb = yk_fkey_inv((uint32_t)(last >> 32));
+      const float Fk = yk_fkey_inv((uint32_t)(ex[k - 1] >> 32));
+      const float4 qs = qstats[qi], bs = *bmax;  // {|x|, |bf x|, |e|, |x|^2}
+      double ebf, e32;
+      if (dot) {
+        ebf = (double)qs.z * bs.x + (double)qs.y * bs.z + gam_bf * (double)qs.y * bs.y;
+        e32 = gam_32 * (double)qs.x * bs.x;
+      } else {
+        ebf = 2.0 * ((double)qs.z * bs.x + (double)qs.y * bs.z) +
+              gam_bf * (2.0 * qs.y * bs.y + qs.w + bs.w);
+        e32 = gam_32 * (2.0 * qs.x * bs.x + qs.w + bs.w);
+      }
+      ok = (double)Fk < (double)Tb - ebf - e32;
+    }
+    if (!ok) unc_list[atomicAdd(n_unc, 1)] = (int)qi;
+  }
+}
+
+// scatter FP32-fallback rows back: out[list[i]] = src[i]
+__global__ void yk_scatter_rows_kernel(const int *__restrict__ list, int n, int k,
+                                       const int *__restrict__ si, const float *__restrict__ sd,
+                                       int *oi, float *od) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)n * k) return;
+  long i = t / k, j = t % k;
+  long dst = (long)list[i] * k + j;
+  oi[dst] = si[t];
+  od[dst] = sd[t];
+}
+
+__global__ void yk_gather_f64_kernel(const double *__restrict__ src, const int *__restrict__ idx,
+                                     long n, double *dst) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+
+static void *g_bs[8];
+static size_t g_bs_bytes[8];
+
+static int yk_bscratch(int slot, size_t bytes, void **out) {
+  if (bytes > g_bs_bytes[slot]) {
+    if (g_bs[slot]) (void)hipFree(g_bs[slot]);
+    g_bs[slot] = nullptr;
+    g_bs_bytes[slot] = 0;
+    hipError_t e = hipMalloc(&g_bs[slot], bytes);
+    if (e != hipSuccess) return yk_hip_fail(e, "hipMalloc(bf16 scratch)");
+    g_bs_bytes[slot] = bytes;
+  }
+  *out = g_bs[slot];
+  return YK_OK;
+}
+
+struct BPlan {
+  int tiles, nsplit;
+  long brange, bstep;
+};
+
+static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, long max_split) {
+  BPlan p;
+  const long qt = 32L * nblk * 4;
+  p.tiles = (int)((nq + qt - 1) / qt);
+  p.bstep = 32 * sample;
+  const long nblocks = (nb + p.bstep - 1) / p.bstep;
+  long want = (1024 + p.tiles - 1) / p.tiles;
+  long maxs = (nblocks + 15) / 16;
+  if (want > maxs) want = maxs;
+  if (want < min_splits) want = min_splits;
+  if (want > max_split) want = max_split;
+  if (want > nblocks) want = nblocks;
+  if (want < 1) want = 1;
+  long per = (nblocks + want - 1) / want;
+  p.brange = per * p.bstep;
+  p.nsplit = (int)((nb + p.brange - 1) / p.brange);
+  return p;
+}
+
+template <int MODE, int R>
+static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
+  a.brange = p.brange;
+  a.bstep = p.bstep;
+  a.nsplit = p.nsplit;
+  a.tiles_x = p.tiles;
+  a.total = p.tiles * p.nsplit;
+  a.per_xcd = (a.total + 7) / 8;
+  int tok = yk_prof_begin(YK_PROF_BF16);
+  if (a.ldh <= 256)
+    hipLaunchKernelGGL((yk_scan_bf16<2, true, MODE, R>), dim3(a.per_xcd * 8), dim3(256), 0,
+                       yk_stream(), a);
+  else
+    hipLaunchKernelGGL((yk_scan_bf16<2, false, MODE, R>), dim3(a.per_xcd * 8), dim3(256), 0,
+                       yk_stream(), a);
+  yk_prof_end(tok);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
+// kp smallest first-pass keys per query -> cand [nq][kp]
+static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
+  const long nq = a.nq;
+  void *vpart;
+  if (kp <= YK_REG_R) {
+    BPlan p = yk_bplan(nq, a.nb, 2, 1, 1, 256);
+    YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * p.nsplit * 2 * YK_REG_R, &vpart));
+    a.part = (uint64_t *)vpart;
+    YK_TRY((yk_launch_bscan<2, YK_REG_R>(p, a)));
+    int n = p.nsplit * 2 * YK_REG_R, p2 = 1;
+    while (p2 < n) p2 <<= 1;
+    hipLaunchKernelGGL(yk_bf_sortsel_kernel, dim3((unsigned)nq), dim3(256), sizeof(uint64_t) * p2,
+                       yk_stream(), a.part, n, kp, p2, cand_out, (uint64_t *)nullptr);
+    YK_HIP(hipGetLastError());
+    return YK_OK;
+  }
+  void *vthr, *vcnt, *vcand;
+  YK_TRY(yk_bscratch(1, sizeof(uint64_t) * nq, &vthr));
+  YK_TRY(yk_bscratch(2, sizeof(int) * (nq + 64), &vcnt));
+  YK_TRY(yk_bscratch(3, sizeof(uint64_t) * nq * YK_CAND_CAP, &vcand));
+  uint64_t *thr = (uint64_t *)vthr;
+  int *cnt = (int *)vcnt, *flag = cnt + nq + 32;
+  long stride = YK_CAND_CAP / (4L * kp);
+  if (stride > 8) stride = 8;
+  if (stride < 1) stride = 1;
+  const long nblocks = (a.nb + 31) / 32;
+  while (stride > 1 && (nblocks / stride) * 32 < 4L * kp) stride >>= 1;
+  BPlan pa = yk_bplan(nq, a.nb, 2, stride, (2L * kp + 31) / 32, 256);
+  YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * pa.nsplit * 2 * YK_REG_R, &vpart));
+  a.part = (uint64_t *)vpart;
+  YK_TRY((yk_launch_bscan<2, YK_REG_R>(pa, a)));
+  int n = pa.nsplit * 2 * YK_REG_R, p2 = 1;
+  while (p2 < n) p2 <<= 1;
+  hipLaunchKernelGGL(yk_bf_sortsel_kernel, dim3((unsigned)nq), dim3(256), sizeof(uint64_t) * p2,
+                     yk_stream(), a.part, n, kp, p2, (uint64_t *)nullptr, thr);
+  hipLaunchKernelGGL(yk_bf_zero_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                     yk_stream(), cnt, nq);
+  hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), flag, 1L);
+  BPlan pb = yk_bplan(nq, a.nb, 2, 1, 1, 4096);
+  a.thr = thr;
+  a.cnt = cnt;
+  a.cand = (uint64_t *)vcand;
+  for (int tries = 0;; tries++) {
+    YK_TRY((yk_launch_bscan<3, 1>(pb, a)));
+    hipLaunchKernelGGL(yk_bf_select_kernel, dim3((unsigned)nq), dim3(256), 0, yk_stream(), a.cand,
+                       cnt, thr, kp, cand_out, flag);
+    YK_HIP(hipGetLastError());
+    int hflag = 0;
+    YK_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
+    YK_HIP(hipStreamSynchronize(yk_stream()));
+    if (!hflag) break;
+    if (tries == 5)
+      return yk_set_error(YK_EINVAL, "k'=%d: too many tied first-pass distances", kp);
+    hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), flag, 1L);
+  }
+  return YK_OK;
+}
+
+extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, const float *q,
+                           const double *qn, const uint16_t *qh, const float *qstats,
+                           const float *b, const float *bn, const uint16_t *bh,
+                           const float *bstats_max, int *assign, float *dis, int *n_fallback) {
+  YK_TRY(yk_init());
+  if (type == 12) type = YK_L2;
+  if (type != YK_L2 && type != YK_DOT)
+    return yk_set_error(YK_EINVAL, "distance type %d has no GPU path (2, 12, 16 only)", type);
+  if (kp > nb) kp = (int)nb;
+  if (kp < k) kp = k;
+  if (k < 1 || k > nb || kp > YK_KMAX)
+    return yk_set_error(YK_EINVAL, "k=%d k'=%d out of range (nb=%ld, max %d)", k, kp, nb, YK_KMAX);
+  if (nq <= 0) return YK_OK;
+  const long QB = 16384;
+  void *vcand, *vunc;
+  const long qb = nq < QB ? nq : QB;
+  YK_TRY(yk_bscratch(4, sizeof(uint64_t) * qb * kp, &vcand));
+  YK_TRY(yk_bscratch(5, sizeof(int) * (qb + 64), &vunc));
+  int *unc = (int *)vunc, *nunc = unc + qb + 32;
+  const int ld = yk_ld(d), ldh = yk_bf16_ld(d, type);
+  const float gam_bf = 2.0f * (ldh + 8) * 0x1p-23f, gam_32 = 2.0f * (ld + 8) * 0x1p-23f;
+  int total_fb = 0;
+  for (long q0 = 0; q0 < nq; q0 += QB) {
+    const long n = nq - q0 < QB ? nq - q0 : QB;
+    Bf16Args a = {};
+    a.q = qh + q0 * ldh;
+    a.nq = n;
+    a.b = bh;
+    a.nb = nb;
+    a.ldh = ldh;
+    YK_TRY(yk_bf16_candidates(a, kp, (uint64_t *)vcand));
+    hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), nunc, 1L);
+    int tok = yk_prof_begin(YK_PROF_RERANK);
+    hipLaunchKernelGGL(yk_rerank_cert_kernel, dim3((unsigned)n), dim3(64), 0, yk_stream(),
+                       q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, nb, ld, type,
+                       (const uint64_t *)vcand, kp, k, (const float4 *)qstats + q0,
+                       (const float4 *)bstats_max, gam_bf, gam_32, assign + q0 * k, dis + q0 * k,
+                       unc, nunc);
+    yk_prof_end(tok);
+    YK_HIP(hipGetLastError());
+    int hn = 0;
+    YK_HIP(hipMemcpyAsync(&hn, nunc, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
+    YK_HIP(hipStreamSynchronize(yk_stream()));
+    if (hn > 0) {
+      // uncertified queries: the FP32 form on their rows
+      void *vq, *vqn, *voi, *vod;
+      YK_TRY(yk_bscratch(6, sizeof(float) * (size_t)hn * ld + sizeof(double) * hn + 256, &vq));
+      YK_TRY(yk_bscratch(7, (sizeof(int) + sizeof(float)) * (size_t)hn * k + 256, &voi));
+      float *fq = (float *)vq;
+      double *fqn = (double *)(fq + (size_t)hn * ld);
+      vqn = fqn;
+      int *foi = (int *)voi;
+      float *fod = (float *)(foi + (size_t)hn * k);
+      vod = fod;
+      YK_TRY(yk_gather_rows(q + q0 * ld, unc, hn, d, fq));
+      if (type != YK_DOT)
+        hipLaunchKernelGGL(yk_gather_f64_kernel, dim3((hn + 255) / 256), dim3(256), 0, yk_stream(),
+                           qn + q0, unc, (long)hn, (double *)vqn);
+      YK_TRY(yk_knn_f32(type, hn, nb, d, k, fq, type == YK_DOT ? nullptr : (double *)vqn, b, bn,
+                        nullptr, foi, (float *)vod));
+      long tot = (long)hn * k;
+      hipLaunchKernelGGL(yk_scatter_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                         yk_stream(), unc, hn, k, foi, fod, assign + q0 * k, dis + q0 * k);
+      YK_HIP(hipGetLastError());
+      total_fb += hn;
+    }
+  }
+  if (n_fallback) *n_fallback = total_fb;
+  return YK_OK;
+}

@@ -1,0 +1,102 @@
+"""Device-resident vector sets and searches over the C-ABI (yk_* functions).
+
+A ``DeviceSet`` holds n vectors in HBM in the library's padded layout (row
+stride yk_ld(d) floats), with the norms the FP32 form needs and, on demand,
+the BF16 copy and error statistics of the precision form.  Searches run on
+the library's HIP stream and return numpy arrays.
+"""
+import ctypes
+
+import numpy as np
+
+from . import lib, check
+
+vp = ctypes.c_void_p
+
+
+class DeviceBuffer:
+    def __init__(self, nbytes):
+        self.ptr = vp()
+        self.nbytes = int(nbytes)
+        check(lib().yk_malloc(ctypes.byref(self.ptr), max(256, self.nbytes)), "yk_malloc")
+
+    def free(self):
+        if self.ptr:
+            lib().yk_free(self.ptr)
+            self.ptr = vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def to_host(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        check(lib().yk_memcpy_d2h(out.ctypes.data_as(vp), self.ptr, out.nbytes), "d2h")
+        return out
+
+
+class DeviceSet:
+    """n vectors of dimension d resident in HBM (float32, padded rows)."""
+
+    def __init__(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        self.n, self.d = x.shape
+        L = lib()
+        self.ld = L.yk_ld(self.d)
+        self.vec = DeviceBuffer(self.n * self.ld * 4)
+        check(L.yk_upload_vectors(self.vec.ptr, x.ctypes.data_as(vp), self.n, self.d), "upload")
+        self.fnorm = DeviceBuffer(((self.n + 31) // 32 * 32) * 4)  # float-accumulated (base role)
+        self.dnorm = DeviceBuffer(self.n * 8)                      # double-accumulated (query role)
+        self.bf = {}
+
+    def compute_norms(self):
+        check(lib().yk_norms(self.vec.ptr, self.n, self.d, self.fnorm.ptr, self.dnorm.ptr), "norms")
+
+    def compute_bf16(self, typ, is_query):
+        """BF16 copy (+ error statistics) for the precision form."""
+        key = (typ, bool(is_query))
+        if key not in self.bf:
+            ldh = lib().yk_bf16_ld(self.d, typ)
+            self.bf[key] = (DeviceBuffer(self.n * ldh * 2), DeviceBuffer(self.n * 16), DeviceBuffer(16))
+        h, st, mx = self.bf[key]
+        check(lib().yk_to_bf16(self.vec.ptr, self.n, self.d, typ, int(is_query), h.ptr, st.ptr,
+                               mx.ptr), "to_bf16")
+        return h, st, mx
+
+    def free(self):
+        for b in (self.vec, self.fnorm, self.dnorm):
+            b.free()
+        for t in self.bf.values():
+            for b in t:
+                b.free()
+
+
+def knn_fp32(queries, base, k, typ=2, norms=True):
+    """FP32 form: (idx, dis) of shape (nq, k).  queries/base are DeviceSets."""
+    L = lib()
+    if norms:
+        queries.compute_norms()
+        base.compute_norms()
+    oi, od = DeviceBuffer(queries.n * k * 4), DeviceBuffer(queries.n * k * 4)
+    check(L.yk_knn_f32(typ, queries.n, base.n, base.d, k, queries.vec.ptr, queries.dnorm.ptr,
+                       base.vec.ptr, base.fnorm.ptr, None, oi.ptr, od.ptr), "knn_f32")
+    return oi.to_host(np.int32, (queries.n, k)), od.to_host(np.float32, (queries.n, k))
+
+
+def knn_bf16(queries, base, k, kprime, typ=2, prepare=True):
+    """Precision form: BF16 first pass, FP32 re-rank of kprime candidates,
+    certificate, FP32 fallback.  Returns (idx, dis, n_fallback)."""
+    L = lib()
+    if prepare:
+        queries.compute_norms()
+        base.compute_norms()
+    qh, qst, _ = queries.compute_bf16(typ, True) if prepare else queries.bf[(typ, True)]
+    bh, _, bmx = base.compute_bf16(typ, False) if prepare else base.bf[(typ, False)]
+    oi, od = DeviceBuffer(queries.n * k * 4), DeviceBuffer(queries.n * k * 4)
+    nfb = ctypes.c_int(0)
+    check(L.yk_knn_bf16(typ, queries.n, base.n, base.d, k, kprime, queries.vec.ptr,
+                        queries.dnorm.ptr, qh.ptr, qst.ptr, base.vec.ptr, base.fnorm.ptr, bh.ptr,
+                        bmx.ptr, oi.ptr, od.ptr, ctypes.byref(nfb)), "knn_bf16")
+    return oi.to_host(np.int32, (queries.n, k)), od.to_host(np.float32, (queries.n, k)), nfb.value
