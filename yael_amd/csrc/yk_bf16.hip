@@ -55,18 +55,25 @@ extern "C" int yk_bf16_ld(int d, int type) {
   return (d + extra + 15) & ~15;
 }
 
-// One thread per vector: BF16 row (augmented for L2) + error statistics
-// stats[i] = {|x|, |bf(x)|, |x - bf(x)|, norm} (float).
-__global__ void yk_to_bf16_kernel(const float *__restrict__ x, long n, int d, int ld, int ldh,
-                                  int type, int is_query, uint16_t *__restrict__ out,
-                                  float4 *__restrict__ stats) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// One wave per vector (coalesced): BF16 row (augmented for L2) + error
+// statistics stats[i] = {|x|, |bf(x)|, |x - bf(x)|, |x|^2} (float); the
+// per-set maxima go to smax (uint bit patterns of non-negative floats, whose
+// integer order is their float order) by atomicMax.
+__global__ __launch_bounds__(256) void yk_to_bf16_kernel(const float *__restrict__ x, long n,
+                                                         int d, int ld, int ldh, int type,
+                                                         int is_query,
+                                                         uint16_t *__restrict__ out,
+                                                         float4 *__restrict__ stats,
+                                                         unsigned *smax) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ float4 wmax[4];
+  float4 mx = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (long i = blockIdx.x * 4L + wave; i < n; i += gridDim.x * 4L) {
   const float *row = x + i * ld;
   uint16_t *o = out + i * ldh;
   double s2 = 0, sb = 0, se = 0;
   const float scale = (type != YK_DOT && is_query) ? -2.0f : 1.0f;
-  for (int j = 0; j < d; j++) {
+  for (int j = lane; j < d; j += 64) {
     float v = row[j];
     uint16_t hb = yk_bf16_rne(v * scale);
     float vb = yk_bf16_f(hb) / scale;  // exact: scale is a power of two
@@ -75,7 +82,14 @@ __global__ void yk_to_bf16_kernel(const float *__restrict__ x, long n, int d, in
     sb += (double)vb * vb;
     se += ((double)v - vb) * ((double)v - vb);
   }
-  int j = d;
+  for (int off = 32; off > 0; off >>= 1) {
+    s2 += __shfl_xor(s2, off);
+    sb += __shfl_xor(sb, off);
+    se += __shfl_xor(se, off);
+  }
+  const uint16_t one = 0x3f80;
+  uint16_t tail[6] = {0, 0, 0, 0, 0, 0};
+  int extra = 0;
   if (type != YK_DOT) {
     float nf = (float)s2;
     uint16_t h1 = yk_bf16_rne(nf);
@@ -83,38 +97,33 @@ __global__ void yk_to_bf16_kernel(const float *__restrict__ x, long n, int d, in
     uint16_t h2 = yk_bf16_rne(r1);
     float r2 = r1 - yk_bf16_f(h2);
     uint16_t h3 = yk_bf16_rne(r2);
-    const uint16_t one = 0x3f80;
     if (is_query) {
-      o[j++] = one; o[j++] = one; o[j++] = one;
-      o[j++] = h1; o[j++] = h2; o[j++] = h3;
+      tail[0] = one; tail[1] = one; tail[2] = one; tail[3] = h1; tail[4] = h2; tail[5] = h3;
     } else {
-      o[j++] = h1; o[j++] = h2; o[j++] = h3;
-      o[j++] = one; o[j++] = one; o[j++] = one;
+      tail[0] = h1; tail[1] = h2; tail[2] = h3; tail[3] = one; tail[4] = one; tail[5] = one;
     }
+    extra = 6;
   }
-  for (; j < ldh; j++) o[j] = 0;
-  stats[i] = make_float4((float)sqrt(s2), (float)sqrt(sb), (float)sqrt(se), (float)s2);
+  for (int j = d + lane; j < ldh; j += 64) o[j] = (j - d) < extra ? tail[j - d] : 0;
+  float4 s = make_float4((float)sqrt(s2), (float)sqrt(sb), (float)sqrt(se), (float)s2);
+  if (lane == 0) stats[i] = s;
+  mx = make_float4(fmaxf(mx.x, s.x), fmaxf(mx.y, s.y), fmaxf(mx.z, s.z), fmaxf(mx.w, s.w));
+  }  // rows
+  if (lane == 0) wmax[wave] = mx;
+  __syncthreads();
+  if (smax && threadIdx.x == 0) {
+    for (int w = 1; w < 4; w++)
+      mx = make_float4(fmaxf(mx.x, wmax[w].x), fmaxf(mx.y, wmax[w].y), fmaxf(mx.z, wmax[w].z),
+                       fmaxf(mx.w, wmax[w].w));
+    atomicMax(smax + 0, __float_as_uint(mx.x));
+    atomicMax(smax + 1, __float_as_uint(mx.y));
+    atomicMax(smax + 2, __float_as_uint(mx.z));
+    atomicMax(smax + 3, __float_as_uint(mx.w));
+  }
 }
 
-// per-set maxima of the statistics (for the certificate), one workgroup
-__global__ __launch_bounds__(256) void yk_stats_max_kernel(const float4 *__restrict__ st, long n,
-                                                           float4 *out) {
-  __shared__ float4 s[256];
-  float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (long i = threadIdx.x; i < n; i += 256) {
-    float4 v = st[i];
-    m.x = fmaxf(m.x, v.x); m.y = fmaxf(m.y, v.y); m.z = fmaxf(m.z, v.z); m.w = fmaxf(m.w, v.w);
-  }
-  s[threadIdx.x] = m;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) {
-      float4 a = s[threadIdx.x], b = s[threadIdx.x + off];
-      s[threadIdx.x] = make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), fmaxf(a.w, b.w));
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *out = s[0];
+__global__ void yk_zero4_kernel(unsigned *p) {
+  if (threadIdx.x < 4) p[threadIdx.x] = 0;
 }
 
 extern "C" int yk_to_bf16(const float *x, long n, int d, int type, int is_query, uint16_t *out,
@@ -123,11 +132,13 @@ extern "C" int yk_to_bf16(const float *x, long n, int d, int type, int is_query,
   if (type == 12) type = YK_L2;
   if (n <= 0) return YK_OK;
   const int ld = yk_ld(d), ldh = yk_bf16_ld(d, type);
-  hipLaunchKernelGGL(yk_to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     yk_stream(), x, n, d, ld, ldh, type, is_query, out, (float4 *)stats);
-  if (stats_max)
-    hipLaunchKernelGGL(yk_stats_max_kernel, dim3(1), dim3(256), 0, yk_stream(),
-                       (const float4 *)stats, n, (float4 *)stats_max);
+  if (stats_max) hipLaunchKernelGGL(yk_zero4_kernel, dim3(1), dim3(64), 0, yk_stream(),
+                                    (unsigned *)stats_max);
+  long blocks = (n + 3) / 4;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(yk_to_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, yk_stream(),
+                     x, n, d, ld, ldh, type, is_query, out, (float4 *)stats,
+                     (unsigned *)stats_max);
   YK_HIP(hipGetLastError());
   return YK_OK;
 }
@@ -159,13 +170,17 @@ __device__ __forceinline__ void yk_reg_insert16(uint64_t (&t)[R], uint64_t x) {
   t[0] = x < t[0] ? x : t[0];
 }
 
-template <int NB, bool QRES, int MODE, int R>
+// NCHMAX > 0: queries register-resident, at most NCHMAX 16-wide chunks
+// (ldh <= 16 * NCHMAX); NCHMAX = 0: both operands streamed from memory.
+template <int NB, int NCHMAX, int MODE, int R>
 __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
+  constexpr bool QRES = NCHMAX > 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int r = lane & 31, h = lane >> 5;
-  const int id = blockIdx.x;
-  const int item = (id & 7) * a.per_xcd + (id >> 3);
-  if (item >= a.total) return;
+  // persistent grid, XCD-contiguous runs of work items (as in yk_select.hip)
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, wpx = gridDim.x >> 3;
+  const int ibeg = xcd * a.per_xcd, iend = min(a.total, ibeg + a.per_xcd);
+  for (int item = ibeg + local; item < iend; item += wpx) {
   const int tile = item % a.tiles_x, split = item / a.tiles_x;
   const long qbase = ((long)tile * nwave + wave) * (32 * NB);
   const long bbeg = (long)split * a.brange;
@@ -176,6 +191,13 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
   long qrow[NB];
   bool qok[NB];
   uint64_t top[NB][MODE == 2 ? R : 1];
+  float bv[NB];
+  int bi[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    bv[j] = INFINITY;
+    bi[j] = -1;
+  }
   uint64_t T[NB];
   float Tv[NB];
 #pragma unroll
@@ -193,18 +215,27 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
     bool act = false;
 #pragma unroll
     for (int j = 0; j < NB; j++) act |= qok[j] && T[j] != 0;
-    if (!__any(act)) return;
+    if (!__any(act)) continue;
   }
 
-  bf16x8 qreg[NB][QRES ? 16 : 1];
+  bf16x8 qreg[NB][QRES ? NCHMAX : 1];
   if (QRES) {
 #pragma unroll
     for (int j = 0; j < NB; j++)
 #pragma unroll
-      for (int c = 0; c < 16; c++)
+      for (int c = 0; c < NCHMAX; c++)
         if (c < nch) qreg[j][c] = *(const bf16x8 *)(a.q + qrow[j] * ldh + c * 16 + h * 8);
   }
 
+  // base operand of the next block is loaded while the current one is
+  // multiplied (register double buffer; QRES only, d' <= 256)
+  bf16x8 anext[QRES ? NCHMAX : 1];
+  if (QRES && bbeg < bend) {
+    const uint16_t *bp = a.b + min(bbeg + r, a.nb - 1) * ldh + h * 8;
+#pragma unroll
+    for (int c = 0; c < NCHMAX; c++)
+      if (c < nch) anext[c] = *(const bf16x8 *)(bp + c * 16);
+  }
   for (long bb = bbeg; bb < bend; bb += a.bstep) {
     const long brow = min(bb + r, a.nb - 1);
     const uint16_t *bp = a.b + brow * ldh + h * 8;
@@ -214,12 +245,18 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
 #pragma unroll
       for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
     if (QRES) {
-      bf16x8 av[16];
+      bf16x8 av[QRES ? NCHMAX : 1];
 #pragma unroll
-      for (int c = 0; c < 16; c++)
-        if (c < nch) av[c] = *(const bf16x8 *)(bp + c * 16);
+      for (int c = 0; c < NCHMAX; c++) av[c] = anext[c];
+      const long nxt = bb + a.bstep;
+      if (nxt < bend) {
+        const uint16_t *np = a.b + min(nxt + r, a.nb - 1) * ldh + h * 8;
 #pragma unroll
-      for (int c = 0; c < 16; c++)
+        for (int c = 0; c < NCHMAX; c++)
+          if (c < nch) anext[c] = *(const bf16x8 *)(np + c * 16);
+      }
+#pragma unroll
+      for (int c = 0; c < NCHMAX; c++)
         if (c < nch) {
 #pragma unroll
           for (int j = 0; j < NB; j++)
@@ -236,14 +273,49 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
         }
       }
     }
+    // epilogue: a min over the lane's 16 values decides whether any of them
+    // can enter (fminf drops NaN, which never enters); only then per value
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-      const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
-      const bool bok = bidx < bend;
+    for (int j = 0; j < NB; j++) {
+      float m = acc[j][0];
 #pragma unroll
-      for (int j = 0; j < NB; j++) {
+      for (int i = 1; i < 16; i++) m = fminf(m, acc[j][i]);
+      bool maybe;
+      if (MODE == 1)
+        maybe = m < bv[j];
+      else if (MODE == 2)
+        maybe = top[j][R - 1] == YK_KEY_NONE || m <= yk_fkey_inv((uint32_t)(top[j][R - 1] >> 32));
+      else
+        maybe = m <= Tv[j];
+      if (!maybe) continue;
+      if (MODE == 3) {
+        // one compare per value; the (rare) append behind a branch
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+          const float v = acc[j][i];
+          if (!(v <= Tv[j])) continue;
+          const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
+          if (bidx < bend && qok[j]) {
+            const uint64_t key = yk_key(v, (int)bidx);
+            if (key <= T[j]) {
+              int slot = atomicAdd(&a.cnt[qrow[j]], 1);
+              if (slot < YK_CAND_CAP) a.cand[qrow[j] * YK_CAND_CAP + slot] = key;
+            }
+          }
+        }
+        continue;
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
+        const bool bok = bidx < bend;
         const float v = acc[j][i];
-        if (MODE == 2) {
+        if (MODE == 1) {
+          if (bok && v < bv[j]) {
+            bv[j] = v;
+            bi[j] = (int)bidx;
+          }
+        } else if (MODE == 2) {
           uint64_t key = (bok && v == v) ? yk_key(v, (int)bidx) : YK_KEY_NONE;
           if (key < top[j][R - 1]) yk_reg_insert16<R>(top[j], key);
         } else {
@@ -258,6 +330,14 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
       }
     }
   }
+  if (MODE == 1) {
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const long qi = qbase + j * 32 + r;
+      if (qok[j])
+        a.part[(qi * a.nsplit + split) * 2 + h] = bi[j] < 0 ? YK_KEY_NONE : yk_key(bv[j], bi[j]);
+    }
+  }
   if (MODE == 2) {
 #pragma unroll
     for (int j = 0; j < NB; j++) {
@@ -268,6 +348,7 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
       for (int s = 0; s < R; s++) dst[s] = top[j][s];
     }
   }
+  }  // work items
 }
 
 // ---------------------------------------------------------------------------
@@ -484,7 +565,7 @@ static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, 
   p.tiles = (int)((nq + qt - 1) / qt);
   p.bstep = 32 * sample;
   const long nblocks = (nb + p.bstep - 1) / p.bstep;
-  long want = (1024 + p.tiles - 1) / p.tiles;
+  long want = (4096 + p.tiles - 1) / p.tiles;  // ~8 items per resident workgroup
   long maxs = (nblocks + 15) / 16;
   if (want > maxs) want = maxs;
   if (want < min_splits) want = min_splits;
@@ -499,19 +580,30 @@ static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, 
 
 template <int MODE, int R>
 static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
+  // queries per wave: 128 for the filtered pass at d' <= 144 (halves the
+  // base-operand traffic per MFMA), else 64
+  const int nbq = ((MODE == 1 || MODE == 3) && a.ldh <= 144 && a.ldh > 64) ? 4 : 2;
   a.brange = p.brange;
   a.bstep = p.bstep;
   a.nsplit = p.nsplit;
-  a.tiles_x = p.tiles;
-  a.total = p.tiles * p.nsplit;
+  a.tiles_x = (int)((a.nq + 128L * nbq - 1) / (128L * nbq));
+  a.total = a.tiles_x * p.nsplit;
   a.per_xcd = (a.total + 7) / 8;
-  int tok = yk_prof_begin(YK_PROF_BF16);
-  if (a.ldh <= 256)
-    hipLaunchKernelGGL((yk_scan_bf16<2, true, MODE, R>), dim3(a.per_xcd * 8), dim3(256), 0,
-                       yk_stream(), a);
+  void (*kern)(Bf16Args);
+  if (a.ldh <= 64)
+    kern = yk_scan_bf16<2, 4, MODE, R>;
+  else if (a.ldh <= 144 && nbq == 4)
+    kern = yk_scan_bf16<4, 9, MODE, R>;
+  else if (a.ldh <= 144)
+    kern = yk_scan_bf16<2, 9, MODE, R>;
+  else if (a.ldh <= 256)
+    kern = yk_scan_bf16<2, 16, MODE, R>;
   else
-    hipLaunchKernelGGL((yk_scan_bf16<2, false, MODE, R>), dim3(a.per_xcd * 8), dim3(256), 0,
-                       yk_stream(), a);
+    kern = yk_scan_bf16<2, 0, MODE, R>;
+  const int per_res = yk_resident_wgs((const void *)kern, 256) / 8;
+  const dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), blk(256);
+  int tok = yk_prof_begin(YK_PROF_BF16);
+  hipLaunchKernelGGL(kern, grid, blk, 0, yk_stream(), a);
   yk_prof_end(tok);
   YK_HIP(hipGetLastError());
   return YK_OK;
@@ -539,16 +631,18 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   YK_TRY(yk_bscratch(3, sizeof(uint64_t) * nq * YK_CAND_CAP, &vcand));
   uint64_t *thr = (uint64_t *)vthr;
   int *cnt = (int *)vcnt, *flag = cnt + nq + 32;
-  long stride = YK_CAND_CAP / (4L * kp);
+  // sketch: per-lane minima over a 1-in-stride sample, >= kp splits (same
+  // reasoning as the FP32 form in yk_select.hip)
+  long stride = 600L / kp;
   if (stride > 8) stride = 8;
   if (stride < 1) stride = 1;
   const long nblocks = (a.nb + 31) / 32;
-  while (stride > 1 && (nblocks / stride) * 32 < 4L * kp) stride >>= 1;
-  BPlan pa = yk_bplan(nq, a.nb, 2, stride, (2L * kp + 31) / 32, 256);
-  YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * pa.nsplit * 2 * YK_REG_R, &vpart));
+  while (stride > 1 && nblocks / stride < 2L * kp) stride >>= 1;
+  BPlan pa = yk_bplan(nq, a.nb, 2, stride, kp, kp > 256 ? kp : 256);
+  YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * pa.nsplit * 2, &vpart));
   a.part = (uint64_t *)vpart;
-  YK_TRY((yk_launch_bscan<2, YK_REG_R>(pa, a)));
-  int n = pa.nsplit * 2 * YK_REG_R, p2 = 1;
+  YK_TRY((yk_launch_bscan<1, 1>(pa, a)));
+  int n = pa.nsplit * 2, p2 = 1;
   while (p2 < n) p2 <<= 1;
   hipLaunchKernelGGL(yk_bf_sortsel_kernel, dim3((unsigned)nq), dim3(256), sizeof(uint64_t) * p2,
                      yk_stream(), a.part, n, kp, p2, (uint64_t *)nullptr, thr);

@@ -11,6 +11,9 @@ int yk_set_error(int code, const char *fmt, ...);
 int yk_hip_fail(hipError_t e, const char *what);
 hipStream_t yk_stream(void);
 int yk_prof_setup(void);
+// Workgroups of `block` threads that fit on the whole device at once for
+// kernel `fn` (occupancy x CU count), at least 8 and a multiple of 8.
+int yk_resident_wgs(const void *fn, int block);
 int yk_prof_begin(int slot);
 void yk_prof_end(int tok);
 

@@ -261,3 +261,38 @@ extern "C" int yk_prof_get(int which, double *total_ms, long *count) {
   *count = g_prof_cnt[which];
   return YK_OK;
 }
+
+// ---------------------------------------------------------------------------
+// persistent-grid sizing
+
+#define YK_OCC_CACHE 64
+static const void *g_occ_fn[YK_OCC_CACHE];
+static int g_occ_block[YK_OCC_CACHE], g_occ_val[YK_OCC_CACHE], g_occ_n = 0;
+static int g_cu_count = 0;
+static pthread_mutex_t g_occ_mu = PTHREAD_MUTEX_INITIALIZER;
+
+int yk_resident_wgs(const void *fn, int block) {
+  pthread_mutex_lock(&g_occ_mu);
+  int res = 0;
+  for (int i = 0; i < g_occ_n; i++)
+    if (g_occ_fn[i] == fn && g_occ_block[i] == block) res = g_occ_val[i];
+  if (!res) {
+    if (!g_cu_count) {
+      hipDeviceProp_t p;
+      g_cu_count = hipGetDeviceProperties(&p, 0) == hipSuccess ? p.multiProcessorCount : 256;
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0) != hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    res = per_cu * g_cu_count;
+    res = res < 8 ? 8 : res & ~7;
+    if (g_occ_n < YK_OCC_CACHE) {
+      g_occ_fn[g_occ_n] = fn;
+      g_occ_block[g_occ_n] = block;
+      g_occ_val[g_occ_n++] = res;
+    }
+  }
+  pthread_mutex_unlock(&g_occ_mu);
+  return res;
+}

@@ -96,8 +96,13 @@ template <int NB, bool QRES, int MODE, int R>
 __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int r = lane & 31, h = lane >> 5;
-  const int item = yk_xcd_item(a.per_xcd);
-  if (item >= a.total) return;
+  // Persistent grid: the hardware deals block ids to the 8 XCDs in turn;
+  // XCD x owns the contiguous run of work items [x*per_xcd, (x+1)*per_xcd)
+  // (split-major, so concurrently running workgroups of one XCD share a base
+  // range in its L2), and its workgroups stride through that run.
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, wpx = gridDim.x >> 3;
+  const int ibeg = xcd * a.per_xcd, iend = min(a.total, ibeg + a.per_xcd);
+  for (int item = ibeg + local; item < iend; item += wpx) {
   const int tile = item % a.tiles_x, split = item / a.tiles_x;
   const long qbase = ((long)tile * nwave + wave) * (32 * NB);
   const long bbeg = (long)split * a.brange;
@@ -122,6 +127,7 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a) {
   int bi[NB];
   uint64_t top[NB][MODE == 2 ? R : 1];
   uint64_t T[NB];
+  float Tv[NB];
 #pragma unroll
   for (int j = 0; j < NB; j++) {
     bv[j] = 1e30f;
@@ -129,12 +135,14 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a) {
 #pragma unroll
     for (int s = 0; s < (MODE == 2 ? R : 1); s++) top[j][s] = YK_KEY_NONE;
     T[j] = MODE == 3 ? a.thr[qrow[j]] : 0;
+    Tv[j] = (T[j] == 0) ? -INFINITY
+                        : (T[j] == YK_KEY_NONE ? INFINITY : yk_fkey_inv((uint32_t)(T[j] >> 32)));
   }
   if (MODE == 3) {
     bool act = false;
 #pragma unroll
     for (int j = 0; j < NB; j++) act |= qok[j] && T[j] != 0;
-    if (!__any(act)) return;  // every query of this wave is already final
+    if (!__any(act)) continue;  // every query of this wave is already final
   }
 
   float4 qreg[NB][QRES ? 16 : 1];
@@ -150,6 +158,18 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a) {
         }
   }
 
+  // the next block's base operand is loaded while the current one is
+  // multiplied (register double buffer, QRES only)
+  float4 anext[QRES ? 16 : 1];
+  if (QRES && bbeg < bend) {
+    const float *np = a.b + min(bbeg + r, a.nb - 1) * ld + h * 8;
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+      if (c < nch) {
+        anext[2 * c] = *(const float4 *)(np + c * 16);
+        anext[2 * c + 1] = *(const float4 *)(np + c * 16 + 4);
+      }
+  }
   for (long bb = bbeg; bb < bend; bb += a.bstep) {
     const long brow = min(bb + r, a.nb - 1);
     const float *bp = a.b + brow * ld + h * 8;
@@ -162,11 +182,16 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a) {
     if (QRES) {
       float4 av[16];
 #pragma unroll
-      for (int c = 0; c < 8; c++)
-        if (c < nch) {
-          av[2 * c] = *(const float4 *)(bp + c * 16);
-          av[2 * c + 1] = *(const float4 *)(bp + c * 16 + 4);
-        }
+      for (int c = 0; c < 16; c++) av[c] = anext[c];
+      if (bb + a.bstep < bend) {
+        const float *np = a.b + min(bb + a.bstep + r, a.nb - 1) * ld + h * 8;
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+          if (c < nch) {
+            anext[2 * c] = *(const float4 *)(np + c * 16);
+            anext[2 * c + 1] = *(const float4 *)(np + c * 16 + 4);
+          }
+      }
 #pragma unroll
       for (int c = 0; c < 8; c++)
         if (c < nch) {
@@ -205,42 +230,59 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a) {
     }
 
     // epilogue: lane holds query column r and base rows 8g + 4h + e
+    float nn[16], ww[16];
 #pragma unroll
     for (int g = 0; g < 4; g++) {
       const long b4 = bb + 8 * g + 4 * h;
       float4 n4 = dot ? make_float4(0.f, 0.f, 0.f, 0.f) : *(const float4 *)(a.bn + b4);
       float4 w4 = a.w ? *(const float4 *)(a.w + b4) : make_float4(1.f, 1.f, 1.f, 1.f);
-      const float nn[4] = {n4.x, n4.y, n4.z, n4.w};
-      const float ww[4] = {w4.x, w4.y, w4.z, w4.w};
+      nn[4 * g] = n4.x; nn[4 * g + 1] = n4.y; nn[4 * g + 2] = n4.z; nn[4 * g + 3] = n4.w;
+      ww[4 * g] = w4.x; ww[4 * g + 1] = w4.y; ww[4 * g + 2] = w4.z; ww[4 * g + 3] = w4.w;
+    }
+    float nmax = 0.f;
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int i = 4 * g + e;
-        const long bidx = b4 + e;
+    for (int i = 0; i < 16; i++) nmax = fmaxf(nmax, nn[i]);
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      // Prefilter on float approximations qn + (bn - 2 acc), which differ
+      // from the exact expression by far less than 2^-19 (|q|^2 + max|b|^2)
+      // (|2 q.b| <= |q|^2 + |b|^2): a min over the lane's 16 values, then a
+      // per-value test; only values that may matter take the exact
+      // FP64-normed expression.  Weighted searches evaluate every value.
+      const float qf = (float)qnd[j];
+      const float slack = dot ? 0.f : 0x1p-19f * (qf + nmax);
+      float ap[16];
+#pragma unroll
+      for (int i = 0; i < 16; i++)
+        ap[i] = dot ? acc[j][i] : qf + __builtin_fmaf(-2.0f, acc[j][i], nn[i]);
+      const float lim = MODE == 3 ? Tv[j] : (MODE == 2 ? INFINITY : bv[j]);
+      if (!a.w && MODE != 2) {
+        float m = ap[0];
+#pragma unroll
+        for (int i = 1; i < 16; i++) m = fminf(m, ap[i]);
+        if (!(m - slack <= lim) && m == m) continue;
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        if (!a.w && MODE != 2 && !(ap[i] - slack <= (MODE == 3 ? Tv[j] : bv[j])) && ap[i] == ap[i])
+          continue;
+        const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
         const bool bok = bidx < bend;
-#pragma unroll
-        for (int j = 0; j < NB; j++) {
-          float v;
-          if (dot)
-            v = acc[j][i];
-          else
-            v = (float)(qnd[j] + (double)nn[e]) - 2.0f * acc[j][i];
-          if (a.w) v *= ww[e];
-          if (MODE == 0) {
-            if (bok && v < bv[j]) {  // strict '<' in ascending index order
-              bv[j] = v;
-              bi[j] = (int)bidx;
-            }
-          } else if (MODE == 2) {
-            uint64_t key = (bok && v == v) ? yk_key(v, (int)bidx) : YK_KEY_NONE;
-            if (key < top[j][R - 1]) yk_reg_insert<R>(top[j], key);
-          } else {
-            if (bok && qok[j] && v == v) {
-              uint64_t key = yk_key(v, (int)bidx);
-              if (key <= T[j]) {
-                int slot = atomicAdd(&a.cnt[qrow[j]], 1);
-                if (slot < YK_CAND_CAP) a.cand[qrow[j] * YK_CAND_CAP + slot] = key;
-              }
-            }
+        float v = dot ? acc[j][i] : (float)(qnd[j] + (double)nn[i]) - 2.0f * acc[j][i];
+        if (a.w) v *= ww[i];
+        if (MODE == 0 || MODE == 1) {
+          if (bok && v < bv[j]) {  // strict '<' in ascending index order
+            bv[j] = v;
+            bi[j] = (int)bidx;
+          }
+        } else if (MODE == 2) {
+          uint64_t key = (bok && v == v) ? yk_key(v, (int)bidx) : YK_KEY_NONE;
+          if (key < top[j][R - 1]) yk_reg_insert<R>(top[j], key);
+        } else if (bok && qok[j] && v == v) {
+          uint64_t key = yk_key(v, (int)bidx);
+          if (key <= T[j]) {
+            int slot = atomicAdd(&a.cnt[qrow[j]], 1);
+            if (slot < YK_CAND_CAP) a.cand[qrow[j] * YK_CAND_CAP + slot] = key;
           }
         }
       }
@@ -266,6 +308,14 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a) {
         }
       }
     }
+  } else if (MODE == 1) {
+    // sketch: each lane's own minimum (distinct rows per lane -> distinct keys)
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const long qi = qbase + j * 32 + r;
+      if (qok[j])
+        a.part[(qi * a.nsplit + split) * 2 + h] = bi[j] < 0 ? YK_KEY_NONE : yk_key(bv[j], bi[j]);
+    }
   } else if (MODE == 2) {
 #pragma unroll
     for (int j = 0; j < NB; j++) {
@@ -276,6 +326,7 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a) {
       for (int s = 0; s < R; s++) dst[s] = top[j][s];
     }
   }
+  }  // work items
 }
 
 // ---------------------------------------------------------------------------
@@ -402,7 +453,9 @@ static YkPlan yk_plan(long nq, long nb, long sample, long min_splits, long max_s
   p.tiles = (int)((nq + qt - 1) / qt);
   p.bstep = 32 * sample;
   const long nblocks = (nb + p.bstep - 1) / p.bstep;  // processed blocks
-  long want = (1024 + p.tiles - 1) / p.tiles;
+  // ~8 work items per resident workgroup (512 of them: 256 CUs x 2) keeps
+  // the tail of the persistent grid short
+  long want = (4096 + p.tiles - 1) / p.tiles;
   long maxs = (nblocks + 15) / 16;  // >= 16 processed blocks per split
   if (want > maxs) want = maxs;
   if (want < min_splits) want = min_splits;
@@ -449,12 +502,11 @@ static int yk_launch_scan(const YkPlan &p, ScanArgs a, bool qres) {
   a.tiles_x = p.tiles;
   a.total = p.tiles * p.nsplit;
   a.per_xcd = (a.total + 7) / 8;
-  dim3 grid(a.per_xcd * 8), block(64 * p.waves);
+  auto kern = qres ? yk_scan_f32<1, true, MODE, R> : yk_scan_f32<1, false, MODE, R>;
+  const int per_xcd_res = yk_resident_wgs((const void *)kern, 64 * p.waves) / 8;
+  dim3 grid(8 * (a.per_xcd < per_xcd_res ? a.per_xcd : per_xcd_res)), block(64 * p.waves);
   int tok = yk_prof_begin(YK_PROF_FUSED);
-  if (qres)
-    hipLaunchKernelGGL((yk_scan_f32<1, true, MODE, R>), grid, block, 0, yk_stream(), a);
-  else
-    hipLaunchKernelGGL((yk_scan_f32<1, false, MODE, R>), grid, block, 0, yk_stream(), a);
+  hipLaunchKernelGGL(kern, grid, block, 0, yk_stream(), a);
   yk_prof_end(tok);
   YK_HIP(hipGetLastError());
   return YK_OK;
@@ -486,13 +538,17 @@ static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *d
   int *cnt = (int *)vcnt;
   int *flag = cnt + qb + 32;
   vflag = flag;
-  // sampling stride of the sketch pass: candidates ~ stride * k must fit
-  long stride = YK_CAND_CAP / (4L * k);
+  // Sketch: every lane's minimum over its rows of a 1-in-stride sample of
+  // base blocks, with >= k splits -> >= 2k distinct real keys per query.
+  // The k-th smallest of 2k subset minima sits near global rank
+  // 1.4 * stride * k, the expected candidate count; ~800 keeps the filtered
+  // pass's append path rare and the list far from overflow.
+  long stride = 600L / k;
   if (stride > 8) stride = 8;
   if (stride < 1) stride = 1;
   const long nblocks = (base.nb + 31) / 32;
-  while (stride > 1 && (nblocks / stride) * 32 < 4L * k) stride >>= 1;
-  const long min_splits = (2L * k + 31) / 32;  // sketch: 32 keys per split and query
+  while (stride > 1 && nblocks / stride < 2L * k) stride >>= 1;
+  const long min_splits = k;
 
   for (long q0 = 0; q0 < nq_all; q0 += QB) {
     const long nq = nq_all - q0 < QB ? nq_all - q0 : QB;
@@ -500,12 +556,12 @@ static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *d
     a.q = base.q + q0 * base.ld;
     a.qn = base.qn ? base.qn + q0 : nullptr;
     a.nq = nq;
-    // (A) sketch: register top-16 per lane over a strided sample
-    YkPlan pa = yk_plan(nq, a.nb, stride, min_splits, 256);
-    YK_TRY(yk_scratch(3, sizeof(uint64_t) * nq * pa.nsplit * 2 * YK_REG_R, &vpart));
+    // (A) sketch: per-lane minima over a strided sample
+    YkPlan pa = yk_plan(nq, a.nb, stride, min_splits, min_splits > 256 ? min_splits : 256);
+    YK_TRY(yk_scratch(3, sizeof(uint64_t) * nq * pa.nsplit * 2, &vpart));
     a.part = (uint64_t *)vpart;
-    YK_TRY((yk_launch_scan<2, YK_REG_R>(pa, a, qres)));
-    YK_TRY(yk_sortsel(a.part, nq, pa.nsplit * 2 * YK_REG_R, k, nullptr, nullptr, thr));
+    YK_TRY((yk_launch_scan<1, 1>(pa, a, qres)));
+    YK_TRY(yk_sortsel(a.part, nq, pa.nsplit * 2, k, nullptr, nullptr, thr));
     // (B) filtered scan + per-query sort, retried for overflowing queries
     YkPlan pb = yk_plan(nq, a.nb, 1, 1, 4096);
     a.thr = thr;
