@@ -147,6 +147,68 @@ def bench_kmeans(ya, dev, v, k, niter, seed):
     return dict(sec=t1 - t0, iters=niter, obj=float(ret.value))
 
 
+def zipf_sizes(ninst, total, lo, hi, s=1.1):
+    """Instance sizes proportional to rank^-s, water-filled into [lo, hi] so
+    that they sum to total."""
+    w = 1.0 / np.arange(1, ninst + 1) ** s
+    a, b = 0.0, float(total) / w.min()
+    for _ in range(200):
+        lam = 0.5 * (a + b)
+        if np.clip(lam * w, lo, hi).sum() > total:
+            b = lam
+        else:
+            a = lam
+    n = np.floor(np.clip(a * w, lo, hi)).astype(np.int64)
+    free = np.nonzero(n < hi)[0]
+    n[free[0]] += total - n.sum()
+    return n.astype(np.int32)
+
+
+def bench_batched(ya, dev, ninst=256, total=2_000_000, d=128, k=64, niter=15):
+    """hkm level-2 pattern: 256 independent k-means instances of Zipf sizes,
+    one batched run (grouped launches, graph-replayed iteration)."""
+    lib = ya.lib()
+    sizes = zipf_sizes(ninst, total, 256, 120_000)
+    v = mixture(4, int(sizes.sum()), d, 1024, 20, centres_seed=0)
+    V = dev.DeviceSet(v)
+    cent = np.empty((ninst, k, d), np.float32)
+    ret = np.empty(ninst, np.float32)
+    iters = np.empty(ninst, np.int32)
+    seeds = np.arange(1, ninst + 1, dtype=np.int64)
+    chunk = ctypes.c_int(0)
+
+    def run(graph):
+        ya.check(lib.yael_kmeans_batched_device(d, ninst, sizes.ctypes.data, k, niter, V.vec.ptr,
+                                                0x10000, seeds.ctypes.data, cent.ctypes.data,
+                                                None, None, None, ret.ctypes.data,
+                                                iters.ctypes.data, graph, ctypes.byref(chunk)))
+    run(1)  # warm-up (and autotuning on a fresh cache)
+    lib.yk_sync()
+    lib.yk_prof_reset()
+    lib.yk_prof_enable(1)
+    t0 = time.perf_counter()
+    run(1)
+    lib.yk_sync()
+    t1 = time.perf_counter()
+    lib.yk_prof_enable(0)
+    loop_ms, _ = prof(lib, PROF_BATCH)
+    # the same instances one kmeans() call at a time (the reference's hkm loop)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    c1 = np.empty((k, d), np.float32)
+    r1 = ctypes.c_float()
+    t2 = time.perf_counter()
+    for i in range(ninst):
+        ya.check(lib.yael_kmeans_device(d, int(sizes[i]), k, niter,
+                                        ctypes.c_void_p(V.vec.ptr.value + int(off[i]) * V.ld * 4),
+                                        0x10000, int(seeds[i]), 1, c1.ctypes.data, None, None, None,
+                                        ctypes.byref(r1)))
+    lib.yk_sync()
+    t3 = time.perf_counter()
+    V.free()
+    return dict(sizes=sizes, call_s=t1 - t0, loop_ms=loop_ms, seq_s=t3 - t2, chunk=chunk.value,
+                iters_max=int(iters.max()), iters_min=int(iters.min()))
+
+
 def cpu_baseline_knn(base, queries, k, budget_s=12.0):
     """yael's own knn_full_thread (compiled reference, OpenBLAS + OpenMP) on a
     bounded sample of the queries against the full base."""
@@ -260,6 +322,20 @@ def main():
                                           "device-resident",
                          "kmeans_obj": km["obj"]}
         print("# kmeans: %.2f iter/s" % (km["iters"] / km["sec"]), file=sys.stderr, flush=True)
+        bt = bench_batched(ya, dev)
+        out["extras"]["batched"] = {
+            "config": "256 k-means instances (hkm level 2), Zipf(1.1) sizes sum 2e6 in [256, "
+                      "120000], d=128, k=64, niter=15, FP32, one batched run, graph-replayed "
+                      "iteration, autotuned update chunk %d" % bt["chunk"],
+            "grouped_iter_per_s": 15.0 / (bt["loop_ms"] * 1e-3),
+            "whole_call_s": bt["call_s"],
+            "sequential_kmeans_calls_s": bt["seq_s"],
+            "speedup_vs_sequential_calls": bt["seq_s"] / bt["call_s"],
+            "iterations_run": [bt["iters_min"], bt["iters_max"]],
+        }
+        print("# batched: %.1f grouped iter/s (loop %.2f ms, call %.3f s, sequential %.3f s)"
+              % (15.0 / (bt["loop_ms"] * 1e-3), bt["loop_ms"], bt["call_s"], bt["seq_s"]),
+              file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         cb = cpu_baseline_knn(base, queries, k)

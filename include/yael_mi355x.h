@@ -104,6 +104,25 @@ int yael_kmeans_device(int d, int n, int k, int niter, const float *v_dev, int f
 int yael_knn_reorder_shortlist_checked(int n, int nb, int d, int k, const float *b,
                                        const float *v, int *idx, float *dis);
 
+/* Batched form: ninst independent k-means instances, instance i being
+ * kmeans(d, n_inst[i], k, niter, v_i, flags, seeds[i], 1, ...) on the rows
+ * v_i of the concatenated point set v (sum n_inst rows), as hkm_learn
+ * clusters every tree node (hkm.c:66-103).  Outputs concatenated in instance
+ * order: centroids ninst*k rows, dis/assign per point (assign local to the
+ * instance), nassign ninst*k, ret ninst values (-1 = abandoned).  No
+ * counterpart in the reference (which loops over kmeans()); flags: QUIET,
+ * INIT_RANDOM and the thread bits only. */
+int yael_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, const float *v,
+                        int flags, const long *seeds, float *centroids, float *dis, int *assign,
+                        int *nassign, float *ret);
+/* the same on device-resident points (padded rows of yk_ld(d) floats);
+ * iters (per instance, may be NULL), use_graph (1: replay one captured
+ * hipGraph per iteration), chunk_used (the autotuned update chunk) */
+int yael_kmeans_batched_device(int d, int ninst, const int *n_inst, int k, int niter,
+                               const float *v_dev, int flags, const long *seeds,
+                               float *centroids, float *dis, int *assign, int *nassign,
+                               float *ret, int *iters, int use_graph, int *chunk_used);
+
 #ifdef __cplusplus
 }
 #endif

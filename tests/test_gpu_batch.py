@@ -1,0 +1,121 @@
+"""GPU tests of the batched form (many k-means instances in one grouped,
+graph-captured run).  Every instance must give the bits of the
+single-instance kmeans() with the same seed: the assign step is the same
+kernel, the split runs on the device from host-precomputed streams with the
+host code's arithmetic, and on integer-valued data every centroid sum is
+exact in any order."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ya():
+    import yael_amd
+    yael_amd.init()
+    return yael_amd
+
+
+def mixture(rng, n, d, ncomp, sigma):
+    cen = rng.uniform(0, 255, (ncomp, d))
+    x = np.clip(cen[rng.integers(0, ncomp, n)] + sigma * rng.standard_normal((n, d)), 0, 255)
+    return np.rint(x).astype(np.float32)
+
+
+def zipf_sizes(ninst, total, lo, hi, s=1.1):
+    w = 1.0 / np.arange(1, ninst + 1) ** s
+    n = np.clip(np.floor(w / w.sum() * total), lo, hi).astype(np.int64)
+    return n.astype(np.int32)
+
+
+def single(ya, v, sizes, k, niter, seeds):
+    """kmeans() per instance through the C-ABI (not ynumpy, which raises on
+    the slightly negative objective the expanded distance can give on
+    duplicated points)."""
+    import ctypes
+    out = []
+    off = 0
+    d = v.shape[1]
+    for n, s in zip(sizes, seeds):
+        vi = np.ascontiguousarray(v[off:off + n])
+        c = np.empty((k, d), np.float32)
+        dis = np.empty(n, np.float32)
+        asg = np.empty(n, np.int32)
+        nas = np.empty(k, np.int32)
+        q = ctypes.c_float()
+        rc = ya.lib().yael_kmeans_checked(d, int(n), k, niter, vi.ctypes.data, 0x10000 | 0x80000,
+                                         int(s), 1, c.ctypes.data, dis.ctypes.data, asg.ctypes.data,
+                                         nas.ctypes.data, ctypes.byref(q))
+        ya.check(rc, "kmeans")
+        out.append((c, q.value, dis, asg, nas))
+        off += n
+    return out
+
+
+def compare(batched, singles, sizes):
+    cent, ret, dis, assign, nassign = batched
+    off = 0
+    for i, (c, q, d1, a1, n1) in enumerate(singles):
+        n = sizes[i]
+        assert np.array_equal(assign[off:off + n], a1), "instance %d assign" % i
+        assert np.array_equal(nassign[i], n1), "instance %d nassign" % i
+        assert np.array_equal(cent[i].view(np.int32), c.view(np.int32)), "instance %d centroids" % i
+        assert np.array_equal(dis[off:off + n].view(np.int32), d1.view(np.int32)), "instance %d dis" % i
+        assert np.float32(ret[i]) == np.float32(q), "instance %d objective" % i
+        off += n
+
+
+def test_uneven_instances_match_single_runs(ya):
+    rng = np.random.default_rng(1)
+    sizes = zipf_sizes(24, 60000, 256, 20000)
+    v = mixture(rng, int(sizes.sum()), 32, 200, 18)
+    seeds = np.arange(1, 25)
+    print("sizes", sizes.tolist(), flush=True)
+    b = ya.kmeans_batched(v, sizes, 16, niter=12, seeds=seeds)
+    compare(b, single(ya, v, sizes, 16, 12, seeds), sizes)
+
+
+def test_splits_and_overflow_fallback(ya):
+    """Duplicated points empty many clusters: splits on the device, and an
+    instance with more empties in one iteration than precomputed draws is
+    recomputed by the single-instance path."""
+    rng = np.random.default_rng(2)
+    blocks = []
+    sizes = []
+    for i in range(6):
+        m = 40 + 10 * i
+        pts = rng.integers(0, 4 + i, (m // 10 + 3, 8)).astype(np.float32)
+        blocks.append(np.repeat(pts, 10, axis=0)[:m] if i % 2 else np.repeat(pts[:2], m // 2 + 1, axis=0)[:m])
+        sizes.append(m)
+    v = np.ascontiguousarray(np.concatenate(blocks))
+    sizes = np.array(sizes, np.int32)
+    seeds = np.arange(11, 17)
+    b = ya.kmeans_batched(v, sizes, 12, niter=5, seeds=seeds)
+    compare(b, single(ya, v, sizes, 12, 5, seeds), sizes)
+
+
+def test_graph_and_direct_launches_agree(ya):
+    import ctypes
+    from yael_amd import device
+    rng = np.random.default_rng(3)
+    sizes = zipf_sizes(40, 50000, 300, 9000)
+    v = mixture(rng, int(sizes.sum()), 64, 100, 20)
+    V = device.DeviceSet(v)
+    lib = ya.lib()
+    outs = []
+    for graph in (0, 1):
+        cent = np.empty((40, 8, 64), np.float32)
+        ret = np.empty(40, np.float32)
+        it = np.empty(40, np.int32)
+        chunk = ctypes.c_int(0)
+        seeds = np.arange(100, 140, dtype=np.int64)
+        rc = lib.yael_kmeans_batched_device(64, 40, sizes.ctypes.data, 8, 9, V.vec.ptr, 0x10000,
+                                            seeds.ctypes.data, cent.ctypes.data, None, None, None,
+                                            ret.ctypes.data, it.ctypes.data, graph, ctypes.byref(chunk))
+        ya.check(rc, "batched")
+        print("graph", graph, "chunk", chunk.value, "iters", it[:8].tolist(), flush=True)
+        outs.append((cent, ret, it))
+    V.free()
+    assert np.array_equal(outs[0][0].view(np.int32), outs[1][0].view(np.int32))
+    assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])

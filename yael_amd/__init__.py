@@ -56,4 +56,4 @@ def device_info():
     return buf.value.decode()
 
 
-from .ynumpy import knn, cross_distances, kmeans, knn_reorder_shortlist  # noqa: E402,F401
+from .ynumpy import knn, cross_distances, kmeans, kmeans_batched, knn_reorder_shortlist  # noqa: E402,F401,E501

@@ -35,6 +35,9 @@ PROTOS = {
     "yael_kmeans_checked": (I, [I, I, I, I, vp, I, L, I, vp, vp, vp, vp, f32p]),
     "yael_knn_reorder_shortlist_checked": (I, [I, I, I, I, vp, vp, vp, vp]),
     "yael_kmeans_device": (I, [I, I, I, I, vp, I, L, I, vp, vp, vp, vp, f32p]),
+    "yael_kmeans_batched": (I, [I, I, vp, I, I, vp, I, vp, vp, vp, vp, vp, vp]),
+    "yael_kmeans_batched_device": (I, [I, I, vp, I, I, vp, I, vp, vp, vp, vp, vp, vp, vp, I,
+                                       vp]),
     "yk_gather_rows": (I, [vp, vp, L, I, vp]),
     "yk_rerank_shortlist": (I, [L, I, I, vp, vp, vp, vp]),
     "knn_reorder_shortlist": (None, [I, I, I, I, vp, vp, vp, vp]),

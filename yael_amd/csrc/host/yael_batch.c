@@ -1,0 +1,381 @@
+/*
+ * yael_batch.c -- many independent k-means instances in one batched run
+ * (the hkm_learn pattern, hkm.c:66-103, where every tree node is clustered
+ * by its own kmeans() call with KMEANS_INIT_RANDOM, redo 1).
+ *
+ * Each instance i has the semantics of
+ *     kmeans(d, n_i, k, niter, v_i, flags, seeds[i], 1, ...)
+ * (seeds[i] == 0 draws lrand48() in instance order, as successive kmeans()
+ * calls would).  The host does what depends only on the seeds: the seed
+ * chains (kmeans.c:379-409), the random initialisation (kmeans.c:15-20) and
+ * the split streams (kmeans.c:183-195) -- for every instance and iteration
+ * the first YH_ECAP (draw, gaussian vector) pairs are precomputed -- plus the
+ * work-item tables.  The device then runs every iteration without the host
+ * (yk_batch.hip), captured once in a hipGraph.  An instance that empties more
+ * than YH_ECAP clusters in one iteration is flagged and recomputed by the
+ * single-instance path, so the results never depend on the cap.
+ *
+ * The chunk size of the update's partial sums is chosen by a small
+ * autotuner: each candidate is timed on the actual data for two iterations
+ * (from the same initial state), the fastest is kept and remembered in a
+ * cache file keyed by the problem shape.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../yk.h"
+#include "yael_host.h"
+
+#define YH_ECAP 4
+#define YH_TILE 128 /* points per assign work item (4 waves x 32) */
+
+typedef struct {
+  void *p[32];
+  int n;
+} yh_allocs;
+
+static void *yh_dmalloc(yh_allocs *a, size_t bytes) {
+  void *p = NULL;
+  if (a->n == 32 || yk_malloc(&p, bytes ? bytes : 256) != YK_OK) return NULL;
+  a->p[a->n++] = p;
+  return p;
+}
+
+static void yh_dfree_all(yh_allocs *a) {
+  for (int i = 0; i < a->n; i++) yk_free(a->p[i]);
+  a->n = 0;
+}
+
+/* ---- autotuner cache: one "key value" pair per line ---- */
+
+static const char *yh_tune_path(char *buf, size_t len) {
+  const char *e = getenv("YAEL_MI355X_TUNE_CACHE");
+  if (e && *e) return e;
+  const char *home = getenv("HOME");
+  snprintf(buf, len, "%s/.cache/yael_mi355x_tune.txt", home ? home : "/tmp");
+  return buf;
+}
+
+int yh_tune_get(const char *key, int *val) {
+  char pbuf[512];
+  FILE *f = fopen(yh_tune_path(pbuf, sizeof pbuf), "r");
+  if (!f) return 0;
+  char k[256];
+  int v, found = 0;
+  while (fscanf(f, "%255s %d", k, &v) == 2)
+    if (!strcmp(k, key)) {
+      *val = v;
+      found = 1;
+    }
+  fclose(f);
+  return found;
+}
+
+void yh_tune_put(const char *key, int val) {
+  char pbuf[512];
+  const char *path = yh_tune_path(pbuf, sizeof pbuf);
+  char dir[512];
+  snprintf(dir, sizeof dir, "%s", path);
+  char *slash = strrchr(dir, '/');
+  if (slash) {
+    *slash = 0;
+    char cmd[600];
+    snprintf(cmd, sizeof cmd, "mkdir -p '%s' 2>/dev/null", dir);
+    if (system(cmd)) { /* best effort */
+    }
+  }
+  FILE *f = fopen(path, "a");
+  if (!f) return;
+  fprintf(f, "%s %d\n", key, val);
+  fclose(f);
+}
+
+/* ---- the batched run ---- */
+
+typedef struct {
+  int ninst, d, ld, k, kpad, niter;
+  long ntot;
+  long *off; /* host offsets */
+  /* device */
+  long *d_off;
+  const float *v;
+  double *vn;
+  float *cent, *cn, *dis;
+  int *assign, *sel, *nassign, *flags;
+  double *rd, *qerr2;
+  float *eps, *pbuf;
+  int *item_inst, *item_tile;
+  long nitems;
+} yh_bstate;
+
+/* update-chunk tables for chunk size P, and the batch object */
+static int yh_batch_bind(yh_bstate *s, int P, yh_allocs *al, yk_batch **out) {
+  const int ninst = s->ninst;
+  long nu = 0;
+  int *uf = (int *)malloc(sizeof(int) * (ninst + 1));
+  for (int i = 0; i < ninst; i++) {
+    uf[i] = (int)nu;
+    nu += (s->off[i + 1] - s->off[i] + P - 1) / P;
+  }
+  uf[ninst] = (int)nu;
+  int *ui = (int *)malloc(sizeof(int) * (nu > 0 ? nu : 1));
+  for (int i = 0; i < ninst; i++)
+    for (long u = uf[i]; u < uf[i + 1]; u++) ui[u] = i;
+  int *d_uf = (int *)yh_dmalloc(al, sizeof(int) * (ninst + 1));
+  int *d_ui = (int *)yh_dmalloc(al, sizeof(int) * nu);
+  double *psum = (double *)yh_dmalloc(al, sizeof(double) * (size_t)nu * s->k * s->d);
+  int *pcnt = (int *)yh_dmalloc(al, sizeof(int) * (size_t)nu * s->k);
+  int rc = (!d_uf || !d_ui || !psum || !pcnt) ? YK_ENOMEM : YK_OK;
+  if (rc == YK_OK) rc = yk_memcpy_h2d(d_uf, uf, sizeof(int) * (ninst + 1));
+  if (rc == YK_OK) rc = yk_memcpy_h2d(d_ui, ui, sizeof(int) * nu);
+  free(uf);
+  free(ui);
+  if (rc != YK_OK) return rc;
+  return yk_batch_create(out, ninst, s->d, s->k, s->kpad, s->niter, YH_ECAP, P, s->ntot, s->v,
+                         s->vn, s->cent, s->cn, s->assign, s->dis, s->d_off, s->nitems,
+                         s->item_inst, s->item_tile, nu, d_ui, d_uf, psum, pcnt, s->nassign,
+                         s->pbuf, s->rd, s->eps, s->flags, s->qerr2);
+}
+
+static int yh_batch_init_state(yh_bstate *s) {
+  int rc = yk_gather_rows(s->v, s->sel, (long)s->ninst * s->kpad, s->d, s->cent);
+  return rc;
+}
+
+int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, const float *v,
+                      int v_on_device, int flags, const long *seeds, float *cent_out,
+                      float *dis_out, int *assign_out, int *nassign_out, float *ret_out,
+                      int *iters_out, int use_graph, int *chunk_used) {
+  if (flags & (KMEANS_L1 | KMEANS_CHI2 | KMEANS_INIT_BERKELEY | KMEANS_INIT_USER |
+               KMEANS_NORMALIZE_CENTS))
+    return yk_set_error_c(YK_EINVAL, "batched k-means: only random init, L2, plain centroids");
+  if (ninst < 1 || k < 1 || d < 1 || niter < 0)
+    return yk_set_error_c(YK_EINVAL, "batched k-means: bad sizes");
+  if (niter == 0) niter = 1000000;
+  if (niter > 100000) return yk_set_error_c(YK_EINVAL, "batched k-means: niter too large");
+  int rc = yk_init();
+  if (rc != YK_OK) return rc;
+  yh_bstate s;
+  memset(&s, 0, sizeof s);
+  s.ninst = ninst;
+  s.d = d;
+  s.ld = yk_ld(d);
+  s.k = k;
+  s.kpad = (k + 31) & ~31;
+  s.niter = niter;
+  s.off = (long *)malloc(sizeof(long) * (ninst + 1));
+  s.off[0] = 0;
+  for (int i = 0; i < ninst; i++) {
+    if (n_inst[i] < k) {
+      free(s.off);
+      return yk_set_error_c(YK_EINVAL, "batched k-means: instance %d has %d < k points", i,
+                            n_inst[i]);
+    }
+    s.off[i + 1] = s.off[i] + n_inst[i];
+  }
+  s.ntot = s.off[ninst];
+  const int ld = s.ld, kpad = s.kpad;
+
+  /* seed chains, initial selections, split streams */
+  int *sel = (int *)malloc(sizeof(int) * (size_t)ninst * kpad);
+  int *tmp = (int *)malloc(sizeof(int) * k);
+  double *rd = (double *)malloc(sizeof(double) * (size_t)ninst * niter * YH_ECAP);
+  float *eps = (float *)malloc(sizeof(float) * (size_t)ninst * niter * YH_ECAP * d);
+  unsigned *core = (unsigned *)malloc(sizeof(unsigned) * ninst);
+  long *seed_used = (long *)malloc(sizeof(long) * ninst);
+  for (int i = 0; i < ninst; i++) {
+    long si = seeds ? seeds[i] : 0;
+    if (si == 0) si = lrand48(); /* kmeans.c:379-380 */
+    seed_used[i] = si;
+    unsigned seed = (unsigned)si;
+    yh_random_init(n_inst[i], k, tmp, (unsigned)rand_r(&seed));
+    for (int c = 0; c < kpad; c++) sel[(long)i * kpad + c] = (int)(s.off[i] + tmp[c < k ? c : 0]);
+    core[i] = (unsigned)rand_r(&seed);
+    unsigned cs = core[i];
+    for (int it = 0; it < niter; it++) {
+      unsigned S = (unsigned)rand_r(&cs); /* kmeans.c:296 */
+      for (int e = 0; e < YH_ECAP; e++) {
+        long t = ((long)i * niter + it) * YH_ECAP + e;
+        rd[t] = yh_drand_r(&S);
+        unsigned gs = (unsigned)rand_r(&S);
+        for (int l = 0; l < d; l++) eps[t * d + l] = (float)yh_gaussrand_r(&gs);
+      }
+    }
+  }
+  free(tmp);
+  free(core);
+
+  /* assign work items: (instance, tile of 128 points) */
+  long nitems = 0;
+  for (int i = 0; i < ninst; i++) nitems += (n_inst[i] + YH_TILE - 1) / YH_TILE;
+  int *ii = (int *)malloc(sizeof(int) * nitems), *it_ = (int *)malloc(sizeof(int) * nitems);
+  for (int i = 0, t = 0; i < ninst; i++)
+    for (int j = 0; j < (n_inst[i] + YH_TILE - 1) / YH_TILE; j++, t++) {
+      ii[t] = i;
+      it_[t] = j;
+    }
+  s.nitems = nitems;
+
+  yh_allocs al;
+  al.n = 0;
+  s.d_off = (long *)yh_dmalloc(&al, sizeof(long) * (ninst + 1));
+  float *vdev = v_on_device ? (float *)v : (float *)yh_dmalloc(&al, sizeof(float) * s.ntot * ld);
+  s.v = vdev;
+  s.vn = (double *)yh_dmalloc(&al, sizeof(double) * s.ntot);
+  s.cent = (float *)yh_dmalloc(&al, sizeof(float) * (size_t)ninst * kpad * ld);
+  s.cn = (float *)yh_dmalloc(&al, sizeof(float) * (size_t)ninst * kpad);
+  s.dis = (float *)yh_dmalloc(&al, sizeof(float) * s.ntot);
+  s.assign = (int *)yh_dmalloc(&al, sizeof(int) * s.ntot);
+  s.sel = (int *)yh_dmalloc(&al, sizeof(int) * (size_t)ninst * kpad);
+  s.nassign = (int *)yh_dmalloc(&al, sizeof(int) * (size_t)ninst * k);
+  s.flags = (int *)yh_dmalloc(&al, sizeof(int) * (7L * ninst + 8));
+  s.qerr2 = (double *)yh_dmalloc(&al, sizeof(double) * 2L * ninst);
+  s.rd = (double *)yh_dmalloc(&al, sizeof(double) * (size_t)ninst * niter * YH_ECAP);
+  s.eps = (float *)yh_dmalloc(&al, sizeof(float) * (size_t)ninst * niter * YH_ECAP * d);
+  s.pbuf = (float *)yh_dmalloc(&al, sizeof(float) * (size_t)ninst * k);
+  s.item_inst = (int *)yh_dmalloc(&al, sizeof(int) * nitems);
+  s.item_tile = (int *)yh_dmalloc(&al, sizeof(int) * nitems);
+  rc = YK_OK;
+  if (!s.d_off || !vdev || !s.vn || !s.cent || !s.cn || !s.dis || !s.assign || !s.sel ||
+      !s.nassign || !s.flags || !s.qerr2 || !s.rd || !s.eps || !s.pbuf || !s.item_inst ||
+      !s.item_tile)
+    rc = YK_ENOMEM;
+  if (rc == YK_OK) rc = yk_memcpy_h2d(s.d_off, s.off, sizeof(long) * (ninst + 1));
+  if (rc == YK_OK && !v_on_device) rc = yk_upload_vectors(vdev, v, s.ntot, d);
+  if (rc == YK_OK) rc = yk_memcpy_h2d(s.sel, sel, sizeof(int) * (size_t)ninst * kpad);
+  if (rc == YK_OK) rc = yk_memcpy_h2d(s.rd, rd, sizeof(double) * (size_t)ninst * niter * YH_ECAP);
+  if (rc == YK_OK)
+    rc = yk_memcpy_h2d(s.eps, eps, sizeof(float) * (size_t)ninst * niter * YH_ECAP * d);
+  if (rc == YK_OK) rc = yk_memcpy_h2d(s.item_inst, ii, sizeof(int) * nitems);
+  if (rc == YK_OK) rc = yk_memcpy_h2d(s.item_tile, it_, sizeof(int) * nitems);
+  if (rc == YK_OK) rc = yk_norms(vdev, s.ntot, d, NULL, s.vn);
+  free(ii);
+  free(it_);
+  free(sel);
+  free(rd);
+  free(eps);
+
+  /* autotune the update chunk (timed from the initial state, then reset) */
+  int P = 0;
+  char key[256];
+  snprintf(key, sizeof key, "batch_chunk:d=%d:k=%d:ninst=%d:ntot=%ld", d, k, ninst, s.ntot);
+  if (rc == YK_OK && !yh_tune_get(key, &P)) {
+    const int cand[4] = {256, 512, 1024, 2048};
+    float best = 1e30f;
+    for (int c = 0; c < 4 && rc == YK_OK; c++) {
+      if ((size_t)cand[c] > (size_t)s.ntot && c > 0) break;
+      yh_allocs tal;
+      tal.n = 0;
+      yk_batch *bt = NULL;
+      float ms = 0;
+      rc = yh_batch_bind(&s, cand[c], &tal, &bt);
+      if (rc == YK_OK) rc = yh_batch_init_state(&s);
+      if (rc == YK_OK) rc = yk_batch_reset(bt);
+      if (rc == YK_OK) rc = yk_batch_run(bt, 1, 0); /* warm-up */
+      if (rc == YK_OK) rc = yk_timer_start();
+      if (rc == YK_OK) rc = yk_batch_run(bt, 2, 0);
+      if (rc == YK_OK) rc = yk_timer_stop(&ms);
+      yk_batch_destroy(bt);
+      yk_sync();
+      yh_dfree_all(&tal);
+      if (rc == YK_OK && ms < best) {
+        best = ms;
+        P = cand[c];
+      }
+    }
+    if (rc == YK_OK) yh_tune_put(key, P);
+  }
+  if (chunk_used) *chunk_used = P;
+
+  /* the run */
+  yk_batch *bt = NULL;
+  yh_allocs bal;
+  bal.n = 0;
+  if (rc == YK_OK) rc = yh_batch_bind(&s, P, &bal, &bt);
+  if (rc == YK_OK) rc = yh_batch_init_state(&s);
+  if (rc == YK_OK) rc = yk_batch_reset(bt);
+  if (rc == YK_OK) rc = yk_batch_run(bt, niter, use_graph);
+  if (rc == YK_OK) rc = yk_sync();
+
+  /* results */
+  int *hflags = (int *)malloc(sizeof(int) * (7L * ninst + 8));
+  double *hq = (double *)malloc(sizeof(double) * 2L * ninst);
+  if (rc == YK_OK) rc = yk_memcpy_d2h(hflags, s.flags, sizeof(int) * (7L * ninst + 8));
+  if (rc == YK_OK) rc = yk_memcpy_d2h(hq, s.qerr2, sizeof(double) * 2L * ninst);
+  if (rc == YK_OK && cent_out) {
+    float *hc = (float *)malloc(sizeof(float) * (size_t)ninst * kpad * ld);
+    rc = yk_memcpy_d2h(hc, s.cent, sizeof(float) * (size_t)ninst * kpad * ld);
+    for (int i = 0; rc == YK_OK && i < ninst; i++)
+      for (int c = 0; c < k; c++)
+        memcpy(cent_out + ((long)i * k + c) * d, hc + ((long)i * kpad + c) * ld,
+               sizeof(float) * d);
+    free(hc);
+  }
+  if (rc == YK_OK && dis_out) rc = yk_memcpy_d2h(dis_out, s.dis, sizeof(float) * s.ntot);
+  if (rc == YK_OK && assign_out) rc = yk_memcpy_d2h(assign_out, s.assign, sizeof(int) * s.ntot);
+  if (rc == YK_OK && nassign_out)
+    rc = yk_memcpy_d2h(nassign_out, s.nassign, sizeof(int) * (size_t)ninst * k);
+  for (int i = 0; rc == YK_OK && i < ninst; i++) {
+    const int *overflow = hflags + ninst, *iters = hflags + 4L * ninst,
+              *aband = hflags + 5L * ninst;
+    const long n = s.off[i + 1] - s.off[i];
+    if (iters_out) iters_out[i] = iters[i];
+    if (ret_out) ret_out[i] = aband[i] ? -1.f : (float)(hq[i] / n);
+    if (overflow[i]) {
+      /* more empties in one iteration than precomputed draws: the exact
+         single-instance path recomputes this instance */
+      float *vi = (float *)malloc(sizeof(float) * n * d);
+      if (v_on_device)
+        rc = yk_download_vectors(vi, vdev + s.off[i] * ld, n, d);
+      else
+        memcpy(vi, v + s.off[i] * d, sizeof(float) * n * d);
+      float *ci = (float *)malloc(sizeof(float) * (size_t)k * d);
+      float r = 0;
+      if (rc == YK_OK)
+        rc = yh_kmeans(d, (int)n, k, niter, vi, 0, flags | KMEANS_QUIET, seed_used[i], 1, ci,
+                       dis_out ? dis_out + s.off[i] : NULL,
+                       assign_out ? assign_out + s.off[i] : NULL,
+                       nassign_out ? nassign_out + (long)i * k : NULL, &r);
+      if (rc == YK_OK) {
+        if (cent_out) memcpy(cent_out + (long)i * k * d, ci, sizeof(float) * (size_t)k * d);
+        if (ret_out) ret_out[i] = r;
+        if (iters_out) iters_out[i] = -1;
+      }
+      free(ci);
+      free(vi);
+    }
+  }
+  free(hflags);
+  free(hq);
+  free(seed_used);
+  yk_batch_destroy(bt);
+  yk_sync();
+  yh_dfree_all(&bal);
+  yh_dfree_all(&al);
+  free(s.off);
+  return rc;
+}
+
+/* Public entry points (declared in include/yael_mi355x.h). */
+int yael_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, const float *v,
+                        int flags, const long *seeds, float *centroids, float *dis, int *assign,
+                        int *nassign, float *ret) {
+  yh_lock();
+  int rc = yh_kmeans_batched(d, ninst, n_inst, k, niter, v, 0, flags, seeds, centroids, dis,
+                             assign, nassign, ret, NULL, 1, NULL);
+  yh_unlock();
+  return rc;
+}
+
+int yael_kmeans_batched_device(int d, int ninst, const int *n_inst, int k, int niter,
+                               const float *v_dev, int flags, const long *seeds,
+                               float *centroids, float *dis, int *assign, int *nassign,
+                               float *ret, int *iters, int use_graph, int *chunk_used) {
+  yh_lock();
+  int rc = yh_kmeans_batched(d, ninst, n_inst, k, niter, v_dev, 1, flags, seeds, centroids, dis,
+                             assign, nassign, ret, iters, use_graph, chunk_used);
+  yh_unlock();
+  return rc;
+}

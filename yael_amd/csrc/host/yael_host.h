@@ -40,5 +40,8 @@ int yh_kmeans(int d, int n, int k, int niter, const float *v, int v_on_device, i
 int yh_reassign_empty(int d, int k, float *cent, long ldc, const int *nassign, unsigned seed);
 /* first k entries of the partial Fisher-Yates of kmeans.c:15-20 */
 void yh_random_init(int n, int k, int *sel, unsigned seed);
+/* kmeans.c:22-24 and vector.c:139-154 */
+double yh_drand_r(unsigned *seed);
+double yh_gaussrand_r(unsigned *seed);
 
 #endif

@@ -20,10 +20,10 @@
 #include "yael_host.h"
 
 /* kmeans.c:22-24 */
-static double yh_drand_r(unsigned *seed) { return rand_r(seed) / ((double)RAND_MAX + 1.0); }
+double yh_drand_r(unsigned *seed) { return rand_r(seed) / ((double)RAND_MAX + 1.0); }
 
 /* vector.c:139-154: ratio-of-uniforms normal deviate */
-static double yh_gaussrand_r(unsigned *seed) {
+double yh_gaussrand_r(unsigned *seed) {
   for (;;) {
     float u1 = (float)yh_drand_r(seed);
     float u2 = (float)yh_drand_r(seed);

@@ -109,6 +109,27 @@ int yk_sum_f64(const float *x, long n, double *out_host, void *ws);
 /* dst row i = src row idx[i] (padded rows; idx is a device array) */
 int yk_gather_rows(const float *src, const int *idx, long n, int d, float *dst);
 
+/* Batched form (yk_batch.hip): many k-means instances advanced together.
+ * yk_batch_assign: the FP32 assign step over work items (instance, 128-point
+ * tile), centroid sets kpad rows apart.  yk_batch_create binds device
+ * arrays laid out by the caller (see host/yael_batch.c); yk_batch_run queues
+ * niter iterations, replaying one captured hipGraph when use_graph. */
+typedef struct yk_batch yk_batch;
+int yk_batch_assign(long nitems, const int *item_inst, const int *item_tile, const long *inst_off,
+                    int d, int kc, int kpad, const float *v, const double *vn, const float *cent,
+                    const float *cn, const int *done, int *assign, float *dis);
+size_t yk_batch_lds_bytes(int k, int d);
+int yk_batch_create(yk_batch **out, int ninst, int d, int k, int kpad, int niter, int ecap,
+                    int chunk, long ntot, const float *v, const double *vn, float *cent, float *cn,
+                    int *assign, float *dis, const long *inst_off, long nitems,
+                    const int *item_inst, const int *item_tile, long nuitems,
+                    const int *uitem_inst, const int *uitem_first, double *psum, int *pcnt,
+                    int *nassign, float *pbuf, const double *rd, const float *eps, int *flags,
+                    double *qerr2);
+int yk_batch_reset(yk_batch *bt);
+int yk_batch_run(yk_batch *bt, int niter, int use_graph);
+int yk_batch_destroy(yk_batch *bt);
+
 /* Timing of the library's stream (HIP events recorded on it). */
 int yk_timer_start(void);
 int yk_timer_stop(float *ms);

@@ -1,0 +1,352 @@
+// yk_batch.hip -- the batched form: many independent k-means instances of
+// uneven size (hierarchical k-means nodes, product-quantisation sub-spaces)
+// advanced together, one grouped launch per step, the whole iteration free
+// of host synchronisation so it can be captured once in a hipGraph and
+// replayed niter times.
+//
+// Per iteration (all instances, instance-major work items):
+//   norms of every centroid           yk_norms
+//   assign                            yk_batch_assign (yk_select.hip): the
+//                                     FP32 scan kernel with a work-item ->
+//                                     (instance, 128-point tile) table
+//   update, partial sums              yk_bupd_partial_kernel: one workgroup
+//                                     per (instance, chunk of P points); one
+//                                     thread per dimension adds the chunk's
+//                                     points in point order into an LDS
+//                                     FP64 table [k][d]
+//   update, reduce + scale            yk_bupd_reduce_kernel: chunks summed in
+//                                     chunk order, (float)(sum * (1/count))
+//   empty-cluster split               yk_bsplit_kernel: kmeans.c:166-209 on
+//                                     the device, its random draws taken from
+//                                     host-precomputed per-(instance,
+//                                     iteration) tables (the streams depend
+//                                     only on the seed, kmeans.c:296)
+//   objective, abandon and stop rules yk_bobj_kernel (kmeans.c:300-313)
+//   iteration counter                 yk_bstep_kernel
+// Finished instances are skipped by every kernel (done flags), so their
+// outputs are those of the iteration at which the reference would stop.
+// Every reduction has a fixed order: results are deterministic.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "yk.h"
+#include "yk_dev.h"
+
+struct BatchDev {
+  int ninst, d, ld, k, kpad, niter, ecap, chunk;
+  long ntot;
+  const float *v;   // ntot x ld
+  const double *vn;  // ntot
+  float *cent;       // ninst x kpad x ld
+  float *cn;         // ninst x kpad
+  int *assign;       // ntot
+  float *dis;        // ntot
+  const long *inst_off;  // ninst + 1
+  long nitems;
+  const int *item_inst, *item_tile;
+  long nuitems;
+  const int *uitem_inst, *uitem_first;  // update chunks; first chunk of each instance
+  double *psum;  // nuitems x k x d
+  int *pcnt;     // nuitems x k
+  int *nassign;  // ninst x k
+  float *pbuf;   // ninst x k (split probabilities)
+  const double *rd;   // ninst x niter x ecap
+  const float *eps;   // ninst x niter x ecap x d
+  int *done, *overflow, *nre, *tot, *iters, *abandoned;
+  double *qerr, *qerr_old;
+  int *it;  // iteration counter
+};
+
+// One workgroup per update chunk; thread t < d owns dimension t.
+__global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
+  extern __shared__ double lsum[];  // k x d
+  int *lcnt = (int *)(lsum + (size_t)b.k * b.d);
+  const long u = blockIdx.x;
+  const int inst = b.uitem_inst[u];
+  if (b.done[inst]) return;
+  const long off = b.inst_off[inst], n = b.inst_off[inst + 1] - off;
+  const long c0 = (u - b.uitem_first[inst]) * (long)b.chunk;
+  const long p0 = off + c0, p1 = off + (c0 + b.chunk < n ? c0 + b.chunk : n);
+  for (int i = threadIdx.x; i < b.k * b.d; i += blockDim.x) lsum[i] = 0.0;
+  for (int i = threadIdx.x; i < b.k; i += blockDim.x) lcnt[i] = 0;
+  __syncthreads();
+  // a point without a nearest centroid (NaN input) has assign -1: the
+  // instance is handed to the single-instance path, which reports it
+  for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x)
+    if ((unsigned)b.assign[p] >= (unsigned)b.k) b.overflow[inst] = 1;
+  __syncthreads();
+  if (b.overflow[inst]) return;
+  for (int t = threadIdx.x; t < b.d; t += blockDim.x) {
+    long p = p0;
+    for (; p + 4 <= p1; p += 4) {
+      const int a0 = b.assign[p], a1 = b.assign[p + 1], a2 = b.assign[p + 2],
+                a3 = b.assign[p + 3];
+      const float x0 = b.v[p * b.ld + t], x1 = b.v[(p + 1) * b.ld + t],
+                  x2 = b.v[(p + 2) * b.ld + t], x3 = b.v[(p + 3) * b.ld + t];
+      lsum[a0 * b.d + t] += (double)x0;
+      lsum[a1 * b.d + t] += (double)x1;
+      lsum[a2 * b.d + t] += (double)x2;
+      lsum[a3 * b.d + t] += (double)x3;
+      if (t == 0) {
+        lcnt[a0]++;
+        lcnt[a1]++;
+        lcnt[a2]++;
+        lcnt[a3]++;
+      }
+    }
+    for (; p < p1; p++) {
+      const int a = b.assign[p];
+      lsum[a * b.d + t] += (double)b.v[p * b.ld + t];
+      if (t == 0) lcnt[a]++;
+    }
+  }
+  __syncthreads();
+  double *ps = b.psum + u * (size_t)b.k * b.d;
+  for (int i = threadIdx.x; i < b.k * b.d; i += blockDim.x) ps[i] = lsum[i];
+  for (int i = threadIdx.x; i < b.k; i += blockDim.x) b.pcnt[u * b.k + i] = lcnt[i];
+}
+
+// One wave per (instance, cluster): chunk sums in chunk order, then
+// centroid = (float)(sum * (1.0 / count)) (kmeans.c:278-288; an empty
+// cluster gets 0 * inf = NaN until the split overwrites it).
+__global__ __launch_bounds__(256) void yk_bupd_reduce_kernel(BatchDev b) {
+  const int lane = threadIdx.x & 63;
+  const long g = blockIdx.x * 4L + (threadIdx.x >> 6);
+  if (g >= (long)b.ninst * b.k) return;
+  const int inst = (int)(g / b.k), c = (int)(g % b.k);
+  if (b.done[inst]) return;
+  const long u0 = b.uitem_first[inst], u1 = b.uitem_first[inst + 1];
+  int cnt = 0;
+  for (long u = u0; u < u1; u++) cnt += b.pcnt[u * b.k + c];
+  const double inv = 1.0 / (double)cnt;
+  float *crow = b.cent + ((long)inst * b.kpad + c) * b.ld;
+  for (int t = lane; t < b.d; t += 64) {
+    double s = 0.0;
+    for (long u = u0; u < u1; u++) s += b.psum[(u * b.k + c) * (size_t)b.d + t];
+    crow[t] = (float)(s * inv);
+  }
+  if (lane == 0) b.nassign[(long)inst * b.k + c] = cnt;
+}
+
+// L1 normalisation of the split probabilities (vector.c:2016-2025)
+__device__ void yk_l1_normalize(float *p, int k) {
+  double s = 0.0;
+  for (int i = 0; i < k; i++) s += fabs((double)p[i]);
+  const double inv = 1. / s;
+  for (int i = 0; i < k; i++) p[i] = (float)((double)p[i] * inv);
+}
+
+// kmeans.c:166-209 for every unfinished instance, one thread per instance.
+__global__ void yk_bsplit_kernel(BatchDev b) {
+  const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+  if (inst >= b.ninst || b.done[inst]) return;
+  const int it = min(*b.it, b.niter - 1);  // tuning runs may go past niter
+  const int k = b.k, d = b.d;
+  const int *na = b.nassign + (long)inst * k;
+  float *p = b.pbuf + (long)inst * k;
+  for (int c = 0; c < k; c++) {
+    const unsigned u = (unsigned)na[c];
+    p[c] = na[c] < 2 ? 0.f : (float)(int)(u * u - 1u);  // 32-bit wrap as in the reference
+  }
+  yk_l1_normalize(p, k);
+  const long tab = ((long)inst * b.niter + it) * b.ecap;
+  int e = 0;
+  for (int c = 0; c < k; c++) {
+    if (na[c] != 0) continue;
+    if (e == b.ecap) {  // more empties than precomputed draws: redo on the host
+      b.overflow[inst] = 1;
+      break;
+    }
+    double rd = b.rd[tab + e];
+    int j = 0;
+    while (j < k - 1) {
+      rd -= p[j];
+      if (rd < 0) break;
+      j++;
+    }
+    float *cc = b.cent + ((long)inst * b.kpad + c) * b.ld;
+    float *cj = b.cent + ((long)inst * b.kpad + j) * b.ld;
+    double nr = 0.0;
+    for (int l = 0; l < d; l++) {
+      cc[l] = cj[l];
+      nr += (double)(cj[l] * cj[l]);  // float squares, double sum
+    }
+    const double s = __dsqrt_rn(nr) * 0.0000001;
+    const float *ep = b.eps + (tab + e) * d;
+    for (int l = 0; l < d; l++) {
+      const float el = (float)((double)ep[l] * s);
+      cj[l] += el;
+      cc[l] -= el;
+    }
+    p[j] = 0;
+    yk_l1_normalize(p, k);
+    e++;
+  }
+  b.nre[inst] = e;
+}
+
+// Objective (FP64, fixed shape) and the abandon / stop rules; one
+// workgroup per instance.
+__global__ __launch_bounds__(256) void yk_bobj_kernel(BatchDev b) {
+  __shared__ double s[256];
+  const int inst = blockIdx.x;
+  if (b.done[inst]) return;
+  const long off = b.inst_off[inst], n = b.inst_off[inst + 1] - off;
+  double acc = 0.0;
+  for (long i = threadIdx.x; i < n; i += 256) acc += (double)b.dis[off + i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x) return;
+  b.iters[inst]++;
+  if (b.overflow[inst]) {
+    b.done[inst] = 1;
+    return;
+  }
+  const int tot = b.tot[inst] + b.nre[inst];
+  b.tot[inst] = tot;
+  if (tot > n / 100 && tot > 1000) {
+    b.abandoned[inst] = 1;
+    b.done[inst] = 1;
+    return;
+  }
+  const double q = s[0], qold = b.qerr_old[inst];
+  b.qerr[inst] = q;
+  b.qerr_old[inst] = q;
+  if (qold == q && b.nre[inst] == 0) b.done[inst] = 1;
+}
+
+__global__ void yk_bstep_kernel(int *it) {
+  if (threadIdx.x == 0) *it += 1;
+}
+
+__global__ void yk_binit_kernel(BatchDev b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *b.it = 0;
+  if (i < b.ninst) {
+    b.done[i] = 0;
+    b.overflow[i] = 0;
+    b.nre[i] = 0;
+    b.tot[i] = 0;
+    b.iters[i] = 0;
+    b.abandoned[i] = 0;
+    b.qerr[i] = 0.0;
+    b.qerr_old[i] = __longlong_as_double(0x7ff0000000000000ll);  // HUGE_VAL
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side of the HIP layer
+
+struct yk_batch {
+  BatchDev dev;
+  void *mem;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  int captured;
+};
+
+extern "C" size_t yk_batch_lds_bytes(int k, int d) {
+  return sizeof(double) * (size_t)k * d + sizeof(int) * (size_t)k;
+}
+
+// The device state of one batched run.  All pointers are device pointers
+// laid out by the caller (host/yael_batch.c) in one allocation.
+extern "C" int yk_batch_create(yk_batch **out, int ninst, int d, int k, int kpad, int niter,
+                               int ecap, int chunk, long ntot, const float *v, const double *vn,
+                               float *cent, float *cn, int *assign, float *dis,
+                               const long *inst_off, long nitems, const int *item_inst,
+                               const int *item_tile, long nuitems, const int *uitem_inst,
+                               const int *uitem_first, double *psum, int *pcnt, int *nassign,
+                               float *pbuf, const double *rd, const float *eps, int *flags,
+                               double *qerr2) {
+  YK_TRY(yk_init());
+  if (yk_batch_lds_bytes(k, d) > 160 * 1024)
+    return yk_set_error(YK_EINVAL, "batched k-means: k*d = %d too large for the LDS table", k * d);
+  yk_batch *bt = new yk_batch();
+  BatchDev &b = bt->dev;
+  b.ninst = ninst; b.d = d; b.ld = yk_ld(d); b.k = k; b.kpad = kpad; b.niter = niter;
+  b.ecap = ecap; b.chunk = chunk; b.ntot = ntot;
+  b.v = v; b.vn = vn; b.cent = cent; b.cn = cn; b.assign = assign; b.dis = dis;
+  b.inst_off = inst_off; b.nitems = nitems; b.item_inst = item_inst; b.item_tile = item_tile;
+  b.nuitems = nuitems; b.uitem_inst = uitem_inst; b.uitem_first = uitem_first;
+  b.psum = psum; b.pcnt = pcnt; b.nassign = nassign; b.pbuf = pbuf; b.rd = rd; b.eps = eps;
+  // flags: 7 int arrays of ninst + the counter
+  b.done = flags; b.overflow = flags + ninst; b.nre = flags + 2L * ninst;
+  b.tot = flags + 3L * ninst; b.iters = flags + 4L * ninst; b.abandoned = flags + 5L * ninst;
+  b.it = flags + 6L * ninst;
+  b.qerr = qerr2; b.qerr_old = qerr2 + ninst;
+  bt->captured = 0;
+  *out = bt;
+  size_t lds = yk_batch_lds_bytes(k, d);
+  if (lds > 65536)
+    YK_HIP(hipFuncSetAttribute((const void *)yk_bupd_partial_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return YK_OK;
+}
+
+extern "C" int yk_batch_reset(yk_batch *bt) {
+  BatchDev &b = bt->dev;
+  hipLaunchKernelGGL(yk_binit_kernel, dim3((b.ninst + 255) / 256), dim3(256), 0, yk_stream(), b);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
+// One iteration of every unfinished instance, queued on the library stream
+// with no host synchronisation (so it can be captured).
+static int yk_batch_enqueue(yk_batch *bt) {
+  BatchDev &b = bt->dev;
+  hipStream_t st = yk_stream();
+  YK_TRY(yk_norms(b.cent, (long)b.ninst * b.kpad, b.d, b.cn, nullptr));
+  YK_TRY(yk_batch_assign(b.nitems, b.item_inst, b.item_tile, b.inst_off, b.d, b.k, b.kpad, b.v,
+                         b.vn, b.cent, b.cn, b.done, b.assign, b.dis));
+  int bt_threads = ((b.d + 63) / 64) * 64;
+  if (bt_threads > 256) bt_threads = 256;
+  hipLaunchKernelGGL(yk_bupd_partial_kernel, dim3((unsigned)b.nuitems), dim3(bt_threads),
+                     yk_batch_lds_bytes(b.k, b.d), st, b);
+  hipLaunchKernelGGL(yk_bupd_reduce_kernel, dim3((unsigned)(((long)b.ninst * b.k + 3) / 4)),
+                     dim3(256), 0, st, b);
+  hipLaunchKernelGGL(yk_bsplit_kernel, dim3((b.ninst + 63) / 64), dim3(64), 0, st, b);
+  hipLaunchKernelGGL(yk_bobj_kernel, dim3(b.ninst), dim3(256), 0, st, b);
+  hipLaunchKernelGGL(yk_bstep_kernel, dim3(1), dim3(64), 0, st, b.it);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
+// Run niter iterations: with use_graph, the iteration is captured once and
+// the graph replayed; otherwise launched directly.
+extern "C" int yk_batch_run(yk_batch *bt, int niter, int use_graph) {
+  hipStream_t st = yk_stream();
+  int tok = yk_prof_begin(YK_PROF_BATCH);
+  if (use_graph) {
+    if (!bt->captured) {
+      YK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = yk_batch_enqueue(bt);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (rc != YK_OK) return rc;
+      if (e != hipSuccess) return yk_hip_fail(e, "hipStreamEndCapture");
+      bt->graph = g;
+      YK_HIP(hipGraphInstantiate(&bt->exec, g, nullptr, nullptr, 0));
+      bt->captured = 1;
+    }
+    for (int i = 0; i < niter; i++) YK_HIP(hipGraphLaunch(bt->exec, st));
+  } else {
+    for (int i = 0; i < niter; i++) YK_TRY(yk_batch_enqueue(bt));
+  }
+  yk_prof_end(tok);
+  return YK_OK;
+}
+
+extern "C" int yk_batch_destroy(yk_batch *bt) {
+  if (!bt) return YK_OK;
+  if (bt->captured) {
+    (void)hipGraphExecDestroy(bt->exec);
+    (void)hipGraphDestroy(bt->graph);
+  }
+  delete bt;
+  return YK_OK;
+}

@@ -71,6 +71,14 @@ struct ScanArgs {
   const uint64_t *thr;  // MODE 3: per-query threshold key (0 = inactive)
   int *cnt;             // MODE 3: per-query candidate count
   uint64_t *cand;       // MODE 3: [nq][YK_CAND_CAP]
+  // batched form (item_inst != NULL): work item -> (instance, point tile);
+  // q/qn/out_* are the concatenated points of all instances, b/bn the
+  // instances' centroid sets (kpad rows apart), nb is unused
+  const int *item_inst;
+  const int *item_tile;
+  const long *inst_off;  // ninst + 1 point offsets
+  int kc, kpad;
+  const int *done;  // instances already finished (skipped)
 };
 
 // The hardware hands consecutive block ids to the 8 XCDs in turn; give each
@@ -93,7 +101,7 @@ __device__ __forceinline__ void yk_reg_insert(uint64_t (&t)[R], uint64_t x) {
 }
 
 template <int NB, bool QRES, int MODE, int R>
-__global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a) {
+__global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int r = lane & 31, h = lane >> 5;
   // Persistent grid: the hardware deals block ids to the 8 XCDs in turn;
@@ -101,9 +109,28 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a) {
   // (split-major, so concurrently running workgroups of one XCD share a base
   // range in its L2), and its workgroups stride through that run.
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, wpx = gridDim.x >> 3;
-  const int ibeg = xcd * a.per_xcd, iend = min(a.total, ibeg + a.per_xcd);
+  const int ibeg = xcd * a0.per_xcd, iend = min(a0.total, ibeg + a0.per_xcd);
   for (int item = ibeg + local; item < iend; item += wpx) {
-  const int tile = item % a.tiles_x, split = item / a.tiles_x;
+  ScanArgs a = a0;
+  int tile, split;
+  if (a0.item_inst) {
+    const int inst = a0.item_inst[item];
+    if (a0.done && a0.done[inst]) continue;
+    tile = a0.item_tile[item];
+    split = 0;
+    const long off = a0.inst_off[inst];
+    a.q = a0.q + off * a0.ld;
+    a.qn = a0.qn + off;
+    a.nq = a0.inst_off[inst + 1] - off;
+    a.b = a0.b + (long)inst * a0.kpad * a0.ld;
+    a.bn = a0.bn + (long)inst * a0.kpad;
+    a.nb = a0.kc;
+    a.out_i = a0.out_i + off;
+    a.out_d = a0.out_d + off;
+  } else {
+    tile = item % a0.tiles_x;
+    split = item / a0.tiles_x;
+  }
   const long qbase = ((long)tile * nwave + wave) * (32 * NB);
   const long bbeg = (long)split * a.brange;
   const long bend = min(a.nb, bbeg + a.brange);
@@ -642,4 +669,42 @@ extern "C" int yk_knn_f32(int type, long nq, long nb, int d, int k, const float 
     return yk_sortsel(a.part, nq, p.nsplit * 2 * YK_REG_R, k, assign, dis, nullptr);
   }
   return yk_knn_large_k(a, k, qres, assign, dis);
+}
+
+// ---------------------------------------------------------------------------
+// batched form: the assign step of many k-means instances in one launch
+
+extern "C" int yk_batch_assign(long nitems, const int *item_inst, const int *item_tile,
+                               const long *inst_off, int d, int kc, int kpad, const float *v,
+                               const double *vn, const float *cent, const float *cn,
+                               const int *done, int *assign, float *dis) {
+  YK_TRY(yk_init());
+  if (nitems <= 0) return YK_OK;
+  ScanArgs a = {};
+  a.q = v;
+  a.qn = vn;
+  a.b = cent;
+  a.bn = cn;
+  a.ld = yk_ld(d);
+  a.type = YK_L2;
+  a.brange = kpad;
+  a.bstep = 32;
+  a.nsplit = 1;
+  a.tiles_x = 1;
+  a.out_i = assign;
+  a.out_d = dis;
+  a.item_inst = item_inst;
+  a.item_tile = item_tile;
+  a.inst_off = inst_off;
+  a.kc = kc;
+  a.kpad = kpad;
+  a.done = done;
+  a.total = (int)nitems;
+  a.per_xcd = (a.total + 7) / 8;
+  auto kern = a.ld <= 128 ? yk_scan_f32<1, true, 0, 1> : yk_scan_f32<1, false, 0, 1>;
+  const int per_res = yk_resident_wgs((const void *)kern, 256) / 8;
+  dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), block(256);
+  hipLaunchKernelGGL(kern, grid, block, 0, yk_stream(), a);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
 }

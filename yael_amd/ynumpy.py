@@ -111,6 +111,31 @@ def kmeans(v, k, distance_type=2, nt=1, niter=30, seed=0, redo=1, verbose=True,
     return cent, qerr, dis, assign, nassign
 
 
+def kmeans_batched(v, sizes, k, niter=30, seeds=None, verbose=False):
+    """ninst independent k-means instances on consecutive row ranges of v
+    (instance i: the next sizes[i] rows), each with kmeans() semantics and its
+    own seed (seed 0: lrand48(), in instance order).  One batched GPU run.
+    Returns (centroids (ninst, k, d), qerr (ninst,), dis, assign, nassign
+    (ninst, k))."""
+    v = _rowf32(v, "v")
+    sizes = np.ascontiguousarray(sizes, np.int32)
+    ninst = len(sizes)
+    n, d = v.shape
+    if int(sizes.sum()) != n:
+        raise ValueError("sizes must sum to the number of rows")
+    seeds = np.zeros(ninst, np.int64) if seeds is None else np.ascontiguousarray(seeds, np.int64)
+    cent = np.empty((ninst, k, d), np.float32)
+    dis = np.empty(n, np.float32)
+    assign = np.empty(n, np.int32)
+    nassign = np.empty((ninst, k), np.int32)
+    ret = np.empty(ninst, np.float32)
+    flags = KMEANS_INIT_RANDOM | (0 if verbose else KMEANS_QUIET)
+    rc = lib().yael_kmeans_batched(d, ninst, _ptr(sizes), k, niter, _ptr(v), flags, _ptr(seeds),
+                                   _ptr(cent), _ptr(dis), _ptr(assign), _ptr(nassign), _ptr(ret))
+    check(rc, "kmeans_batched")
+    return cent, ret, dis, assign, nassign
+
+
 def knn_reorder_shortlist(base, queries, idx, dis=None):
     """Re-rank shortlists idx (nq, k) by exact distance (nn.c:528-580)."""
     base = _rowf32(base, "base")
