@@ -23,20 +23,25 @@
 //   * the 32x32 accumulator gives each lane one query and 16 base rows, so
 //     the epilogue is lane-local:
 //       MODE 0  running (value, index) minimum           -> k = 1
-//       MODE 2  register top-R of keys (R <= 16)          -> 2 <= k <= 16, and
-//               the threshold sketch for larger k
+//       MODE 1  each lane's own minimum over its rows     -> the threshold
+//               sketch for k > 16
+//       MODE 2  register top-R of keys (R <= 16)          -> 2 <= k <= 16
 //       MODE 3  append every key <= the query's threshold T_q to a per-query
 //               candidate list (global atomics; rare)     -> k > 16
-//   * the base is split over workgroups when the query tiles alone cannot
+//   * a persistent grid (resident workgroups only) walks the work items;
+//     each XCD owns a contiguous run of items so workgroups that share a
+//     base range share that XCD's L2;
+//   * the base is split over work items when the query tiles alone cannot
 //     fill 256 CUs; split results are merged by small sort kernels.
 //
-// k > 16 is exact in two passes: (A) MODE 2 with R = 16 over a strided
-// sample of base blocks gives, per query, 32 * nsplit distinct real keys; their
-// k-th smallest T_q bounds the true k-th smallest key from above.  (B) MODE 3
-// over the whole base collects every key <= T_q, then a per-query sort keeps
-// the k smallest.  A query whose candidates overflow the list gets a tighter
-// T_q (the k-th smallest of the keys it did capture, again an upper bound)
-// and is scanned again.
+// k > 16 is exact in two passes: (A) MODE 1 over a 1-in-stride sample of
+// base blocks with >= k splits gives, per query, >= 2k minima of disjoint
+// row subsets -- distinct real keys -- whose k-th smallest T_q bounds the
+// true k-th smallest key from above.  (B) MODE 3 over the whole base
+// collects every key <= T_q, then a per-query sort keeps the k smallest.  A
+// query whose candidates overflow the list gets a tighter T_q (the k-th
+// smallest of the keys it did capture, again an upper bound) and is scanned
+// again.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
