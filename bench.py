@@ -147,6 +147,172 @@ def bench_kmeans(ya, dev, v, k, niter, seed):
     return dict(sec=t1 - t0, iters=niter, obj=float(ret.value))
 
 
+def bench_kmeans_large(ya, dev, n=10_000_000, d=128, k=65536, niter=20, seed=1, nsub=8,
+                       ksub=256):
+    """n=1e7, d=128, k=65,536 k-means with the BF16 assign form (BF16 first
+    pass, FP32 re-rank of each point's 8 best centroids, certificate, FP32
+    search of the uncertified points), FP64 sums and objective; data from a
+    65,536-component mixture (sigma 15, clipped to [0,255]) generated in HBM.
+    Then the assign step alone on the final centroids in both forms, the
+    FP64 objective of that assignment, and 8 PQ sub-spaces (d=16, k=256)
+    trained in one batched run."""
+    lib = ya.lib()
+    t = time.perf_counter()
+    C0 = dev.DeviceSet.synth(65536, d, 0, 41, 0.0, 255.0)
+    V = dev.DeviceSet.synth(n, d, 1, 42, 0.0, 255.0, centres=C0, sigma=15.0)
+    C0.free()
+    lib.yk_sync()
+    out = {"generate_s": time.perf_counter() - t}
+    cent = np.empty((k, d), np.float32)
+    ret = ctypes.c_float()
+    lib.yk_prof_reset()
+    lib.yk_prof_enable(1)
+    lib.yk_sync()
+    t0 = time.perf_counter()
+    ya.check(lib.yael_kmeans_device(d, n, k, niter, V.vec.ptr, 0x10000 | 0x1000000, seed, 1,
+                                    cent.ctypes.data_as(ctypes.c_void_p), None, None, None,
+                                    ctypes.byref(ret)), "kmeans bf16")
+    lib.yk_sync()
+    sec = time.perf_counter() - t0
+    lib.yk_prof_enable(0)
+    sl = {s_: prof(lib, s_) for s_ in range(7)}
+    st = [ctypes.c_long() for _ in range(3)]
+    lib.yael_kmeans_last_stats(*[ctypes.byref(x) for x in st])
+    iters, assigned, fallback = [x.value for x in st]
+    out.update(sec=sec, iters=iters, obj=float(ret.value), assigned=assigned, fallback=fallback,
+               bf16_ms=sl[PROF_BF16][0], rerank_ms=sl[PROF_RERANK][0], fp32_ms=sl[PROF_FUSED][0],
+               update_ms=sl[PROF_UPDATE][0])
+    print("# kmeans n=%d k=%d: %d iterations in %.2f s (%.3f iter/s), bf16 scan %.1f ms, "
+          "rerank %.1f ms, update %.1f ms, fp32 fallback %d of %d assignments"
+          % (n, k, iters, sec, iters / sec, sl[PROF_BF16][0], sl[PROF_RERANK][0],
+             sl[PROF_UPDATE][0], fallback, assigned), file=sys.stderr, flush=True)
+
+    # the assign step alone, both forms, on the final centroids
+    C = dev.DeviceSet(cent)
+    V.compute_norms()
+    C.compute_norms()
+    qh, qst, _ = V.compute_bf16(2, True)
+    bh, _, bmx = C.compute_bf16(2, False)
+    oi16, od16 = dev.DeviceBuffer(n * 4), dev.DeviceBuffer(n * 4)
+    oi32, od32 = dev.DeviceBuffer(n * 4), dev.DeviceBuffer(n * 4)
+    nfb = ctypes.c_int(0)
+
+    def a16():
+        ya.check(lib.yk_knn_bf16(2, n, k, d, 1, 8, V.vec.ptr, V.dnorm.ptr, qh.ptr, qst.ptr,
+                                 C.vec.ptr, C.fnorm.ptr, bh.ptr, bmx.ptr, oi16.ptr, od16.ptr,
+                                 ctypes.byref(nfb)))
+
+    def a32():
+        ya.check(lib.yk_knn_f32(2, n, k, d, 1, V.vec.ptr, V.dnorm.ptr, C.vec.ptr, C.fnorm.ptr,
+                                None, oi32.ptr, od32.ptr))
+    s16, sl16 = timed(lib, a16, 1, 0)
+    s32, sl32 = timed(lib, a32, 1, 0)
+    i16, d16 = oi16.to_host(np.int32, n), od16.to_host(np.float32, n)
+    i32, d32 = oi32.to_host(np.int32, n), od32.to_host(np.float32, n)
+    o64 = dev.DeviceBuffer(n * 8)
+    ya.check(lib.yk_pair_dist_f64(2, n, d, 1, V.vec.ptr, C.vec.ptr, oi16.ptr, o64.ptr))
+    d64 = o64.to_host(np.float64, n)
+    obj32, obj64 = float(np.sum(d16.astype(np.float64))), float(np.sum(d64))
+    flops = 2.0 * n * k * d
+    out["assign"] = {
+        "bf16_rerank_s": s16, "fp32_s": s32, "speedup": s32 / s16,
+        "bf16_scan_ms": sl16[PROF_BF16][0], "rerank_ms": sl16[PROF_RERANK][0],
+        "fp32_scan_ms": sl32[PROF_FUSED][0],
+        "bf16_scan_tflops": flops / (sl16[PROF_BF16][0] * 1e-3) / 1e12,
+        "fp32_scan_tflops": flops / (sl32[PROF_FUSED][0] * 1e-3) / 1e12,
+        "fallback_points": nfb.value,
+        "assign_identical": bool(np.array_equal(i16, i32)),
+        "dis_bit_identical": bool(np.array_equal(d16.view(np.int32), d32.view(np.int32))),
+        "objective_fp32": obj32, "objective_fp64": obj64,
+        "objective_rel_err_vs_fp64": abs(obj32 - obj64) / abs(obj64),
+    }
+    print("# assign alone: bf16+rerank %.3f s, fp32 %.3f s (x%.2f), identical %s, "
+          "objective rel err vs fp64 %.2e" % (s16, s32, s32 / s16, out["assign"]["assign_identical"],
+                                              out["assign"]["objective_rel_err_vs_fp64"]),
+          file=sys.stderr, flush=True)
+    for b in (oi16, od16, oi32, od32, o64):
+        b.free()
+    C.free()
+
+    # product-quantiser sub-spaces, one batched run
+    cq = np.empty((nsub, ksub, d // nsub), np.float32)
+    rq = np.empty(nsub, np.float32)
+    itq = np.empty(nsub, np.int32)
+    chunk = ctypes.c_int(0)
+    seeds = np.arange(1, nsub + 1, dtype=np.int64)
+
+    def pq(it):
+        ya.check(lib.yael_kmeans_subspaces_device(d, n, nsub, ksub, it, V.vec.ptr, 0x10000,
+                                                  seeds.ctypes.data, cq.ctypes.data, None, None,
+                                                  None, rq.ctypes.data, itq.ctypes.data,
+                                                  ctypes.byref(chunk)), "subspaces")
+    pq(1)  # the update chunk is autotuned on the first call of a shape
+    lib.yk_sync()
+    t0 = time.perf_counter()
+    pq(niter)
+    lib.yk_sync()
+    out["pq"] = {"sec": time.perf_counter() - t0, "iters": itq.tolist(), "chunk": chunk.value,
+                 "objective": rq.tolist()}
+    print("# pq %d x (d=%d, k=%d): %.3f s for %s iterations" % (nsub, d // nsub, ksub,
+                                                               out["pq"]["sec"], itq.tolist()),
+          file=sys.stderr, flush=True)
+    V.free()
+    return out
+
+
+def bench_precision_d960(ya, dev, nb=1_000_000, nq=10_000, d=960, k=10, kp=64, steps=3):
+    """Precision form at d=960: |N(0,1)| vectors scaled to unit norm,
+    generated in HBM; k=10 from k'=64 BF16 candidates."""
+    lib = ya.lib()
+    B = dev.DeviceSet.synth(nb, d, 2, 51, unit_rows=True)
+    Q = dev.DeviceSet.synth(nq, d, 2, 52, unit_rows=True)
+    oi, od = dev.DeviceBuffer(nq * k * 4), dev.DeviceBuffer(nq * k * 4)
+    oi16, od16 = dev.DeviceBuffer(nq * k * 4), dev.DeviceBuffer(nq * k * 4)
+    nfb = ctypes.c_int(0)
+
+    def f32():
+        Q.compute_norms()
+        B.compute_norms()
+        ya.check(lib.yk_knn_f32(2, nq, nb, d, k, Q.vec.ptr, Q.dnorm.ptr, B.vec.ptr, B.fnorm.ptr,
+                                None, oi.ptr, od.ptr))
+
+    def b16():
+        Q.compute_norms()
+        B.compute_norms()
+        qh, qst, _ = Q.compute_bf16(2, True)
+        bh, _, bmx = B.compute_bf16(2, False)
+        ya.check(lib.yk_knn_bf16(2, nq, nb, d, k, kp, Q.vec.ptr, Q.dnorm.ptr, qh.ptr, qst.ptr,
+                                 B.vec.ptr, B.fnorm.ptr, bh.ptr, bmx.ptr, oi16.ptr, od16.ptr,
+                                 ctypes.byref(nfb)))
+    s32, sl32 = timed(lib, f32, 2, 1)
+    s16, sl16 = timed(lib, b16, steps, 1)
+    i32, d32 = oi.to_host(np.int32, (nq, k)), od.to_host(np.float32, (nq, k))
+    i16, d16 = oi16.to_host(np.int32, (nq, k)), od16.to_host(np.float32, (nq, k))
+    o64 = dev.DeviceBuffer(nq * k * 8)
+    ya.check(lib.yk_pair_dist_f64(2, nq, d, k, Q.vec.ptr, B.vec.ptr, oi16.ptr, o64.ptr))
+    d64 = o64.to_host(np.float64, (nq, k))
+    rel = np.abs(d16.astype(np.float64) - d64) / np.maximum(np.abs(d64), 1e-30)
+    flops = 2.0 * nq * nb * d
+    for b in (oi, od, oi16, od16, o64):
+        b.free()
+    B.free()
+    Q.free()
+    r = {"config": "nb=%d nq=%d d=%d k=%d k'=%d, |N(0,1)| rows scaled to unit L2 norm"
+                   % (nb, nq, d, k, kp),
+         "bf16_rerank_queries_per_s": nq / s16, "fp32_queries_per_s": nq / s32,
+         "speedup": s32 / s16,
+         "bf16_scan_tflops": flops / (sl16[PROF_BF16][0] / steps * 1e-3) / 1e12,
+         "fp32_scan_tflops": flops / (sl32[PROF_FUSED][0] / 2 * 1e-3) / 1e12,
+         "index_agreement_with_fp32": float(np.mean(np.all(i16 == i32, axis=1))),
+         "distances_bit_identical_to_fp32": bool(np.array_equal(d16.view(np.int32),
+                                                                d32.view(np.int32))),
+         "max_rel_err_vs_fp64": float(np.nanmax(rel)), "fallback_queries": nfb.value}
+    print("# d=960: bf16+rerank %.1f q/s, fp32 %.1f q/s (x%.2f), agreement %.4f, fallback %d"
+          % (nq / s16, nq / s32, s32 / s16, r["index_agreement_with_fp32"], nfb.value),
+          file=sys.stderr, flush=True)
+    return r
+
+
 def zipf_sizes(ninst, total, lo, hi, s=1.1):
     """Instance sizes proportional to rank^-s, water-filled into [lo, hi] so
     that they sum to total."""
@@ -241,6 +407,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the d=960 precision-form and n=1e7, k=65536 k-means extras")
     ap.add_argument("--only-bf16", action="store_true", help="profile runs: the precision form only")
     ap.add_argument("--nb", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
@@ -336,6 +504,23 @@ def main():
         print("# batched: %.1f grouped iter/s (loop %.2f ms, call %.3f s, sequential %.3f s)"
               % (15.0 / (bt["loop_ms"] * 1e-3), bt["loop_ms"], bt["call_s"], bt["seq_s"]),
               file=sys.stderr, flush=True)
+
+    if not args.no_large:
+        out.setdefault("extras", {})
+        out["extras"]["precision_d960"] = bench_precision_d960(ya, dev)
+        kl = bench_kmeans_large(ya, dev)
+        out["extras"]["kmeans_large"] = {
+            "config": "n=1e7 d=128 k=65536 niter=20 seed 1, 65,536-component mixture sigma 15 in "
+                      "[0,255] generated in HBM; BF16 first pass + FP32 re-rank of the 8 best "
+                      "centroids + certificate, FP64 centroid sums and objective",
+            "iter_per_s": kl["iters"] / kl["sec"], "iterations": kl["iters"], "sec": kl["sec"],
+            "objective": kl["obj"], "fp32_fallback_assignments": kl["fallback"],
+            "assignments": kl["assigned"], "bf16_scan_ms": kl["bf16_ms"],
+            "rerank_ms": kl["rerank_ms"], "update_ms": kl["update_ms"],
+            "assign_step": kl["assign"],
+            "pq_subspaces": dict(kl["pq"], config="8 sub-spaces d=16 k=256 niter=20 on the same "
+                                                  "points, one batched run"),
+        }
 
     if not args.no_cpu:
         cb = cpu_baseline_knn(base, queries, k)

@@ -32,6 +32,13 @@ extern "C" {
 #define KMEANS_INIT_USER 0x100000
 #define KMEANS_L1 0x200000   /* not served on the GPU: error */
 #define KMEANS_CHI2 0x400000 /* not served on the GPU: error */
+/* Assign-step precision; no counterpart in kmeans.h.  YAEL_KMEANS_BF16: BF16
+ * MFMA first pass keeping each point's 8 best centroids, exact FP32 re-rank
+ * and a certificate, FP32 search for uncertified points -- the result is the
+ * FP32 form's bit for bit.  YAEL_KMEANS_FP32: FP32 form only.  Neither: BF16
+ * when k >= 4096. */
+#define YAEL_KMEANS_BF16 0x1000000
+#define YAEL_KMEANS_FP32 0x2000000
 
 /* replaces kmeans.h:41-44 (kmeans.c:332-447) */
 float kmeans(int d, int n, int k, int niter, const float *v, int flags, long seed, int redo,
@@ -101,6 +108,10 @@ int yael_kmeans_checked(int d, int n, int k, int niter, const float *v, int flag
 int yael_kmeans_device(int d, int n, int k, int niter, const float *v_dev, int flags, long seed,
                        int redo, float *centroids, float *dis, int *assign, int *nassign,
                        float *ret);
+/* Counters of the last kmeans call in this process: Lloyd iterations,
+ * point assignments, assignments the BF16 first pass could not certify
+ * (searched again with the FP32 form).  No counterpart in the reference. */
+int yael_kmeans_last_stats(long *iters, long *assigned, long *fallback);
 int yael_knn_reorder_shortlist_checked(int n, int nb, int d, int k, const float *b,
                                        const float *v, int *idx, float *dis);
 
@@ -122,6 +133,15 @@ int yael_kmeans_batched_device(int d, int ninst, const int *n_inst, int k, int n
                                const float *v_dev, int flags, const long *seeds,
                                float *centroids, float *dis, int *assign, int *nassign,
                                float *ret, int *iters, int use_graph, int *chunk_used);
+/* Product-quantiser training: nsub k-means, sub-space j being the columns
+ * [j*d/nsub, (j+1)*d/nsub) of the n device-resident points, run as one
+ * batched k-means (instance j = sub-space j, seed seeds[j]).  Outputs as
+ * yael_kmeans_batched_device, sub-space-major.  No counterpart in the
+ * reference. */
+int yael_kmeans_subspaces_device(int d, int n, int nsub, int k, int niter, const float *v_dev,
+                                 int flags, const long *seeds, float *centroids, float *dis,
+                                 int *assign, int *nassign, float *ret, int *iters,
+                                 int *chunk_used);
 
 #ifdef __cplusplus
 }

@@ -119,3 +119,33 @@ def test_graph_and_direct_launches_agree(ya):
     V.free()
     assert np.array_equal(outs[0][0].view(np.int32), outs[1][0].view(np.int32))
     assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])
+
+
+def test_pq_subspaces_match_single_runs(ya):
+    """yael_kmeans_subspaces_device: sub-space j (columns 8j..8j+7, padded to
+    16 on the device) trained in one batched run equals kmeans() on that
+    column block."""
+    import ctypes
+    from yael_amd import device
+    rng = np.random.default_rng(4)
+    n, nsub, dsub, k, niter = 5000, 4, 8, 24, 7
+    v = mixture(rng, n, nsub * dsub, 50, 22)
+    V = device.DeviceSet(v)
+    cent = np.empty((nsub, k, dsub), np.float32)
+    dis = np.empty(nsub * n, np.float32)
+    asg = np.empty(nsub * n, np.int32)
+    nas = np.empty((nsub, k), np.int32)
+    ret = np.empty(nsub, np.float32)
+    it = np.empty(nsub, np.int32)
+    chunk = ctypes.c_int(0)
+    seeds = np.arange(31, 31 + nsub, dtype=np.int64)
+    rc = ya.lib().yael_kmeans_subspaces_device(nsub * dsub, n, nsub, k, niter, V.vec.ptr, 0x10000,
+                                               seeds.ctypes.data, cent.ctypes.data,
+                                               dis.ctypes.data, asg.ctypes.data, nas.ctypes.data,
+                                               ret.ctypes.data, it.ctypes.data,
+                                               ctypes.byref(chunk))
+    ya.check(rc, "subspaces")
+    V.free()
+    sub = np.concatenate([v[:, j * dsub:(j + 1) * dsub] for j in range(nsub)])
+    sizes = np.full(nsub, n, np.int32)
+    compare((cent, ret, dis, asg, nas), single(ya, sub, sizes, k, niter, seeds), sizes)

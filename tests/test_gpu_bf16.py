@@ -80,3 +80,41 @@ def test_small_base(dev):
     b = mixture(rng, 50, 16, 4, 10)
     q = mixture(rng, 70, 16, 4, 10)
     same_as_fp32(dev, q, b, 5, 64)
+
+
+def test_k1_top8(dev):
+    """The k-means assign form: k = 1 from 8 re-ranked candidates."""
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal((3000, 128)).astype(np.float32)
+    q = rng.standard_normal((5000, 128)).astype(np.float32)
+    nfb = same_as_fp32(dev, q, b, 1, 8)
+    print("fallback queries", nfb, flush=True)
+
+
+@pytest.mark.parametrize("k,kp", [(1, 8), (5, 16)])
+def test_many_queries_direct_merge(dev, k, kp):
+    """Enough query tiles for one base range: the lane pairs merge their
+    candidate lists in registers and write them directly."""
+    rng = np.random.default_rng(12 + k)
+    b = mixture(rng, 2048, 32, 64, 25, integer=False)
+    q = mixture(rng, 600000, 32, 64, 25, integer=False)
+    nfb = same_as_fp32(dev, q, b, k, kp)
+    print("k", k, "fallback queries", nfb, flush=True)
+
+
+def test_k1_no_value_below_1e30(dev):
+    """k = 1 takes only values below 1e30 (nn.c:404-440): queries whose every
+    distance is larger get (-1, 1e30), as in the FP32 form."""
+    rng = np.random.default_rng(13)
+    b = rng.standard_normal((200, 16)).astype(np.float32)
+    b[:100] *= np.float32(1e16)
+    q = rng.standard_normal((300, 16)).astype(np.float32)
+    q[:150] *= np.float32(1e16)
+    same_as_fp32(dev, q, b, 1, 8)
+    Q, B = dev.DeviceSet(q), dev.DeviceSet(b[:100])
+    i32, d32 = dev.knn_fp32(Q, B, 1)
+    i16, d16, nfb = dev.knn_bf16(Q, B, 1, 8)
+    Q.free()
+    B.free()
+    assert np.array_equal(i16, i32) and np.array_equal(d16.view(np.int32), d32.view(np.int32))
+    assert (i32[:, 0] == -1).any()

@@ -177,3 +177,42 @@ def test_kmeans_deterministic(ya):
     assert r1[1] == r2[1]
     assert np.array_equal(r1[0].view(np.int32), r2[0].view(np.int32))
     assert np.array_equal(r1[3], r2[3])
+
+
+def _kmeans_flags(ya, v, k, niter, seed, flags):
+    import ctypes
+    n, d = v.shape
+    c = np.empty((k, d), np.float32)
+    dis = np.empty(n, np.float32)
+    asg = np.empty(n, np.int32)
+    nas = np.empty(k, np.int32)
+    q = ctypes.c_float()
+    rc = ya.lib().yael_kmeans_checked(d, n, k, niter, v.ctypes.data, flags, seed, 1,
+                                     c.ctypes.data, dis.ctypes.data, asg.ctypes.data,
+                                     nas.ctypes.data, ctypes.byref(q))
+    ya.check(rc, "kmeans")
+    st = [ctypes.c_long() for _ in range(3)]
+    ya.lib().yael_kmeans_last_stats(*[ctypes.byref(x) for x in st])
+    return (c, q.value, dis, asg, nas), [x.value for x in st]
+
+
+@pytest.mark.parametrize("integer", [True, False])
+def test_bf16_assign_same_run_as_fp32(ya, integer):
+    """The precision form of the assign step (BF16 first pass, FP32 re-rank
+    of 8 candidates, certificate, FP32 search of the uncertified points)
+    gives the FP32 form's run bit for bit."""
+    rng = np.random.default_rng(21)
+    n, d, k = 40000, 64, 600
+    cen = rng.uniform(0, 255, (300, d))
+    v = np.clip(cen[rng.integers(0, 300, n)] + 15 * rng.standard_normal((n, d)), 0, 255)
+    v = (np.rint(v) if integer else v).astype(np.float32)
+    base = 0x10000 | 0x80000
+    a, st32 = _kmeans_flags(ya, v, k, 8, 5, base | 0x2000000)
+    b, st16 = _kmeans_flags(ya, v, k, 8, 5, base | 0x1000000)
+    print("iters", st32[0], st16[0], "assigned", st16[1], "fallback", st16[2], flush=True)
+    assert st32[2] == 0 and st16[0] == st32[0]
+    assert np.array_equal(a[3], b[3]) and np.array_equal(a[4], b[4])
+    assert np.array_equal(a[0].view(np.int32), b[0].view(np.int32))
+    assert np.array_equal(a[2].view(np.int32), b[2].view(np.int32))
+    assert a[1] == b[1]
+    assert st16[2] < st16[1]

@@ -38,6 +38,12 @@ PROTOS = {
     "yael_kmeans_batched": (I, [I, I, vp, I, I, vp, I, vp, vp, vp, vp, vp, vp]),
     "yael_kmeans_batched_device": (I, [I, I, vp, I, I, vp, I, vp, vp, vp, vp, vp, vp, vp, I,
                                        vp]),
+    "yael_kmeans_last_stats": (I, [C.POINTER(L), C.POINTER(L), C.POINTER(L)]),
+    "yael_kmeans_subspaces_device": (I, [I, I, I, I, I, vp, I, vp, vp, vp, vp, vp, vp, vp,
+                                         vp]),
+    "yk_copy_cols": (I, [vp, I, vp, I, L, I]),
+    "yk_synth": (I, [vp, L, I, I, I, C.c_float, C.c_float, vp, I, C.c_float, I, C.c_ulonglong,
+                     I]),
     "yk_gather_rows": (I, [vp, vp, L, I, vp]),
     "yk_rerank_shortlist": (I, [L, I, I, vp, vp, vp, vp]),
     "knn_reorder_shortlist": (None, [I, I, I, I, vp, vp, vp, vp]),

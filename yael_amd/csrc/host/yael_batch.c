@@ -262,10 +262,14 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   char key[256];
   snprintf(key, sizeof key, "batch_chunk:d=%d:k=%d:ninst=%d:ntot=%ld", d, k, ninst, s.ntot);
   if (rc == YK_OK && !yh_tune_get(key, &P)) {
-    const int cand[4] = {256, 512, 1024, 2048};
+    /* candidates: enough chunks to give every CU work (>= 512), partial
+       sums within 8 GB; the smallest chunk is always a candidate */
+    const int cand[7] = {256, 512, 1024, 2048, 4096, 8192, 16384};
     float best = 1e30f;
-    for (int c = 0; c < 4 && rc == YK_OK; c++) {
-      if ((size_t)cand[c] > (size_t)s.ntot && c > 0) break;
+    for (int c = 0; c < 7 && rc == YK_OK; c++) {
+      long nch = 0;
+      for (int i = 0; i < ninst; i++) nch += (n_inst[i] + cand[c] - 1) / cand[c];
+      if (c > 0 && (nch < 512 || (double)nch * k * d * 8 > 8e9)) continue;
       yh_allocs tal;
       tal.n = 0;
       yk_batch *bt = NULL;
@@ -376,6 +380,39 @@ int yael_kmeans_batched_device(int d, int ninst, const int *n_inst, int k, int n
   yh_lock();
   int rc = yh_kmeans_batched(d, ninst, n_inst, k, niter, v_dev, 1, flags, seeds, centroids, dis,
                              assign, nassign, ret, iters, use_graph, chunk_used);
+  yh_unlock();
+  return rc;
+}
+
+/* Product-quantiser training: nsub k-means, sub-space j being columns
+ * [j*dsub, (j+1)*dsub) of the n device-resident points (padded rows of
+ * yk_ld(d) floats, d = nsub*dsub), all advanced in one batched run (instance
+ * j = sub-space j, seed seeds[j]).  Outputs as yael_kmeans_batched_device
+ * with instances sub-space-major: centroids nsub*k*dsub, dis/assign nsub*n,
+ * nassign nsub*k, ret/iters nsub. */
+int yael_kmeans_subspaces_device(int d, int n, int nsub, int k, int niter, const float *v_dev,
+                                 int flags, const long *seeds, float *centroids, float *dis,
+                                 int *assign, int *nassign, float *ret, int *iters,
+                                 int *chunk_used) {
+  if (nsub < 1 || d % nsub != 0 || n < 1)
+    return yk_set_error_c(YK_EINVAL, "subspaces: d=%d is not a multiple of nsub=%d", d, nsub);
+  const int dsub = d / nsub, ld = yk_ld(d), lds = yk_ld(dsub);
+  yh_lock();
+  int rc = yk_init();
+  float *vs = NULL;
+  int *sizes = (int *)malloc(sizeof(int) * nsub);
+  if (rc == YK_OK && !sizes) rc = yk_set_error_c(YK_ENOMEM, "host malloc");
+  if (rc == YK_OK) rc = yk_malloc((void **)&vs, sizeof(float) * (size_t)nsub * n * lds);
+  if (rc == YK_OK && lds != dsub) rc = yk_memset(vs, 0, sizeof(float) * (size_t)nsub * n * lds);
+  for (int j = 0; j < nsub && rc == YK_OK; j++) {
+    sizes[j] = n;
+    rc = yk_copy_cols(vs + (size_t)j * n * lds, lds, v_dev + (size_t)j * dsub, ld, n, dsub);
+  }
+  if (rc == YK_OK)
+    rc = yh_kmeans_batched(dsub, nsub, sizes, k, niter, vs, 1, flags, seeds, centroids, dis,
+                           assign, nassign, ret, iters, 1, chunk_used);
+  if (vs) yk_free(vs);
+  free(sizes);
   yh_unlock();
   return rc;
 }

@@ -8,7 +8,7 @@
 enum {
   YH_B = 0, YH_Q, YH_BN, YH_QN, YH_W, YH_OUTI, YH_OUTD,
   YH_KM_V, YH_KM_QN, YH_KM_C, YH_KM_CN, YH_KM_A, YH_KM_D, YH_KM_CNT, YH_KM_WS,
-  YH_KM_BEST_C, YH_KM_SEL,
+  YH_KM_BEST_C, YH_KM_SEL, YH_KM_VH, YH_KM_QS, YH_KM_CH, YH_KM_CS,
   YH_NSLOTS
 };
 
@@ -35,6 +35,13 @@ int yh_kmeans(int d, int n, int k, int niter, const float *v, int v_on_device, i
 #define KMEANS_INIT_USER 0x100000
 #define KMEANS_L1 0x200000
 #define KMEANS_CHI2 0x400000
+/* assign-step precision (not in kmeans.h): BF16 first pass + exact FP32
+ * re-rank of the 8 best (results identical to the FP32 form), or the FP32
+ * form only; neither set: BF16 when k >= YH_KM_BF16_MIN_K */
+#define YAEL_KMEANS_BF16 0x1000000
+#define YAEL_KMEANS_FP32 0x2000000
+#define YH_KM_BF16_MIN_K 4096
+#define YH_KM_KPRIME 8
 
 /* empty-cluster split (kmeans.c:166-209) on host copies of the centroids */
 int yh_reassign_empty(int d, int k, float *cent, long ldc, const int *nassign, unsigned seed);

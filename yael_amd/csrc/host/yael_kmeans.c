@@ -10,6 +10,13 @@
  * (kmeans.c:300-313) and the best-of-redo bookkeeping (kmeans.c:416-428).
  * Per iteration the host reads back k counts and one double; the centroid
  * matrix crosses the bus only in an iteration that splits a cluster.
+ *
+ * Assign precision: for large k the assign step takes the precision form
+ * (yk_knn_bf16 with k = 1, k' = 8): a BF16 MFMA pass keeps each point's 8
+ * best centroids, the FP32 form's own arithmetic re-ranks them and a bound
+ * on the BF16 error certifies that the FP32 form's nearest centroid is among
+ * them; uncertified points are searched again with the FP32 form.  The
+ * result is the FP32 form's, bit for bit, either way.
  */
 #include <math.h>
 #include <stdio.h>
@@ -103,7 +110,26 @@ typedef struct {
   double *qn, *obj;
   int *assign, *counts, *sel;
   void *ws;
+  /* precision form (bf16 != 0): BF16 copies and their statistics */
+  int bf16;
+  uint16_t *vh, *ch;
+  float *vs, *cs, *csmax;
 } yh_km_dev;
+
+/* counters of the last kmeans call (yael_kmeans_last_stats) */
+static long g_km_iters, g_km_assigned, g_km_fallback;
+
+static int yh_km_alloc_bf16(yh_km_dev *m, long n, int k, int d) {
+  const int ldh = yk_bf16_ld(d, 2);
+  long kp = (k + 31) & ~31;
+  m->vh = (uint16_t *)yh_buf(YH_KM_VH, sizeof(uint16_t) * (size_t)n * ldh);
+  m->vs = (float *)yh_buf(YH_KM_QS, sizeof(float) * 4 * (size_t)n);
+  m->ch = (uint16_t *)yh_buf(YH_KM_CH, sizeof(uint16_t) * (size_t)kp * ldh);
+  m->cs = (float *)yh_buf(YH_KM_CS, sizeof(float) * 4 * (size_t)(kp + 1));
+  if (!m->vh || !m->vs || !m->ch || !m->cs) return YK_ENOMEM;
+  m->csmax = m->cs + 4 * kp;
+  return YK_OK;
+}
 
 static int yh_km_alloc(yh_km_dev *m, long n, int k, int d, int v_on_device) {
   int ld = yk_ld(d);
@@ -135,9 +161,19 @@ static int yh_kmeans_core(int d, int n, int k, int niter, int flags, int verbose
   for (int iter = 1; iter <= niter; iter++) {
     (*iter_tot)++;
     if ((rc = yk_norms(m->c, k, d, m->cn, NULL)) != YK_OK) return rc;
-    if ((rc = yk_knn_f32(2, n, k, d, 1, m->v, m->qn, m->c, m->cn, NULL, m->assign, m->dis)) !=
-        YK_OK)
+    if (m->bf16) {
+      int nfb = 0;
+      if ((rc = yk_to_bf16(m->c, k, d, 2, 0, m->ch, m->cs, m->csmax)) != YK_OK) return rc;
+      if ((rc = yk_knn_bf16(2, n, k, d, 1, YH_KM_KPRIME, m->v, m->qn, m->vh, m->vs, m->c, m->cn,
+                            m->ch, m->csmax, m->assign, m->dis, &nfb)) != YK_OK)
+        return rc;
+      g_km_fallback += nfb;
+    } else if ((rc = yk_knn_f32(2, n, k, d, 1, m->v, m->qn, m->c, m->cn, NULL, m->assign,
+                                m->dis)) != YK_OK) {
       return rc;
+    }
+    g_km_assigned += n;
+    g_km_iters++;
     if ((rc = yk_kmeans_update(n, k, d, m->v, m->assign, m->c, m->counts,
                                (flags & KMEANS_NORMALIZE_CENTS) != 0, m->ws)) != YK_OK)
       return rc;
@@ -187,13 +223,21 @@ int yh_kmeans(int d, int n, int k, int niter, const float *v, int v_on_device, i
   int rc = yk_init();
   if (rc != YK_OK) return rc;
   const int ld = yk_ld(d);
-  yh_km_dev m;
+  yh_km_dev m = {0};
   if ((rc = yh_km_alloc(&m, n, k, d, v_on_device)) != YK_OK) return rc;
   if (v_on_device)
     m.v = (float *)v; /* caller's padded device vectors */
   else if ((rc = yk_upload_vectors(m.v, v, n, d)) != YK_OK)
     return rc;
   if ((rc = yk_norms(m.v, n, d, NULL, m.qn)) != YK_OK) return rc;
+  g_km_iters = g_km_assigned = g_km_fallback = 0;
+  m.bf16 = (flags & YAEL_KMEANS_BF16) ||
+           (!(flags & YAEL_KMEANS_FP32) && k >= YH_KM_BF16_MIN_K);
+  if (m.bf16) {
+    /* the points' BF16 copy is made once per call */
+    if ((rc = yh_km_alloc_bf16(&m, n, k, d)) != YK_OK) return rc;
+    if ((rc = yk_to_bf16(m.v, n, d, 2, 1, m.vh, m.vs, NULL)) != YK_OK) return rc;
+  }
 
   int *hcounts = (int *)malloc(sizeof(int) * k);
   float *hcent = (float *)malloc(sizeof(float) * (size_t)k * ld);
@@ -319,4 +363,16 @@ int yael_kmeans_device(int d, int n, int k, int niter, const float *v_dev, int f
                      ret);
   yh_unlock();
   return rc;
+}
+
+/* Counters of the last kmeans call on this process: Lloyd iterations run,
+ * point assignments made, and assignments the precision form could not
+ * certify (searched again with the FP32 form). */
+int yael_kmeans_last_stats(long *iters, long *assigned, long *fallback) {
+  yh_lock();
+  if (iters) *iters = g_km_iters;
+  if (assigned) *assigned = g_km_assigned;
+  if (fallback) *fallback = g_km_fallback;
+  yh_unlock();
+  return 0;
 }

@@ -50,6 +50,8 @@ int yk_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int yk_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int yk_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int yk_memset(void *dst, int value, size_t bytes);
+/* device 2-D copy: columns [0, ncols) of n rows, row strides ldd / lds floats */
+int yk_copy_cols(float *dst, int ldd, const float *src, int lds, long n, int ncols);
 int yk_sync(void);
 
 /* Norm kernel: out_f[i] = float-accumulated ||x_i||^2 (nn.c:108-114) when
@@ -81,6 +83,15 @@ int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, const float *q
                 const double *qn, const uint16_t *qh, const float *qstats, const float *b,
                 const float *bn, const uint16_t *bh, const float *bstats_max, int *assign,
                 float *dis, int *n_fallback);
+
+/* Synthetic sets generated in HBM (counter-based, reproducible from seed):
+ * n rows of stride ld, pad columns zero.  kind 0: uniform [lo, hi); 1:
+ * mixture -- centre of a hashed component (centres: ncomp x d on the device)
+ * + sigma * N(0,1), clipped to [lo, hi], rounded when round_values; 2:
+ * |N(0,1)|.  unit_rows scales every row to unit L2 norm. */
+int yk_synth(float *out, long n, int d, int ld, int kind, float lo, float hi,
+             const float *centres, int ncomp, float sigma, int round_values,
+             unsigned long long seed, int unit_rows);
 
 /* Cross distances, dist2[i + ldd*j] for a (na, float norms) x b (nb, double
  * norms), types 2/12/16 (nn.c:92-129, 266-306). */

@@ -117,11 +117,13 @@ __global__ __launch_bounds__(256) void yk_bupd_reduce_kernel(BatchDev b) {
   if (b.done[inst]) return;
   const long u0 = b.uitem_first[inst], u1 = b.uitem_first[inst + 1];
   int cnt = 0;
+#pragma unroll 8
   for (long u = u0; u < u1; u++) cnt += b.pcnt[u * b.k + c];
   const double inv = 1.0 / (double)cnt;
   float *crow = b.cent + ((long)inst * b.kpad + c) * b.ld;
   for (int t = lane; t < b.d; t += 64) {
     double s = 0.0;
+#pragma unroll 8
     for (long u = u0; u < u1; u++) s += b.psum[(u * b.k + c) * (size_t)b.d + t];
     crow[t] = (float)(s * inv);
   }

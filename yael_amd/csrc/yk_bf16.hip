@@ -157,7 +157,8 @@ struct Bf16Args {
   uint64_t *part;       // MODE 2: [nq][nsplit][2][R]
   const uint64_t *thr;  // MODE 3
   int *cnt;
-  uint64_t *cand;
+  uint64_t *cand;       // MODE 3: candidate lists; MODE 2 with kp_direct: [nq][kp]
+  int kp_direct;        // MODE 2, nsplit == 1: the lane pair's kp <= R smallest keys
 };
 
 template <int R>
@@ -338,7 +339,38 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
         a.part[(qi * a.nsplit + split) * 2 + h] = bi[j] < 0 ? YK_KEY_NONE : yk_key(bv[j], bi[j]);
     }
   }
-  if (MODE == 2) {
+  if (MODE == 2 && a.kp_direct) {
+    // lanes r and r + 32 hold sorted top-R lists of disjoint rows of the same
+    // query: the first half-cleaner of the bitonic sequence (mine, partner's
+    // reversed) keeps the R smallest of the union, R/2..1 cleaners sort them
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      uint64_t o[R], m[R];
+#pragma unroll
+      for (int s = 0; s < R; s++) {
+        const uint32_t lo = __shfl_xor((uint32_t)top[j][s], 32);
+        const uint32_t hi = __shfl_xor((uint32_t)(top[j][s] >> 32), 32);
+        o[s] = ((uint64_t)hi << 32) | lo;
+      }
+#pragma unroll
+      for (int s = 0; s < R; s++) m[s] = top[j][s] < o[R - 1 - s] ? top[j][s] : o[R - 1 - s];
+#pragma unroll
+      for (int w = R / 2; w >= 1; w >>= 1)
+#pragma unroll
+        for (int s = 0; s < R; s++)
+          if ((s & w) == 0) {
+            const uint64_t x = m[s], y = m[s + w];
+            m[s] = x < y ? x : y;
+            m[s + w] = x < y ? y : x;
+          }
+      const long qi = qbase + j * 32 + r;
+      if (h == 0 && qok[j]) {
+#pragma unroll
+        for (int s = 0; s < R; s++)
+          if (s < a.kp_direct) a.cand[qi * a.kp_direct + s] = m[s];
+      }
+    }
+  } else if (MODE == 2) {
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       if (!qok[j]) continue;
@@ -471,7 +503,8 @@ __global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
         if (ck != YK_KEY_NONE) {
           const int bi = (int)(uint32_t)ck;
           float v = dot ? acc[i] : (float)(qnd + (double)bn[bi]) - 2.0f * acc[i];
-          if (v == v) xk = yk_key(v, bi);
+          // k = 1 follows nn.c:404-440: only values below 1e30 are taken
+          if (v == v && (k > 1 || v < 1e30f)) xk = yk_key(v, bi);
         }
         ex[cidx] = xk;
       }
@@ -483,11 +516,13 @@ __global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
   for (int i = kp + lane; i < p2; i += 64) ex[i] = YK_KEY_NONE;
   __syncthreads();
   yk_bitonic_b(ex, p2);
+  // no result: (-1, 1e30) for k = 1 (nn.c:404), (-1, NaN bits) padding for
+  // k > 1 (nn.c:515-518)
+  const float none_d = k == 1 ? 1e30f : __uint_as_float(0xffffffffu);
   for (int i = lane; i < k; i += 64) {
     uint64_t key = ex[i];
     out_i[qi * k + i] = key == YK_KEY_NONE ? -1 : (int)(uint32_t)key;
-    out_d[qi * k + i] =
-        key == YK_KEY_NONE ? __uint_as_float(0xffffffffu) : yk_fkey_inv((uint32_t)(key >> 32));
+    out_d[qi * k + i] = key == YK_KEY_NONE ? none_d : yk_fkey_inv((uint32_t)(key >> 32));
   }
   if (lane == 0) {
     // the first pass kept the kp smallest keys; fewer than kp real ones means
@@ -615,10 +650,26 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   void *vpart;
   if (kp <= YK_REG_R) {
     BPlan p = yk_bplan(nq, a.nb, 2, 1, 1, 256);
-    YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * p.nsplit * 2 * YK_REG_R, &vpart));
+    // enough query tiles to fill the chip: no base split, and the lane pairs
+    // write their merged lists straight to cand_out
+    if (p.tiles >= 2048) p = yk_bplan(nq, a.nb, 2, 1, 1, 1);
+    const int R = kp <= 8 ? 8 : YK_REG_R;
+    if (p.nsplit == 1) {
+      a.cand = cand_out;
+      a.kp_direct = kp;
+      if (R == 8)
+        YK_TRY((yk_launch_bscan<2, 8>(p, a)));
+      else
+        YK_TRY((yk_launch_bscan<2, YK_REG_R>(p, a)));
+      return YK_OK;
+    }
+    YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * p.nsplit * 2 * R, &vpart));
     a.part = (uint64_t *)vpart;
-    YK_TRY((yk_launch_bscan<2, YK_REG_R>(p, a)));
-    int n = p.nsplit * 2 * YK_REG_R, p2 = 1;
+    if (R == 8)
+      YK_TRY((yk_launch_bscan<2, 8>(p, a)));
+    else
+      YK_TRY((yk_launch_bscan<2, YK_REG_R>(p, a)));
+    int n = p.nsplit * 2 * R, p2 = 1;
     while (p2 < n) p2 <<= 1;
     hipLaunchKernelGGL(yk_bf_sortsel_kernel, dim3((unsigned)nq), dim3(256), sizeof(uint64_t) * p2,
                        yk_stream(), a.part, n, kp, p2, cand_out, (uint64_t *)nullptr);
@@ -682,7 +733,9 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
   if (k < 1 || k > nb || kp > YK_KMAX)
     return yk_set_error(YK_EINVAL, "k=%d k'=%d out of range (nb=%ld, max %d)", k, kp, nb, YK_KMAX);
   if (nq <= 0) return YK_OK;
-  const long QB = 16384;
+  // query batches bound the candidate memory: kp <= 16 keeps kp keys per
+  // query, larger kp a YK_CAND_CAP list per query during the filtered pass
+  const long QB = kp <= YK_REG_R ? (1L << 20) : 16384;
   void *vcand, *vunc;
   const long qb = nq < QB ? nq : QB;
   YK_TRY(yk_bscratch(4, sizeof(uint64_t) * qb * kp, &vcand));

@@ -152,6 +152,15 @@ extern "C" int yk_upload_vectors(float *dst, const float *src, long n, int d) {
   return YK_OK;
 }
 
+extern "C" int yk_copy_cols(float *dst, int ldd, const float *src, int lds, long n, int ncols) {
+  YK_TRY(yk_init());
+  if (n <= 0 || ncols <= 0) return YK_OK;
+  if (ncols > ldd || ncols > lds) return yk_set_error(YK_EINVAL, "yk_copy_cols: ncols > stride");
+  YK_HIP(hipMemcpy2DAsync(dst, sizeof(float) * ldd, src, sizeof(float) * lds,
+                          sizeof(float) * ncols, n, hipMemcpyDeviceToDevice, g_stream));
+  return YK_OK;
+}
+
 extern "C" int yk_download_vectors(float *dst, const float *src, long n, int d) {
   YK_TRY(yk_init());
   if (n <= 0) return YK_OK;

@@ -40,16 +40,38 @@ class DeviceBuffer:
 class DeviceSet:
     """n vectors of dimension d resident in HBM (float32, padded rows)."""
 
-    def __init__(self, x):
-        x = np.ascontiguousarray(x, np.float32)
-        self.n, self.d = x.shape
+    def __init__(self, x=None, shape=None):
+        """Upload host array x (n, d), or allocate an uninitialised set of
+        shape (n, d) (filled on the device, e.g. by ``synth``)."""
+        if x is not None:
+            x = np.ascontiguousarray(x, np.float32)
+            shape = x.shape
+        self.n, self.d = int(shape[0]), int(shape[1])
         L = lib()
         self.ld = L.yk_ld(self.d)
         self.vec = DeviceBuffer(self.n * self.ld * 4)
-        check(L.yk_upload_vectors(self.vec.ptr, x.ctypes.data_as(vp), self.n, self.d), "upload")
+        if x is not None:
+            check(L.yk_upload_vectors(self.vec.ptr, x.ctypes.data_as(vp), self.n, self.d),
+                  "upload")
         self.fnorm = DeviceBuffer(((self.n + 31) // 32 * 32) * 4)  # float-accumulated (base role)
         self.dnorm = DeviceBuffer(self.n * 8)                      # double-accumulated (query role)
         self.bf = {}
+
+    @classmethod
+    def synth(cls, n, d, kind, seed, lo=0.0, hi=1.0, centres=None, sigma=0.0, round_values=False,
+              unit_rows=False):
+        """A synthetic set generated in HBM (yk_synth): kind 0 uniform
+        [lo, hi), 1 mixture around the rows of ``centres`` (a DeviceSet with
+        ld == d) clipped to [lo, hi], 2 |N(0,1)|."""
+        s = cls(shape=(n, d))
+        cptr, ncomp = None, 0
+        if centres is not None:
+            if centres.ld != centres.d or centres.d != d:
+                raise ValueError("centres must be a DeviceSet of dimension d with ld == d")
+            cptr, ncomp = centres.vec.ptr, centres.n
+        check(lib().yk_synth(s.vec.ptr, n, d, s.ld, kind, lo, hi, cptr, ncomp, sigma,
+                             int(round_values), seed, int(unit_rows)), "synth")
+        return s
 
     def compute_norms(self):
         check(lib().yk_norms(self.vec.ptr, self.n, self.d, self.fnorm.ptr, self.dnorm.ptr), "norms")
