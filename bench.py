@@ -410,6 +410,9 @@ def main():
     ap.add_argument("--no-large", action="store_true",
                     help="skip the d=960 precision-form and n=1e7, k=65536 k-means extras")
     ap.add_argument("--only-bf16", action="store_true", help="profile runs: the precision form only")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="profile runs: first copy this many MiB three times with a plain "
+                         "copy kernel (the reference point of the HBM traffic counters)")
     ap.add_argument("--nb", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--kprime", type=int, default=200)
@@ -423,6 +426,16 @@ def main():
     from yael_amd import device as dev
     ya.init()
     print("# device:", ya.device_info(), file=sys.stderr, flush=True)
+
+    if args.calib_copy_mb:
+        nbytes = args.calib_copy_mb << 20
+        src, dst = dev.DeviceBuffer(nbytes), dev.DeviceBuffer(nbytes)
+        ya.check(ya.lib().yk_memset(src.ptr, 0, nbytes))
+        for _ in range(3):
+            ya.check(ya.lib().yk_copy_kernel(dst.ptr, src.ptr, nbytes))
+        ya.lib().yk_sync()
+        src.free()
+        dst.free()
 
     d, k = 128, 100
     t = time.perf_counter()

@@ -42,6 +42,8 @@ PROTOS = {
     "yael_kmeans_subspaces_device": (I, [I, I, I, I, I, vp, I, vp, vp, vp, vp, vp, vp, vp,
                                          vp]),
     "yk_copy_cols": (I, [vp, I, vp, I, L, I]),
+    "yk_copy_kernel": (I, [vp, vp, C.c_size_t]),
+    "yk_bf16_bytes": (L, [L, I, I, I]),
     "yk_synth": (I, [vp, L, I, I, I, C.c_float, C.c_float, vp, I, C.c_float, I, C.c_ulonglong,
                      I]),
     "yk_gather_rows": (I, [vp, vp, L, I, vp]),

@@ -50,6 +50,8 @@ int yk_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int yk_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int yk_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int yk_memset(void *dst, int value, size_t bytes);
+/* device copy by a plain kernel (16 bytes per lane; bytes % 16 == 0) */
+int yk_copy_kernel(void *dst, const void *src, size_t bytes);
 /* device 2-D copy: columns [0, ncols) of n rows, row strides ldd / lds floats */
 int yk_copy_cols(float *dst, int ldd, const float *src, int lds, long n, int ncols);
 int yk_sync(void);
@@ -77,6 +79,11 @@ int yk_knn_f32(int type, long nq, long nb, int d, int k, const float *q,
  * certifies the result; uncertified queries are searched with the FP32 form
  * (*n_fallback of them).  Output as yk_knn_f32. */
 int yk_bf16_ld(int d, int type);
+/* bytes of a BF16 copy of n vectors: queries row-major (n rows of
+ * yk_bf16_ld), base fragment-major (rows padded to a multiple of 32; block
+ * b, chunk c is the 1 KB MFMA fragment at ((b * ldh/16 + c) * 64) * 8
+ * elements, lane h*32 + r holding row 32b + r, columns 16c + 8h .. +7) */
+long yk_bf16_bytes(long n, int d, int type, int is_query);
 int yk_to_bf16(const float *x, long n, int d, int type, int is_query, uint16_t *out,
                float *stats, float *stats_max);
 int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, const float *q,

@@ -13,6 +13,14 @@
 // (type 16: plain bf(q).bf(b)).  The epilogue is a compare and a rare
 // append: v_mfma_f32_32x32x16_bf16 does 16 k-steps per instruction.
 //
+// Layout.  Query copies are row-major (row stride d').  Base copies are
+// fragment-major, the order in which the MFMA consumes them: the rows are
+// taken in blocks of 32 (the set padded with zero rows to a multiple of 32);
+// block b, 16-wide chunk c is one 1 KB fragment at ((b * nch + c) * 64) * 8
+// elements, lane h * 32 + r of a wave holding row 32b + r, columns 16c + 8h
+// .. 16c + 8h + 7.  A wave reads a fragment with one fully coalesced 1 KB
+// load (a row-major copy would touch 32 rows' 32-byte pieces per load).
+//
 // Re-rank.  Each candidate's distance is recomputed with the FP32 form's own
 // instruction sequence (v_mfma_f32_32x32x2_f32, same k order, same
 // norm/epilogue arithmetic), so re-ranked values are bit-identical to what the
@@ -38,6 +46,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 #define YK_CAND_CAP 4096
 #define YK_REG_R 16
+#define YK_CSTAGE 8  // MODE 3: keys staged in LDS per (lane, query tile)
 
 // ---------------------------------------------------------------------------
 // conversion
@@ -55,6 +64,19 @@ extern "C" int yk_bf16_ld(int d, int type) {
   return (d + extra + 15) & ~15;
 }
 
+extern "C" long yk_bf16_bytes(long n, int d, int type, int is_query) {
+  const long rows = is_query ? n : (n + 31) & ~31L;
+  return rows * yk_bf16_ld(d, type) * (long)sizeof(uint16_t);
+}
+
+// element (i, j) of a BF16 copy: row-major for queries, fragment-major for
+// the base
+__device__ __forceinline__ long yk_bf_at(long i, int j, int ldh, bool frag) {
+  if (!frag) return i * ldh + j;
+  const int nch = ldh >> 4;
+  return (((i >> 5) * nch + (j >> 4)) * 64 + ((j >> 3) & 1) * 32 + (i & 31)) * 8 + (j & 7);
+}
+
 // One wave per vector (coalesced): BF16 row (augmented for L2) + error
 // statistics stats[i] = {|x|, |bf(x)|, |x - bf(x)|, |x|^2} (float); the
 // per-set maxima go to smax (uint bit patterns of non-negative floats, whose
@@ -68,16 +90,21 @@ __global__ __launch_bounds__(256) void yk_to_bf16_kernel(const float *__restrict
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __shared__ float4 wmax[4];
   float4 mx = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (long i = blockIdx.x * 4L + wave; i < n; i += gridDim.x * 4L) {
+  const bool frag = !is_query;
+  const long nrows = frag ? (n + 31) & ~31L : n;
+  for (long i = blockIdx.x * 4L + wave; i < nrows; i += gridDim.x * 4L) {
+  if (i >= n) {  // zero pad rows of the fragment-major base copy
+    for (int j = lane; j < ldh; j += 64) out[yk_bf_at(i, j, ldh, true)] = 0;
+    continue;
+  }
   const float *row = x + i * ld;
-  uint16_t *o = out + i * ldh;
   double s2 = 0, sb = 0, se = 0;
   const float scale = (type != YK_DOT && is_query) ? -2.0f : 1.0f;
   for (int j = lane; j < d; j += 64) {
     float v = row[j];
     uint16_t hb = yk_bf16_rne(v * scale);
     float vb = yk_bf16_f(hb) / scale;  // exact: scale is a power of two
-    o[j] = hb;
+    out[yk_bf_at(i, j, ldh, frag)] = hb;
     s2 += (double)v * v;
     sb += (double)vb * vb;
     se += ((double)v - vb) * ((double)v - vb);
@@ -104,7 +131,8 @@ __global__ __launch_bounds__(256) void yk_to_bf16_kernel(const float *__restrict
     }
     extra = 6;
   }
-  for (int j = d + lane; j < ldh; j += 64) o[j] = (j - d) < extra ? tail[j - d] : 0;
+  for (int j = d + lane; j < ldh; j += 64)
+    out[yk_bf_at(i, j, ldh, frag)] = (j - d) < extra ? tail[j - d] : 0;
   float4 s = make_float4((float)sqrt(s2), (float)sqrt(sb), (float)sqrt(se), (float)s2);
   if (lane == 0) stats[i] = s;
   mx = make_float4(fmaxf(mx.x, s.x), fmaxf(mx.y, s.y), fmaxf(mx.z, s.z), fmaxf(mx.w, s.w));
@@ -134,7 +162,7 @@ extern "C" int yk_to_bf16(const float *x, long n, int d, int type, int is_query,
   const int ld = yk_ld(d), ldh = yk_bf16_ld(d, type);
   if (stats_max) hipLaunchKernelGGL(yk_zero4_kernel, dim3(1), dim3(64), 0, yk_stream(),
                                     (unsigned *)stats_max);
-  long blocks = (n + 3) / 4;
+  long blocks = (n + 31 + 3) / 4;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(yk_to_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, yk_stream(),
                      x, n, d, ld, ldh, type, is_query, out, (float4 *)stats,
@@ -171,111 +199,57 @@ __device__ __forceinline__ void yk_reg_insert16(uint64_t (&t)[R], uint64_t x) {
   t[0] = x < t[0] ? x : t[0];
 }
 
-// NCHMAX > 0: queries register-resident, at most NCHMAX 16-wide chunks
-// (ldh <= 16 * NCHMAX); NCHMAX = 0: both operands streamed from memory.
-template <int NB, int NCHMAX, int MODE, int R>
-__global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
-  constexpr bool QRES = NCHMAX > 0;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  const int r = lane & 31, h = lane >> 5;
-  // persistent grid, XCD-contiguous runs of work items (as in yk_select.hip)
-  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, wpx = gridDim.x >> 3;
-  const int ibeg = xcd * a.per_xcd, iend = min(a.total, ibeg + a.per_xcd);
-  for (int item = ibeg + local; item < iend; item += wpx) {
-  const int tile = item % a.tiles_x, split = item / a.tiles_x;
-  const long qbase = ((long)tile * nwave + wave) * (32 * NB);
-  const long bbeg = (long)split * a.brange;
-  const long bend = min(a.nb, bbeg + a.brange);
-  const int nch = a.ldh >> 4;
-  const int ldh = a.ldh;
-
+// Per-wave selection state over NB 32-query column tiles: lane (r, h) owns
+// query r of each tile and, per 32-row block, rows 8g + 4h + e (g, e < 4).
+template <int NB, int MODE, int R>
+struct BfSel {
   long qrow[NB];
   bool qok[NB];
   uint64_t top[NB][MODE == 2 ? R : 1];
   float bv[NB];
   int bi[NB];
-#pragma unroll
-  for (int j = 0; j < NB; j++) {
-    bv[j] = INFINITY;
-    bi[j] = -1;
-  }
   uint64_t T[NB];
   float Tv[NB];
+  // MODE 3: candidate keys are staged in LDS, YK_CSTAGE per (query tile,
+  // lane), and appended to the query's global list a region at a time (one
+  // atomic per region instead of one per key)
+  uint64_t *cs;
+  int ncs[NB];
+  int lane;
+
+  __device__ __forceinline__ void init(const Bf16Args &a, long qbase, int r, int lane_,
+                                       uint64_t *cs_wave) {
+    lane = lane_;
+    cs = cs_wave;
 #pragma unroll
-  for (int j = 0; j < NB; j++) {
-    long qi = qbase + j * 32 + r;
-    qok[j] = qi < a.nq;
-    qrow[j] = qok[j] ? qi : a.nq - 1;
+    for (int j = 0; j < NB; j++) {
+      ncs[j] = 0;
+      const long qi = qbase + j * 32 + r;
+      qok[j] = qi < a.nq;
+      qrow[j] = qok[j] ? qi : a.nq - 1;
+      bv[j] = INFINITY;
+      bi[j] = -1;
 #pragma unroll
-    for (int s = 0; s < (MODE == 2 ? R : 1); s++) top[j][s] = YK_KEY_NONE;
-    T[j] = MODE == 3 ? a.thr[qrow[j]] : 0;
-    Tv[j] = (T[j] == 0) ? -INFINITY
-                        : (T[j] == YK_KEY_NONE ? INFINITY : yk_fkey_inv((uint32_t)(T[j] >> 32)));
+      for (int s = 0; s < (MODE == 2 ? R : 1); s++) top[j][s] = YK_KEY_NONE;
+      T[j] = MODE == 3 ? a.thr[qrow[j]] : 0;
+      Tv[j] = (T[j] == 0) ? -INFINITY
+                          : (T[j] == YK_KEY_NONE ? INFINITY : yk_fkey_inv((uint32_t)(T[j] >> 32)));
+    }
   }
-  if (MODE == 3) {
+
+  // MODE 3: does any owned query still need the filtered scan?
+  __device__ __forceinline__ bool active() const {
     bool act = false;
 #pragma unroll
     for (int j = 0; j < NB; j++) act |= qok[j] && T[j] != 0;
-    if (!__any(act)) continue;
+    return act;
   }
 
-  bf16x8 qreg[NB][QRES ? NCHMAX : 1];
-  if (QRES) {
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-#pragma unroll
-      for (int c = 0; c < NCHMAX; c++)
-        if (c < nch) qreg[j][c] = *(const bf16x8 *)(a.q + qrow[j] * ldh + c * 16 + h * 8);
-  }
-
-  // base operand of the next block is loaded while the current one is
-  // multiplied (register double buffer; QRES only, d' <= 256)
-  bf16x8 anext[QRES ? NCHMAX : 1];
-  if (QRES && bbeg < bend) {
-    const uint16_t *bp = a.b + min(bbeg + r, a.nb - 1) * ldh + h * 8;
-#pragma unroll
-    for (int c = 0; c < NCHMAX; c++)
-      if (c < nch) anext[c] = *(const bf16x8 *)(bp + c * 16);
-  }
-  for (long bb = bbeg; bb < bend; bb += a.bstep) {
-    const long brow = min(bb + r, a.nb - 1);
-    const uint16_t *bp = a.b + brow * ldh + h * 8;
-    f32x16 acc[NB];
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-#pragma unroll
-      for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
-    if (QRES) {
-      bf16x8 av[QRES ? NCHMAX : 1];
-#pragma unroll
-      for (int c = 0; c < NCHMAX; c++) av[c] = anext[c];
-      const long nxt = bb + a.bstep;
-      if (nxt < bend) {
-        const uint16_t *np = a.b + min(nxt + r, a.nb - 1) * ldh + h * 8;
-#pragma unroll
-        for (int c = 0; c < NCHMAX; c++)
-          if (c < nch) anext[c] = *(const bf16x8 *)(np + c * 16);
-      }
-#pragma unroll
-      for (int c = 0; c < NCHMAX; c++)
-        if (c < nch) {
-#pragma unroll
-          for (int j = 0; j < NB; j++)
-            acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[c], qreg[j][QRES ? c : 0], acc[j],
-                                                             0, 0, 0);
-        }
-    } else {
-      for (int c = 0; c < nch; c++) {
-        bf16x8 av = *(const bf16x8 *)(bp + c * 16);
-#pragma unroll
-        for (int j = 0; j < NB; j++) {
-          bf16x8 bq = *(const bf16x8 *)(a.q + qrow[j] * ldh + c * 16 + h * 8);
-          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bq, acc[j], 0, 0, 0);
-        }
-      }
-    }
-    // epilogue: a min over the lane's 16 values decides whether any of them
-    // can enter (fminf drops NaN, which never enters); only then per value
+  // one 32-row block starting at row bb: a min over the lane's 16 values
+  // decides whether any can enter (fminf drops NaN, which never enters);
+  // only then per value
+  __device__ __forceinline__ void block(const Bf16Args &a, const f32x16 (&acc)[NB], long bb,
+                                        long bend, int h) {
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       float m = acc[j][0];
@@ -289,23 +263,6 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
       else
         maybe = m <= Tv[j];
       if (!maybe) continue;
-      if (MODE == 3) {
-        // one compare per value; the (rare) append behind a branch
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-          const float v = acc[j][i];
-          if (!(v <= Tv[j])) continue;
-          const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
-          if (bidx < bend && qok[j]) {
-            const uint64_t key = yk_key(v, (int)bidx);
-            if (key <= T[j]) {
-              int slot = atomicAdd(&a.cnt[qrow[j]], 1);
-              if (slot < YK_CAND_CAP) a.cand[qrow[j] * YK_CAND_CAP + slot] = key;
-            }
-          }
-        }
-        continue;
-      }
 #pragma unroll
       for (int i = 0; i < 16; i++) {
         const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
@@ -320,68 +277,173 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
           uint64_t key = (bok && v == v) ? yk_key(v, (int)bidx) : YK_KEY_NONE;
           if (key < top[j][R - 1]) yk_reg_insert16<R>(top[j], key);
         } else {
-          if (v <= Tv[j] && bok && qok[j]) {
-            uint64_t key = yk_key(v, (int)bidx);
+          if (!(v <= Tv[j])) continue;
+          if (bok && qok[j]) {
+            const uint64_t key = yk_key(v, (int)bidx);
             if (key <= T[j]) {
-              int slot = atomicAdd(&a.cnt[qrow[j]], 1);
-              if (slot < YK_CAND_CAP) a.cand[qrow[j] * YK_CAND_CAP + slot] = key;
+              const int n = ncs[j];
+              cs[(j * 64 + lane) * YK_CSTAGE + n] = key;
+              if (n + 1 == YK_CSTAGE) {
+                flush(a, j, YK_CSTAGE);
+                ncs[j] = 0;
+              } else {
+                ncs[j] = n + 1;
+              }
             }
           }
         }
       }
     }
   }
-  if (MODE == 1) {
+
+  // append n staged keys of tile j to the query's list (the count keeps
+  // growing past YK_CAND_CAP: the select kernel detects the overflow)
+  __device__ __forceinline__ void flush(const Bf16Args &a, int j, int n) {
+    const int base = atomicAdd(&a.cnt[qrow[j]], n);
+    const uint64_t *src = cs + (j * 64 + lane) * YK_CSTAGE;
+    for (int s = 0; s < n; s++)
+      if (base + s < YK_CAND_CAP) a.cand[qrow[j] * YK_CAND_CAP + base + s] = src[s];
+  }
+
+  // end of a work item: MODE 1 the lane minima, MODE 2 the lane top-R lists
+  // (or, with kp_direct, the pair-merged kp smallest keys), MODE 3 the keys
+  // still staged
+  __device__ __forceinline__ void finish(const Bf16Args &a, long qbase, int split, int r, int h) {
+    if (MODE == 3) {
 #pragma unroll
-    for (int j = 0; j < NB; j++) {
-      const long qi = qbase + j * 32 + r;
-      if (qok[j])
-        a.part[(qi * a.nsplit + split) * 2 + h] = bi[j] < 0 ? YK_KEY_NONE : yk_key(bv[j], bi[j]);
+      for (int j = 0; j < NB; j++)
+        if (ncs[j] > 0) flush(a, j, ncs[j]);
+    } else if (MODE == 1) {
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        const long qi = qbase + j * 32 + r;
+        if (qok[j])
+          a.part[(qi * a.nsplit + split) * 2 + h] = bi[j] < 0 ? YK_KEY_NONE : yk_key(bv[j], bi[j]);
+      }
+    } else if (MODE == 2 && a.kp_direct) {
+      // lanes r and r + 32 hold sorted top-R lists of disjoint rows of the
+      // same query: the first half-cleaner of the bitonic sequence (mine,
+      // partner's reversed) keeps the R smallest of the union, R/2..1
+      // cleaners sort them
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        uint64_t o[R], m[R];
+#pragma unroll
+        for (int s = 0; s < R; s++) {
+          const uint32_t lo = __shfl_xor((uint32_t)top[j][s], 32);
+          const uint32_t hi = __shfl_xor((uint32_t)(top[j][s] >> 32), 32);
+          o[s] = ((uint64_t)hi << 32) | lo;
+        }
+#pragma unroll
+        for (int s = 0; s < R; s++) m[s] = top[j][s] < o[R - 1 - s] ? top[j][s] : o[R - 1 - s];
+#pragma unroll
+        for (int w = R / 2; w >= 1; w >>= 1)
+#pragma unroll
+          for (int s = 0; s < R; s++)
+            if ((s & w) == 0) {
+              const uint64_t x = m[s], y = m[s + w];
+              m[s] = x < y ? x : y;
+              m[s + w] = x < y ? y : x;
+            }
+        const long qi = qbase + j * 32 + r;
+        if (h == 0 && qok[j]) {
+#pragma unroll
+          for (int s = 0; s < R; s++)
+            if (s < a.kp_direct) a.cand[qi * a.kp_direct + s] = m[s];
+        }
+      }
+    } else if (MODE == 2) {
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        if (!qok[j]) continue;
+        const long qi = qbase + j * 32 + r;
+        uint64_t *dst = a.part + ((qi * a.nsplit + split) * 2 + h) * R;
+#pragma unroll
+        for (int s = 0; s < R; s++) dst[s] = top[j][s];
+      }
     }
   }
-  if (MODE == 2 && a.kp_direct) {
-    // lanes r and r + 32 hold sorted top-R lists of disjoint rows of the same
-    // query: the first half-cleaner of the bitonic sequence (mine, partner's
-    // reversed) keeps the R smallest of the union, R/2..1 cleaners sort them
+};
+
+// Streaming scan, each wave on its own: NCH > 0 queries register-resident
+// (exactly NCH = d'/16 chunks, so every loop is unrolled without guards),
+// base fragments loaded straight to registers one block ahead; NCH = 0 both
+// operands streamed from memory (long vectors).
+template <int NB, int NCH, int MODE, int R>
+__global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
+  constexpr bool QRES = NCH > 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int r = lane & 31, h = lane >> 5;
+  // persistent grid, XCD-contiguous runs of work items (as in yk_select.hip)
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, wpx = gridDim.x >> 3;
+  const int ibeg = xcd * a.per_xcd, iend = min(a.total, ibeg + a.per_xcd);
+  const int nch = a.ldh >> 4;
+  const int ldh = a.ldh;
+  __shared__ uint64_t cstage[MODE == 3 ? 4 * NB * 64 * YK_CSTAGE : 1];
+  for (int item = ibeg + local; item < iend; item += wpx) {
+    const int tile = item % a.tiles_x, split = item / a.tiles_x;
+    const long qbase = ((long)tile * nwave + wave) * (32 * NB);
+    const long bbeg = (long)split * a.brange;
+    const long bend = min(a.nb, bbeg + a.brange);
+    BfSel<NB, MODE, R> sel;
+    sel.init(a, qbase, r, lane, cstage + wave * (MODE == 3 ? NB * 64 * YK_CSTAGE : 0));
+    if (MODE == 3 && !__any(sel.active())) continue;
+
+    bf16x8 qreg[NB][QRES ? NCH : 1];
+    if (QRES) {
 #pragma unroll
-    for (int j = 0; j < NB; j++) {
-      uint64_t o[R], m[R];
+      for (int j = 0; j < NB; j++)
 #pragma unroll
-      for (int s = 0; s < R; s++) {
-        const uint32_t lo = __shfl_xor((uint32_t)top[j][s], 32);
-        const uint32_t hi = __shfl_xor((uint32_t)(top[j][s] >> 32), 32);
-        o[s] = ((uint64_t)hi << 32) | lo;
-      }
+        for (int c = 0; c < NCH; c++)
+          qreg[j][c] = *(const bf16x8 *)(a.q + sel.qrow[j] * ldh + c * 16 + h * 8);
+    }
+    // fragment c of the block starting at row bb: bfrag(bb) + c * 512
+    auto bfrag = [&](long bb) { return a.b + ((bb >> 5) * nch * 64 + lane) * 8; };
+    bf16x8 anext[QRES ? NCH : 1];
+    if (QRES && bbeg < bend) {
+      const uint16_t *bp = bfrag(bbeg);
 #pragma unroll
-      for (int s = 0; s < R; s++) m[s] = top[j][s] < o[R - 1 - s] ? top[j][s] : o[R - 1 - s];
+      for (int c = 0; c < NCH; c++) anext[c] = *(const bf16x8 *)(bp + c * 512);
+    }
+    for (long bb = bbeg; bb < bend; bb += a.bstep) {
+      const uint16_t *bp = bfrag(bb);
+      f32x16 acc[NB];
 #pragma unroll
-      for (int w = R / 2; w >= 1; w >>= 1)
+      for (int j = 0; j < NB; j++)
 #pragma unroll
-        for (int s = 0; s < R; s++)
-          if ((s & w) == 0) {
-            const uint64_t x = m[s], y = m[s + w];
-            m[s] = x < y ? x : y;
-            m[s + w] = x < y ? y : x;
+        for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
+      if (QRES) {
+        bf16x8 av[QRES ? NCH : 1];
+#pragma unroll
+        for (int c = 0; c < NCH; c++) av[c] = anext[c];
+        const long nxt = bb + a.bstep;
+        if (nxt < bend) {
+          const uint16_t *np = bfrag(nxt);
+#pragma unroll
+          for (int c = 0; c < NCH; c++) anext[c] = *(const bf16x8 *)(np + c * 512);
+        }
+#pragma unroll
+        for (int c = 0; c < NCH; c++)
+#pragma unroll
+          for (int j = 0; j < NB; j++)
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[c], qreg[j][QRES ? c : 0], acc[j],
+                                                             0, 0, 0);
+      } else {
+        for (int c = 0; c < nch; c++) {
+          bf16x8 av = *(const bf16x8 *)(bp + c * 512);
+#pragma unroll
+          for (int j = 0; j < NB; j++) {
+            bf16x8 bq = *(const bf16x8 *)(a.q + sel.qrow[j] * ldh + c * 16 + h * 8);
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bq, acc[j], 0, 0, 0);
           }
-      const long qi = qbase + j * 32 + r;
-      if (h == 0 && qok[j]) {
-#pragma unroll
-        for (int s = 0; s < R; s++)
-          if (s < a.kp_direct) a.cand[qi * a.kp_direct + s] = m[s];
+        }
       }
+      sel.block(a, acc, bb, bend, h);
     }
-  } else if (MODE == 2) {
-#pragma unroll
-    for (int j = 0; j < NB; j++) {
-      if (!qok[j]) continue;
-      const long qi = qbase + j * 32 + r;
-      uint64_t *dst = a.part + ((qi * a.nsplit + split) * 2 + h) * R;
-#pragma unroll
-      for (int s = 0; s < R; s++) dst[s] = top[j][s];
-    }
-  }
+    sel.finish(a, qbase, split, r, h);
   }  // work items
 }
+
 
 // ---------------------------------------------------------------------------
 // per-query sorts (same contracts as in yk_select.hip)
@@ -613,28 +675,44 @@ static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, 
   return p;
 }
 
+// streaming kernel with exactly nch register-resident chunks (nch <= 16),
+// else the one streaming both operands
+template <int NB, int MODE, int R>
+static void (*yk_bscan_pick(int nch))(Bf16Args) {
+  switch (nch) {
+    case 1: return yk_scan_bf16<NB, 1, MODE, R>;
+    case 2: return yk_scan_bf16<NB, 2, MODE, R>;
+    case 3: return yk_scan_bf16<NB, 3, MODE, R>;
+    case 4: return yk_scan_bf16<NB, 4, MODE, R>;
+    case 5: return yk_scan_bf16<NB, 5, MODE, R>;
+    case 6: return yk_scan_bf16<NB, 6, MODE, R>;
+    case 7: return yk_scan_bf16<NB, 7, MODE, R>;
+    case 8: return yk_scan_bf16<NB, 8, MODE, R>;
+    case 9: return yk_scan_bf16<NB, 9, MODE, R>;
+    case 10: return yk_scan_bf16<NB, 10, MODE, R>;
+    case 11: return yk_scan_bf16<NB, 11, MODE, R>;
+    case 12: return yk_scan_bf16<NB, 12, MODE, R>;
+    case 13: return yk_scan_bf16<NB, 13, MODE, R>;
+    case 14: return yk_scan_bf16<NB, 14, MODE, R>;
+    case 15: return yk_scan_bf16<NB, 15, MODE, R>;
+    case 16: return yk_scan_bf16<NB, 16, MODE, R>;
+    default: return yk_scan_bf16<NB, 0, MODE, R>;
+  }
+}
+
 template <int MODE, int R>
 static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
-  // queries per wave: 128 for the filtered pass at d' <= 144 (halves the
-  // base-operand traffic per MFMA), else 64
-  const int nbq = ((MODE == 1 || MODE == 3) && a.ldh <= 144 && a.ldh > 64) ? 4 : 2;
+  // 64 queries per wave (two 32-column MFMA tiles): with the base fragments
+  // prefetched in registers one block ahead this fits two waves per SIMD,
+  // which measured faster than 128 queries per wave at one wave per SIMD
+  const int nbq = 2;
+  void (*kern)(Bf16Args) = yk_bscan_pick<2, MODE, R>(a.ldh >> 4);
   a.brange = p.brange;
   a.bstep = p.bstep;
   a.nsplit = p.nsplit;
   a.tiles_x = (int)((a.nq + 128L * nbq - 1) / (128L * nbq));
   a.total = a.tiles_x * p.nsplit;
   a.per_xcd = (a.total + 7) / 8;
-  void (*kern)(Bf16Args);
-  if (a.ldh <= 64)
-    kern = yk_scan_bf16<2, 4, MODE, R>;
-  else if (a.ldh <= 144 && nbq == 4)
-    kern = yk_scan_bf16<4, 9, MODE, R>;
-  else if (a.ldh <= 144)
-    kern = yk_scan_bf16<2, 9, MODE, R>;
-  else if (a.ldh <= 256)
-    kern = yk_scan_bf16<2, 16, MODE, R>;
-  else
-    kern = yk_scan_bf16<2, 0, MODE, R>;
   const int per_res = yk_resident_wgs((const void *)kern, 256) / 8;
   const dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), blk(256);
   int tok = yk_prof_begin(YK_PROF_BF16);

@@ -107,6 +107,23 @@ extern "C" int yk_memcpy_d2d(void *dst, const void *src, size_t bytes) {
   return YK_OK;
 }
 
+// plain 16-byte-per-lane copy: the reference point for HBM traffic counters
+__global__ void yk_copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+extern "C" int yk_copy_kernel(void *dst, const void *src, size_t bytes) {
+  YK_TRY(yk_init());
+  if (bytes % 16) return yk_set_error(YK_EINVAL, "yk_copy_kernel: size %zu not a multiple of 16", bytes);
+  if (!bytes) return YK_OK;
+  hipLaunchKernelGGL(yk_copy16_kernel, dim3(4096), dim3(256), 0, g_stream, (const uint4 *)src,
+                     (uint4 *)dst, bytes / 16);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
 __global__ void yk_fill_u32_kernel(uint32_t *p, uint32_t v, size_t n) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;

@@ -27,7 +27,8 @@
 //               sketch for k > 16
 //       MODE 2  register top-R of keys (R <= 16)          -> 2 <= k <= 16
 //       MODE 3  append every key <= the query's threshold T_q to a per-query
-//               candidate list (global atomics; rare)     -> k > 16
+//               candidate list (staged in LDS, one global
+//               atomic per YK_CSTAGE keys)                -> k > 16
 //   * a persistent grid (resident workgroups only) walks the work items;
 //     each XCD owns a contiguous run of items so workgroups that share a
 //     base range share that XCD's L2;
@@ -52,6 +53,7 @@
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 #define YK_CAND_CAP 4096  // candidate keys kept per query (MODE 3)
+#define YK_CSTAGE 8       // MODE 3: keys staged in LDS per (lane, query tile)
 #define YK_REG_R 16       // register top-R of MODE 2
 
 struct ScanArgs {
@@ -115,6 +117,12 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
   // range in its L2), and its workgroups stride through that run.
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, wpx = gridDim.x >> 3;
   const int ibeg = xcd * a0.per_xcd, iend = min(a0.total, ibeg + a0.per_xcd);
+  // MODE 3: candidate keys are staged in LDS, YK_CSTAGE per (query tile,
+  // lane), and appended to the query's global list a region at a time (one
+  // atomic per region instead of one per key; the count keeps growing past
+  // YK_CAND_CAP so the select kernel sees an overflow)
+  __shared__ uint64_t cstage[MODE == 3 ? 4 * NB * 64 * YK_CSTAGE : 1];
+  uint64_t *const cs = cstage + (MODE == 3 ? wave * NB * 64 * YK_CSTAGE : 0);
   for (int item = ibeg + local; item < iend; item += wpx) {
   ScanArgs a = a0;
   int tile, split;
@@ -157,6 +165,13 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
   // selection state
   float bv[NB];
   int bi[NB];
+  int ncs[NB];
+  auto flush = [&](int j, int n) {
+    const int base = atomicAdd(&a.cnt[qrow[j]], n);
+    const uint64_t *src = cs + (j * 64 + lane) * YK_CSTAGE;
+    for (int s_ = 0; s_ < n; s_++)
+      if (base + s_ < YK_CAND_CAP) a.cand[qrow[j] * YK_CAND_CAP + base + s_] = src[s_];
+  };
   uint64_t top[NB][MODE == 2 ? R : 1];
   uint64_t T[NB];
   float Tv[NB];
@@ -164,6 +179,7 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
   for (int j = 0; j < NB; j++) {
     bv[j] = 1e30f;
     bi[j] = -1;
+    ncs[j] = 0;
 #pragma unroll
     for (int s = 0; s < (MODE == 2 ? R : 1); s++) top[j][s] = YK_KEY_NONE;
     T[j] = MODE == 3 ? a.thr[qrow[j]] : 0;
@@ -313,8 +329,14 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
         } else if (bok && qok[j] && v == v) {
           uint64_t key = yk_key(v, (int)bidx);
           if (key <= T[j]) {
-            int slot = atomicAdd(&a.cnt[qrow[j]], 1);
-            if (slot < YK_CAND_CAP) a.cand[qrow[j] * YK_CAND_CAP + slot] = key;
+            const int n = ncs[j];
+            cs[(j * 64 + lane) * YK_CSTAGE + n] = key;
+            if (n + 1 == YK_CSTAGE) {
+              flush(j, YK_CSTAGE);
+              ncs[j] = 0;
+            } else {
+              ncs[j] = n + 1;
+            }
           }
         }
       }
@@ -357,6 +379,10 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
 #pragma unroll
       for (int s = 0; s < R; s++) dst[s] = top[j][s];
     }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+      if (ncs[j] > 0) flush(j, ncs[j]);
   }
   }  // work items
 }

@@ -80,8 +80,8 @@ class DeviceSet:
         """BF16 copy (+ error statistics) for the precision form."""
         key = (typ, bool(is_query))
         if key not in self.bf:
-            ldh = lib().yk_bf16_ld(self.d, typ)
-            self.bf[key] = (DeviceBuffer(self.n * ldh * 2), DeviceBuffer(self.n * 16), DeviceBuffer(16))
+            nbytes = lib().yk_bf16_bytes(self.n, self.d, typ, int(is_query))
+            self.bf[key] = (DeviceBuffer(nbytes), DeviceBuffer(self.n * 16), DeviceBuffer(16))
         h, st, mx = self.bf[key]
         check(lib().yk_to_bf16(self.vec.ptr, self.n, self.d, typ, int(is_query), h.ptr, st.ptr,
                                mx.ptr), "to_bf16")
