@@ -37,7 +37,9 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 (matrix)
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16
 PEAK_HBM_GBS = 8000.0
 
-PROF_FUSED, PROF_NORM, PROF_UPDATE, PROF_MERGE, PROF_RERANK, PROF_BATCH, PROF_BF16 = range(7)
+(PROF_FUSED, PROF_NORM, PROF_UPDATE, PROF_MERGE, PROF_RERANK, PROF_BATCH, PROF_BF16,
+ PROF_BF16_SKETCH, PROF_F32_SKETCH) = range(9)
+NSLOTS = 9
 
 
 def mixture(seed, n, d, ncomp, sigma, integer=True, centres_seed=None):
@@ -55,6 +57,21 @@ def mixture(seed, n, d, ncomp, sigma, integer=True, centres_seed=None):
             np.rint(x, out=x)
         out[i:i + m] = x
     return out
+
+
+def traffic_from_profile(name):
+    """HBM bytes per launch of the dominant kernel, as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in profiles/<name>
+    (a line 'hbm_bytes_per_launch: <fetch+write>'); None when absent."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
 
 
 def prof(lib, slot):
@@ -78,7 +95,7 @@ def timed(lib, fn, steps, warmup):
     lib.yk_sync()
     t1 = time.perf_counter()
     lib.yk_prof_enable(0)
-    slots = {s: prof(lib, s) for s in range(7)}
+    slots = {s: prof(lib, s) for s in range(NSLOTS)}
     return (t1 - t0) / steps, slots
 
 
@@ -175,7 +192,7 @@ def bench_kmeans_large(ya, dev, n=10_000_000, d=128, k=65536, niter=20, seed=1, 
     lib.yk_sync()
     sec = time.perf_counter() - t0
     lib.yk_prof_enable(0)
-    sl = {s_: prof(lib, s_) for s_ in range(7)}
+    sl = {s_: prof(lib, s_) for s_ in range(NSLOTS)}
     st = [ctypes.c_long() for _ in range(3)]
     lib.yael_kmeans_last_stats(*[ctypes.byref(x) for x in st])
     iters, assigned, fallback = [x.value for x in st]
@@ -413,6 +430,8 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="profile runs: first copy this many MiB three times with a plain "
                          "copy kernel (the reference point of the HBM traffic counters)")
+    ap.add_argument("--only", choices=["kmeans-large", "d960"], default=None,
+                    help="profile runs: run one extra workload alone and print its record")
     ap.add_argument("--nb", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--kprime", type=int, default=200)
@@ -437,6 +456,11 @@ def main():
         src.free()
         dst.free()
 
+    if args.only:
+        rec = bench_kmeans_large(ya, dev) if args.only == "kmeans-large" else bench_precision_d960(ya, dev)
+        print(json.dumps(rec, default=float), flush=True)
+        return
+
     d, k = 128, 100
     t = time.perf_counter()
     base = mixture(1, args.nb, d, 1024, 20, centres_seed=0)
@@ -447,9 +471,14 @@ def main():
     nq, nb = args.nq, args.nb
     flops = 2.0 * nq * nb * d
     qps16 = nq / r["sec16"]
+    # dominant kernel: the BF16 filtered pass (one launch per step); its
+    # algorithmic work is the whole distance matrix, 2*nq*nb*d flop.  The
+    # sketch pass (a 1-in-stride sample) is reported beside it.
     bf_ms, bf_n = r["sl16"][PROF_BF16]
-    bf_ms_step = bf_ms / args.steps
-    ach16 = flops / (bf_ms_step * 1e-3) / 1e12
+    sk_ms, sk_n = r["sl16"][PROF_BF16_SKETCH]
+    bf_ms_launch = bf_ms / max(bf_n, 1)
+    ach16 = flops / (bf_ms_launch * 1e-3) / 1e12
+    traffic = traffic_from_profile("knn_nb1e6_nq1e4_d128_k100_traffic.md")
     out = {
         "metric": "exact kNN queries/s at d=128 (BF16+FP32-rerank form, exact-NN match with FP32)",
         "value": qps16,
@@ -467,32 +496,38 @@ def main():
                                % (nb, nq, d, k, args.kprime),
                    "nb": nb, "nq": nq, "d": d, "k": k, "kprime": args.kprime},
         "roofline": {"bound": "mfma", "achieved": ach16, "peak": PEAK_BF16_TFLOPS,
-                     "unit": "TFLOP/s", "frac": ach16 / PEAK_BF16_TFLOPS, "traffic": None,
-                     "kernel": "yk_scan_bf16 (first pass: sketch + filtered scan)",
-                     "flop_per_step": flops, "kernel_ms_per_step": bf_ms_step,
-                     "launches_per_step": bf_n / args.steps},
+                     "unit": "TFLOP/s", "frac": ach16 / PEAK_BF16_TFLOPS,
+                     "traffic": traffic,
+                     "kernel": "yk_scan_bf16<2,9,3,1> (BF16 filtered pass, MFMA 32x32x16)",
+                     "flop_per_launch": flops, "kernel_ms_per_launch": bf_ms_launch,
+                     "launches_per_step": bf_n / args.steps,
+                     "sketch_kernel_ms_per_launch": sk_ms / max(sk_n, 1)},
         "accuracy": {"max_rel_err_vs_fp64": r["max_rel_err_f64"], "sorted": r["sorted_ok"],
                      "fallback_queries": r["nfb"]},
     }
-    print("# bf16 scan kernels: %.3f ms/step (%.1f launches) -> %.1f TFLOP/s"
-          % (bf_ms_step, bf_n / args.steps, ach16), file=sys.stderr, flush=True)
+    print("# bf16 filtered pass: %.3f ms/launch -> %.1f TFLOP/s; sketch %.3f ms/launch"
+          % (bf_ms_launch, ach16, sk_ms / max(sk_n, 1)), file=sys.stderr, flush=True)
     if not args.only_bf16:
         qps32 = nq / r["sec32"]
         f32_ms, f32_n = r["sl32"][PROF_FUSED]
-        f32_ms_step = f32_ms / args.steps
-        ach32 = flops / (f32_ms_step * 1e-3) / 1e12
+        f32_ms_launch = f32_ms / max(f32_n, 1)
+        ach32 = flops / (f32_ms_launch * 1e-3) / 1e12
+        f32_sk_ms, f32_sk_n = r["sl32"][PROF_F32_SKETCH]
         out["forms"] = {
             "fp32": {"queries_per_s": qps32, "ms_per_step": r["sec32"] * 1e3,
                      "roofline": {"bound": "mfma", "achieved": ach32, "peak": PEAK_FP32_TFLOPS,
                                   "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32_TFLOPS,
-                                  "kernel": "yk_scan_f32", "kernel_ms_per_step": f32_ms_step,
-                                  "launches_per_step": f32_n / args.steps}},
+                                  "kernel": "yk_scan_f32<1,true,3,1> (FP32 filtered pass)",
+                                  "kernel_ms_per_launch": f32_ms_launch,
+                                  "launches_per_step": f32_n / args.steps,
+                                  "sketch_kernel_ms_per_launch":
+                                      f32_sk_ms / max(f32_sk_n, 1)}},
             "bf16_rerank_over_fp32": qps16 / qps32,
             "index_agreement_with_fp32": float(np.mean(np.all(r["i16"] == r["i32"], axis=1))),
             "distances_bit_identical_to_fp32": bool(
                 np.array_equal(r["d16"].view(np.int32), r["d32"].view(np.int32))),
         }
-        print("# fp32 scan kernels: %.3f ms/step -> %.1f TFLOP/s" % (f32_ms_step, ach32),
+        print("# fp32 filtered pass: %.3f ms/launch -> %.1f TFLOP/s" % (f32_ms_launch, ach32),
               file=sys.stderr, flush=True)
 
     if not args.no_extras:

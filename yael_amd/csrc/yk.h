@@ -151,8 +151,11 @@ int yk_batch_destroy(yk_batch *bt);
 /* Timing of the library's stream (HIP events recorded on it). */
 int yk_timer_start(void);
 int yk_timer_stop(float *ms);
-/* Per-kernel timing on the library stream: when enabled, every launch of
- * the fused distance kernel is bracketed by events; totals are returned. */
+/* Per-kernel timing on the library stream: when enabled, launches are
+ * bracketed by events and summed per slot: 0 FP32 scan (filtered pass, k=1
+ * and register top-k), 1 norms, 2 k-means update, 3 merges/selects,
+ * 4 re-rank, 5 batched iteration, 6 BF16 scan (filtered pass, top-k),
+ * 7 BF16 sketch pass, 8 FP32 sketch pass. */
 int yk_prof_enable(int on);
 int yk_prof_get(int which, double *total_ms, long *count);
 int yk_prof_reset(void);

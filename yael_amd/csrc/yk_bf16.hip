@@ -247,48 +247,51 @@ struct BfSel {
 
   // one 32-row block starting at row bb: a min over the lane's 16 values
   // decides whether any can enter (fminf drops NaN, which never enters);
-  // only then per value
+  // only then, per value, a float compare against the entry bound before
+  // any key is built.  Lane row i is bb + 8(i/4) + (i%4) + 4h; rows at or
+  // past bend are masked with one integer compare against rl = bend - bb - 4h.
   __device__ __forceinline__ void block(const Bf16Args &a, const f32x16 (&acc)[NB], long bb,
                                         long bend, int h) {
+    const int rl = (int)min(bend - bb, 64L) - 4 * h;
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       float m = acc[j][0];
 #pragma unroll
       for (int i = 1; i < 16; i++) m = fminf(m, acc[j][i]);
-      bool maybe;
+      float lim;
       if (MODE == 1)
-        maybe = m < bv[j];
+        lim = bv[j];
       else if (MODE == 2)
-        maybe = top[j][R - 1] == YK_KEY_NONE || m <= yk_fkey_inv((uint32_t)(top[j][R - 1] >> 32));
+        lim = top[j][R - 1] == YK_KEY_NONE ? INFINITY
+                                            : yk_fkey_inv((uint32_t)(top[j][R - 1] >> 32));
       else
-        maybe = m <= Tv[j];
+        lim = Tv[j];
+      const bool maybe = MODE == 1 ? m < lim : m <= lim;
       if (!maybe) continue;
 #pragma unroll
       for (int i = 0; i < 16; i++) {
-        const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
-        const bool bok = bidx < bend;
+        const int ro = 8 * (i >> 2) + (i & 3);
         const float v = acc[j][i];
+        if (!(MODE == 1 ? v < lim : v <= lim) || ro >= rl) continue;
+        const int bidx = (int)bb + ro + 4 * h;
         if (MODE == 1) {
-          if (bok && v < bv[j]) {
+          if (v < bv[j]) {  // bv can have moved within this block
             bv[j] = v;
-            bi[j] = (int)bidx;
+            bi[j] = bidx;
           }
         } else if (MODE == 2) {
-          uint64_t key = (bok && v == v) ? yk_key(v, (int)bidx) : YK_KEY_NONE;
+          const uint64_t key = yk_key(v, bidx);
           if (key < top[j][R - 1]) yk_reg_insert16<R>(top[j], key);
-        } else {
-          if (!(v <= Tv[j])) continue;
-          if (bok && qok[j]) {
-            const uint64_t key = yk_key(v, (int)bidx);
-            if (key <= T[j]) {
-              const int n = ncs[j];
-              cs[(j * 64 + lane) * YK_CSTAGE + n] = key;
-              if (n + 1 == YK_CSTAGE) {
-                flush(a, j, YK_CSTAGE);
-                ncs[j] = 0;
-              } else {
-                ncs[j] = n + 1;
-              }
+        } else if (qok[j]) {
+          const uint64_t key = yk_key(v, bidx);
+          if (key <= T[j]) {
+            const int n = ncs[j];
+            cs[(j * 64 + lane) * YK_CSTAGE + n] = key;
+            if (n + 1 == YK_CSTAGE) {
+              flush(a, j, YK_CSTAGE);
+              ncs[j] = 0;
+            } else {
+              ncs[j] = n + 1;
             }
           }
         }
@@ -399,36 +402,47 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
     }
     // fragment c of the block starting at row bb: bfrag(bb) + c * 512
     auto bfrag = [&](long bb) { return a.b + ((bb >> 5) * nch * 64 + lane) * 8; };
-    bf16x8 anext[QRES ? NCH : 1];
-    if (QRES && bbeg < bend) {
-      const uint16_t *bp = bfrag(bbeg);
+    if (QRES) {
+      // two register buffers used in turn (the loop is unrolled by two so
+      // neither is copied): block bb is multiplied from one while the next
+      // block's fragments load into the other
+      bf16x8 bufa[QRES ? NCH : 1], bufb[QRES ? NCH : 1];
+      auto load = [&](bf16x8(&buf)[QRES ? NCH : 1], long bb) {
+        const uint16_t *bp = bfrag(bb);
 #pragma unroll
-      for (int c = 0; c < NCH; c++) anext[c] = *(const bf16x8 *)(bp + c * 512);
-    }
-    for (long bb = bbeg; bb < bend; bb += a.bstep) {
-      const uint16_t *bp = bfrag(bb);
-      f32x16 acc[NB];
+        for (int c = 0; c < NCH; c++) buf[c] = *(const bf16x8 *)(bp + c * 512);
+      };
+      auto step = [&](const bf16x8(&buf)[QRES ? NCH : 1], long bb) {
+        f32x16 acc[NB];
 #pragma unroll
-      for (int j = 0; j < NB; j++)
+        for (int j = 0; j < NB; j++)
 #pragma unroll
-        for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
-      if (QRES) {
-        bf16x8 av[QRES ? NCH : 1];
-#pragma unroll
-        for (int c = 0; c < NCH; c++) av[c] = anext[c];
-        const long nxt = bb + a.bstep;
-        if (nxt < bend) {
-          const uint16_t *np = bfrag(nxt);
-#pragma unroll
-          for (int c = 0; c < NCH; c++) anext[c] = *(const bf16x8 *)(np + c * 512);
-        }
+          for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
 #pragma unroll
         for (int c = 0; c < NCH; c++)
 #pragma unroll
           for (int j = 0; j < NB; j++)
-            acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[c], qreg[j][QRES ? c : 0], acc[j],
+            acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(buf[c], qreg[j][QRES ? c : 0], acc[j],
                                                              0, 0, 0);
-      } else {
+        sel.block(a, acc, bb, bend, h);
+      };
+      if (bbeg < bend) load(bufa, bbeg);
+      for (long bb = bbeg; bb < bend; bb += 2 * a.bstep) {
+        const long b1 = bb + a.bstep, b2 = b1 + a.bstep;
+        if (b1 < bend) load(bufb, b1);
+        step(bufa, bb);
+        if (b1 >= bend) break;
+        if (b2 < bend) load(bufa, b2);
+        step(bufb, b1);
+      }
+    } else {
+      for (long bb = bbeg; bb < bend; bb += a.bstep) {
+        const uint16_t *bp = bfrag(bb);
+        f32x16 acc[NB];
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+#pragma unroll
+          for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
         for (int c = 0; c < nch; c++) {
           bf16x8 av = *(const bf16x8 *)(bp + c * 512);
 #pragma unroll
@@ -437,8 +451,8 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
             acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bq, acc[j], 0, 0, 0);
           }
         }
+        sel.block(a, acc, bb, bend, h);
       }
-      sel.block(a, acc, bb, bend, h);
     }
     sel.finish(a, qbase, split, r, h);
   }  // work items
@@ -715,7 +729,7 @@ static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
   a.per_xcd = (a.total + 7) / 8;
   const int per_res = yk_resident_wgs((const void *)kern, 256) / 8;
   const dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), blk(256);
-  int tok = yk_prof_begin(YK_PROF_BF16);
+  int tok = yk_prof_begin(MODE == 1 ? YK_PROF_BF16_SKETCH : YK_PROF_BF16);
   hipLaunchKernelGGL(kern, grid, blk, 0, yk_stream(), a);
   yk_prof_end(tok);
   YK_HIP(hipGetLastError());

@@ -19,7 +19,8 @@ void yk_prof_end(int tok);
 
 // profiling slots
 enum { YK_PROF_FUSED = 0, YK_PROF_NORM = 1, YK_PROF_UPDATE = 2, YK_PROF_MERGE = 3,
-       YK_PROF_RERANK = 4, YK_PROF_BATCH = 5, YK_PROF_BF16 = 6 };
+       YK_PROF_RERANK = 4, YK_PROF_BATCH = 5, YK_PROF_BF16 = 6, YK_PROF_BF16_SKETCH = 7,
+       YK_PROF_F32_SKETCH = 8 };
 
 #define YK_TRY(x)                 \
   do {                            \

@@ -220,7 +220,7 @@ extern "C" int yk_device_info(char *buf, int len) {
 // per-kernel event timing (used by the benchmark to time the dominant kernel
 // on the stream it runs on)
 
-#define YK_PROF_SLOTS 8
+#define YK_PROF_SLOTS 16
 #define YK_PROF_EV 4096
 static int g_prof_on = 0;
 static hipEvent_t g_pev[YK_PROF_EV][2];

@@ -563,7 +563,7 @@ static int yk_launch_scan(const YkPlan &p, ScanArgs a, bool qres) {
   auto kern = qres ? yk_scan_f32<1, true, MODE, R> : yk_scan_f32<1, false, MODE, R>;
   const int per_xcd_res = yk_resident_wgs((const void *)kern, 64 * p.waves) / 8;
   dim3 grid(8 * (a.per_xcd < per_xcd_res ? a.per_xcd : per_xcd_res)), block(64 * p.waves);
-  int tok = yk_prof_begin(YK_PROF_FUSED);
+  int tok = yk_prof_begin(MODE == 1 ? YK_PROF_F32_SKETCH : YK_PROF_FUSED);
   hipLaunchKernelGGL(kern, grid, block, 0, yk_stream(), a);
   yk_prof_end(tok);
   YK_HIP(hipGetLastError());
