@@ -207,7 +207,6 @@ struct BfSel {
   bool qok[NB];
   uint64_t top[NB][MODE == 2 ? R : 1];
   float bv[NB];
-  int bi[NB];
   uint64_t T[NB];
   float Tv[NB];
   // MODE 3: candidate keys are staged in LDS, YK_CSTAGE per (query tile,
@@ -228,7 +227,6 @@ struct BfSel {
       qok[j] = qi < a.nq;
       qrow[j] = qok[j] ? qi : a.nq - 1;
       bv[j] = INFINITY;
-      bi[j] = -1;
 #pragma unroll
       for (int s = 0; s < (MODE == 2 ? R : 1); s++) top[j][s] = YK_KEY_NONE;
       T[j] = MODE == 3 ? a.thr[qrow[j]] : 0;
@@ -248,38 +246,52 @@ struct BfSel {
   // one 32-row block starting at row bb: a min over the lane's 16 values
   // decides whether any can enter (fminf drops NaN, which never enters);
   // only then, per value, a float compare against the entry bound before
-  // any key is built.  Lane row i is bb + 8(i/4) + (i%4) + 4h; rows at or
-  // past bend are masked with one integer compare against rl = bend - bb - 4h.
+  // any key is built.  Lane row i is bb + 8(i/4) + (i%4) + 4h; only the last
+  // block of the base can hold rows past bend (the zero pad rows of the
+  // fragment-major copy), masked with one integer compare against
+  // rl = bend - bb - 4h.
   __device__ __forceinline__ void block(const Bf16Args &a, const f32x16 (&acc)[NB], long bb,
                                         long bend, int h) {
+    // a uniform branch: the per-row mask exists only in the (rare) partial
+    // block, so the common path carries no per-value row compares (MODE 2
+    // keeps one masked copy: its insertion network is large enough that a
+    // second copy costs the register budget of two waves per SIMD)
     const int rl = (int)min(bend - bb, 64L) - 4 * h;
+    if (MODE != 2 && bb + 32 <= bend)
+      block_impl<true>(a, acc, bb, rl, h);
+    else
+      block_impl<false>(a, acc, bb, rl, h);
+  }
+
+  template <bool FULL>
+  __device__ __forceinline__ void block_impl(const Bf16Args &a, const f32x16 (&acc)[NB], long bb,
+                                             int rl, int h) {
 #pragma unroll
     for (int j = 0; j < NB; j++) {
+      if (MODE == 1) {
+        // sketch: the lane's minimum value only (see finish)
+        float m = bv[j];
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+          if (FULL || 8 * (i >> 2) + (i & 3) < rl) m = fminf(m, acc[j][i]);
+        bv[j] = m;
+        continue;
+      }
       float m = acc[j][0];
 #pragma unroll
       for (int i = 1; i < 16; i++) m = fminf(m, acc[j][i]);
-      float lim;
-      if (MODE == 1)
-        lim = bv[j];
-      else if (MODE == 2)
-        lim = top[j][R - 1] == YK_KEY_NONE ? INFINITY
-                                            : yk_fkey_inv((uint32_t)(top[j][R - 1] >> 32));
-      else
-        lim = Tv[j];
-      const bool maybe = MODE == 1 ? m < lim : m <= lim;
-      if (!maybe) continue;
+      const float lim = MODE == 2 ? (top[j][R - 1] == YK_KEY_NONE
+                                         ? INFINITY
+                                         : yk_fkey_inv((uint32_t)(top[j][R - 1] >> 32)))
+                                  : Tv[j];
+      if (!(m <= lim)) continue;
 #pragma unroll
       for (int i = 0; i < 16; i++) {
         const int ro = 8 * (i >> 2) + (i & 3);
         const float v = acc[j][i];
-        if (!(MODE == 1 ? v < lim : v <= lim) || ro >= rl) continue;
+        if (!(v <= lim) || (!FULL && ro >= rl)) continue;
         const int bidx = (int)bb + ro + 4 * h;
-        if (MODE == 1) {
-          if (v < bv[j]) {  // bv can have moved within this block
-            bv[j] = v;
-            bi[j] = bidx;
-          }
-        } else if (MODE == 2) {
+        if (MODE == 2) {
           const uint64_t key = yk_key(v, bidx);
           if (key < top[j][R - 1]) yk_reg_insert16<R>(top[j], key);
         } else if (qok[j]) {
@@ -317,11 +329,17 @@ struct BfSel {
       for (int j = 0; j < NB; j++)
         if (ncs[j] > 0) flush(a, j, ncs[j]);
     } else if (MODE == 1) {
+      // Each lane's minimum over its rows of the item: the item's two lane
+      // halves and the splits cover disjoint row sets, so the k-th smallest
+      // of these values is attained by k distinct rows; the key (value,
+      // 0xffffffff) therefore bounds the k-th smallest key from above
+      // without the row index being known.
 #pragma unroll
       for (int j = 0; j < NB; j++) {
         const long qi = qbase + j * 32 + r;
         if (qok[j])
-          a.part[(qi * a.nsplit + split) * 2 + h] = bi[j] < 0 ? YK_KEY_NONE : yk_key(bv[j], bi[j]);
+          a.part[(qi * a.nsplit + split) * 2 + h] =
+              bv[j] < INFINITY ? (((uint64_t)yk_fkey(bv[j]) << 32) | 0xffffffffu) : YK_KEY_NONE;
       }
     } else if (MODE == 2 && a.kp_direct) {
       // lanes r and r + 32 hold sorted top-R lists of disjoint rows of the
@@ -426,13 +444,17 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
                                                              0, 0, 0);
         sel.block(a, acc, bb, bend, h);
       };
+      // The prefetches are unconditional (past the range they reload the
+      // current block): a conditional load would join two paths with
+      // different outstanding-load counts, and the wait before the next
+      // MFMAs would then also wait for the loads just issued.
       if (bbeg < bend) load(bufa, bbeg);
       for (long bb = bbeg; bb < bend; bb += 2 * a.bstep) {
         const long b1 = bb + a.bstep, b2 = b1 + a.bstep;
-        if (b1 < bend) load(bufb, b1);
+        load(bufb, b1 < bend ? b1 : bb);
         step(bufa, bb);
         if (b1 >= bend) break;
-        if (b2 < bend) load(bufa, b2);
+        load(bufa, b2 < bend ? b2 : b1);
         step(bufb, b1);
       }
     } else {

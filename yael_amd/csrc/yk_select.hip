@@ -107,8 +107,12 @@ __device__ __forceinline__ void yk_reg_insert(uint64_t (&t)[R], uint64_t x) {
   t[0] = x < t[0] ? x : t[0];
 }
 
-template <int NB, bool QRES, int MODE, int R>
+// NCH > 0: queries register-resident, exactly NCH 16-wide chunks (ld = 16
+// NCH, so every chunk loop is unrolled without guards); NCH = 0: both
+// operands streamed from memory.
+template <int NB, int NCH, int MODE, int R>
 __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
+  constexpr bool QRES = NCH > 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int r = lane & 31, h = lane >> 5;
   // Persistent grid: the hardware deals block ids to the 8 XCDs in turn;
@@ -193,30 +197,28 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
     if (!__any(act)) continue;  // every query of this wave is already final
   }
 
-  float4 qreg[NB][QRES ? 16 : 1];
+  float4 qreg[NB][QRES ? 2 * NCH : 1];
   if (QRES) {
 #pragma unroll
     for (int j = 0; j < NB; j++)
 #pragma unroll
-      for (int c = 0; c < 8; c++)
-        if (c < nch) {
-          const float *p = a.q + qrow[j] * ld + c * 16 + h * 8;
-          qreg[j][2 * c] = *(const float4 *)p;
-          qreg[j][2 * c + 1] = *(const float4 *)(p + 4);
-        }
+      for (int c = 0; c < NCH; c++) {
+        const float *p = a.q + qrow[j] * ld + c * 16 + h * 8;
+        qreg[j][2 * c] = *(const float4 *)p;
+        qreg[j][2 * c + 1] = *(const float4 *)(p + 4);
+      }
   }
 
   // the next block's base operand is loaded while the current one is
   // multiplied (register double buffer, QRES only)
-  float4 anext[QRES ? 16 : 1];
+  float4 anext[QRES ? 2 * NCH : 1];
   if (QRES && bbeg < bend) {
     const float *np = a.b + min(bbeg + r, a.nb - 1) * ld + h * 8;
 #pragma unroll
-    for (int c = 0; c < 8; c++)
-      if (c < nch) {
-        anext[2 * c] = *(const float4 *)(np + c * 16);
-        anext[2 * c + 1] = *(const float4 *)(np + c * 16 + 4);
-      }
+    for (int c = 0; c < NCH; c++) {
+      anext[2 * c] = *(const float4 *)(np + c * 16);
+      anext[2 * c + 1] = *(const float4 *)(np + c * 16 + 4);
+    }
   }
   for (long bb = bbeg; bb < bend; bb += a.bstep) {
     const long brow = min(bb + r, a.nb - 1);
@@ -228,21 +230,25 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
       for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
 
     if (QRES) {
-      float4 av[16];
+      float4 av[QRES ? 2 * NCH : 1];
 #pragma unroll
-      for (int c = 0; c < 16; c++) av[c] = anext[c];
-      if (bb + a.bstep < bend) {
-        const float *np = a.b + min(bb + a.bstep + r, a.nb - 1) * ld + h * 8;
+      for (int c = 0; c < 2 * NCH; c++) av[c] = anext[c];
+      {
+        // unconditional (past the range it reloads the current block): a
+        // conditional prefetch joins paths with different outstanding-load
+        // counts, and the wait before the MFMAs would then also wait for the
+        // loads just issued
+        const long nxt = bb + a.bstep < bend ? bb + a.bstep : bb;
+        const float *np = a.b + min(nxt + r, a.nb - 1) * ld + h * 8;
 #pragma unroll
-        for (int c = 0; c < 8; c++)
-          if (c < nch) {
-            anext[2 * c] = *(const float4 *)(np + c * 16);
-            anext[2 * c + 1] = *(const float4 *)(np + c * 16 + 4);
-          }
+        for (int c = 0; c < NCH; c++) {
+          anext[2 * c] = *(const float4 *)(np + c * 16);
+          anext[2 * c + 1] = *(const float4 *)(np + c * 16 + 4);
+        }
       }
 #pragma unroll
-      for (int c = 0; c < 8; c++)
-        if (c < nch) {
+      for (int c = 0; c < NCH; c++)
+        {
 #pragma unroll
           for (int j = 0; j < NB; j++) {
             float4 a0 = av[2 * c], a1 = av[2 * c + 1];
@@ -552,6 +558,23 @@ static int yk_scratch(int slot, size_t bytes, void **out) {
   return YK_OK;
 }
 
+// the kernel with exactly ld/16 register-resident chunks (ld <= 128), else
+// the one streaming both operands
+template <int MODE, int R>
+static void (*yk_f32_pick(int ld))(ScanArgs) {
+  switch (ld >> 4) {
+    case 1: return yk_scan_f32<1, 1, MODE, R>;
+    case 2: return yk_scan_f32<1, 2, MODE, R>;
+    case 3: return yk_scan_f32<1, 3, MODE, R>;
+    case 4: return yk_scan_f32<1, 4, MODE, R>;
+    case 5: return yk_scan_f32<1, 5, MODE, R>;
+    case 6: return yk_scan_f32<1, 6, MODE, R>;
+    case 7: return yk_scan_f32<1, 7, MODE, R>;
+    case 8: return yk_scan_f32<1, 8, MODE, R>;
+    default: return yk_scan_f32<1, 0, MODE, R>;
+  }
+}
+
 template <int MODE, int R>
 static int yk_launch_scan(const YkPlan &p, ScanArgs a, bool qres) {
   a.brange = p.brange;
@@ -560,7 +583,8 @@ static int yk_launch_scan(const YkPlan &p, ScanArgs a, bool qres) {
   a.tiles_x = p.tiles;
   a.total = p.tiles * p.nsplit;
   a.per_xcd = (a.total + 7) / 8;
-  auto kern = qres ? yk_scan_f32<1, true, MODE, R> : yk_scan_f32<1, false, MODE, R>;
+  (void)qres;
+  auto kern = yk_f32_pick<MODE, R>(a.ld);
   const int per_xcd_res = yk_resident_wgs((const void *)kern, 64 * p.waves) / 8;
   dim3 grid(8 * (a.per_xcd < per_xcd_res ? a.per_xcd : per_xcd_res)), block(64 * p.waves);
   int tok = yk_prof_begin(MODE == 1 ? YK_PROF_F32_SKETCH : YK_PROF_FUSED);
@@ -732,7 +756,7 @@ extern "C" int yk_batch_assign(long nitems, const int *item_inst, const int *ite
   a.done = done;
   a.total = (int)nitems;
   a.per_xcd = (a.total + 7) / 8;
-  auto kern = a.ld <= 128 ? yk_scan_f32<1, true, 0, 1> : yk_scan_f32<1, false, 0, 1>;
+  auto kern = yk_f32_pick<0, 1>(a.ld);
   const int per_res = yk_resident_wgs((const void *)kern, 256) / 8;
   dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), block(256);
   hipLaunchKernelGGL(kern, grid, block, 0, yk_stream(), a);
