@@ -43,6 +43,9 @@
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+// word 3 of a raw (untyped, stride 0) buffer descriptor on gfx9-family GPUs
+#define YK_BUF_DWORD3 0x00020000
 
 #define YK_CAND_CAP 4096
 #define YK_REG_R 16
@@ -215,11 +218,13 @@ struct BfSel {
   uint64_t *cs;
   int ncs[NB];
   int lane;
+  float *vs;  // MODES 2, 3: this lane's 16-value LDS slot
 
   __device__ __forceinline__ void init(const Bf16Args &a, long qbase, int r, int lane_,
-                                       uint64_t *cs_wave) {
+                                       uint64_t *cs_wave, float *vs_lane) {
     lane = lane_;
     cs = cs_wave;
+    vs = vs_lane;
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       ncs[j] = 0;
@@ -285,29 +290,40 @@ struct BfSel {
                                          : yk_fkey_inv((uint32_t)(top[j][R - 1] >> 32)))
                                   : Tv[j];
       if (!(m <= lim)) continue;
+      // A lane that may take a value: a 16-bit mask of its passing values
+      // (branch-free), its 16 values parked in its LDS slot, then one
+      // iteration per set bit -- a handful of iterations for the wave
+      // instead of a branch per value.
+      unsigned msk = 0;
 #pragma unroll
       for (int i = 0; i < 16; i++) {
-        const int ro = 8 * (i >> 2) + (i & 3);
-        const float v = acc[j][i];
-        if (!(v <= lim) || (!FULL && ro >= rl)) continue;
-        const int bidx = (int)bb + ro + 4 * h;
+        const bool p = acc[j][i] <= lim && (FULL || 8 * (i >> 2) + (i & 3) < rl);
+        msk |= p ? (1u << i) : 0u;
+      }
+      if (MODE == 3 && !qok[j]) msk = 0;
+      if (msk == 0) continue;
+      float4 *slot = (float4 *)vs;
+#pragma unroll
+      for (int q4 = 0; q4 < 4; q4++)
+        slot[q4] = make_float4(acc[j][4 * q4], acc[j][4 * q4 + 1], acc[j][4 * q4 + 2],
+                               acc[j][4 * q4 + 3]);
+      do {
+        const int i = __builtin_ctz(msk);
+        msk &= msk - 1;
+        const uint64_t key = yk_key(vs[i], (int)bb + 8 * (i >> 2) + (i & 3) + 4 * h);
         if (MODE == 2) {
-          const uint64_t key = yk_key(v, bidx);
           if (key < top[j][R - 1]) yk_reg_insert16<R>(top[j], key);
-        } else if (qok[j]) {
-          const uint64_t key = yk_key(v, bidx);
-          if (key <= T[j]) {
-            const int n = ncs[j];
-            cs[(j * 64 + lane) * YK_CSTAGE + n] = key;
-            if (n + 1 == YK_CSTAGE) {
-              flush(a, j, YK_CSTAGE);
-              ncs[j] = 0;
-            } else {
-              ncs[j] = n + 1;
-            }
+        } else if (key <= T[j]) {
+          const int n = ncs[j];
+          cs[(j * 64 + lane) * YK_CSTAGE + n] = key;
+          if (n + 1 == YK_CSTAGE) {
+            flush(a, j, YK_CSTAGE);
+            ncs[j] = 0;
+          } else {
+            ncs[j] = n + 1;
           }
         }
-      }
+      } while (msk);
     }
   }
 
@@ -401,13 +417,15 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
   const int nch = a.ldh >> 4;
   const int ldh = a.ldh;
   __shared__ uint64_t cstage[MODE == 3 ? 4 * NB * 64 * YK_CSTAGE : 1];
+  __shared__ float vstage[MODE >= 2 ? 4 * 64 * 16 : 1];
   for (int item = ibeg + local; item < iend; item += wpx) {
     const int tile = item % a.tiles_x, split = item / a.tiles_x;
     const long qbase = ((long)tile * nwave + wave) * (32 * NB);
     const long bbeg = (long)split * a.brange;
     const long bend = min(a.nb, bbeg + a.brange);
     BfSel<NB, MODE, R> sel;
-    sel.init(a, qbase, r, lane, cstage + wave * (MODE == 3 ? NB * 64 * YK_CSTAGE : 0));
+    sel.init(a, qbase, r, lane, cstage + wave * (MODE == 3 ? NB * 64 * YK_CSTAGE : 0),
+             vstage + (MODE >= 2 ? (wave * 64 + lane) * 16 : 0));
     if (MODE == 3 && !__any(sel.active())) continue;
 
     bf16x8 qreg[NB][QRES ? NCH : 1];
@@ -425,10 +443,25 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
       // neither is copied): block bb is multiplied from one while the next
       // block's fragments load into the other
       bf16x8 bufa[QRES ? NCH : 1], bufb[QRES ? NCH : 1];
+      // Buffer loads: the item's fragments through a descriptor based at its
+      // first block, the block/chunk offset in a scalar register and the
+      // lane's 16 bytes in one constant vector register -- no per-block
+      // 64-bit vector address arithmetic, and no address register for the
+      // allocator to recycle as a load destination (which forces a wait for
+      // the outstanding loads before the next address can be formed).
+      const long nbk = (a.nb + 31) >> 5;  // blocks of the padded base copy
+      const long rbytes = (nbk - (bbeg >> 5)) * (long)NCH * 1024;
+      const uint16_t *ibase = a.b + (bbeg >> 5) * (long)NCH * 512;
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)ibase, 0, (int)min(rbytes, 0x7fffffffL), YK_BUF_DWORD3);
+      const int voff = lane * 16;
       auto load = [&](bf16x8(&buf)[QRES ? NCH : 1], long bb) {
-        const uint16_t *bp = bfrag(bb);
+        const int soff = (int)(((bb - bbeg) >> 5) * NCH * 1024);
 #pragma unroll
-        for (int c = 0; c < NCH; c++) buf[c] = *(const bf16x8 *)(bp + c * 512);
+        for (int c = 0; c < NCH; c++) {
+          const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff + c * 1024, 0);
+          buf[c] = __builtin_bit_cast(bf16x8, v);
+        }
       };
       auto step = [&](const bf16x8(&buf)[QRES ? NCH : 1], long bb) {
         f32x16 acc[NB];
@@ -706,6 +739,10 @@ static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, 
   if (want > nblocks) want = nblocks;
   if (want < 1) want = 1;
   long per = (nblocks + want - 1) / want;
+  // an item's fragments stay within 2^30 bytes (buffer-load offsets are
+  // 32-bit; 16 KB per 32-row block at d' = 256)
+  const long max_per = (1L << 30) / (16384L * sample);
+  if (per > max_per) per = max_per;
   p.brange = per * p.bstep;
   p.nsplit = (int)((nb + p.brange - 1) / p.brange);
   return p;

@@ -51,6 +51,8 @@
 #include "yk_dev.h"
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+// word 3 of a raw (untyped, stride 0) buffer descriptor on gfx9-family GPUs
+#define YK_BUF_DWORD3 0x00020000
 
 #define YK_CAND_CAP 4096  // candidate keys kept per query (MODE 3)
 #define YK_CSTAGE 8       // MODE 3: keys staged in LDS per (lane, query tile)
@@ -210,16 +212,27 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
   }
 
   // the next block's base operand is loaded while the current one is
-  // multiplied (register double buffer, QRES only)
+  // multiplied (register double buffer, QRES only).  Buffer loads: a
+  // descriptor based at the item's first row, the block's row offset in a
+  // scalar register, the lane's (row, half) offset in a vector register
+  // (rows past nb clamped to the last row, masked by the epilogue) -- no
+  // 64-bit vector address arithmetic per block.  The planner keeps an
+  // item's rows within 2^31 bytes.
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(a.b + bbeg * ld), 0, (int)min((a.nb - bbeg) * ld * 4, 0x7fffffffL), YK_BUF_DWORD3);
   float4 anext[QRES ? 2 * NCH : 1];
-  if (QRES && bbeg < bend) {
-    const float *np = a.b + min(bbeg + r, a.nb - 1) * ld + h * 8;
+  auto loadb = [&](long bb) {
+    const int soff = (int)((bb - bbeg) * ld * 4);
+    const int voff = ((int)min((long)r, a.nb - 1 - bb) * ld + h * 8) * 4;
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
-      anext[2 * c] = *(const float4 *)(np + c * 16);
-      anext[2 * c + 1] = *(const float4 *)(np + c * 16 + 4);
+      anext[2 * c] = __builtin_bit_cast(
+          float4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff + c * 64, 0));
+      anext[2 * c + 1] = __builtin_bit_cast(
+          float4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff + 16, soff + c * 64, 0));
     }
-  }
+  };
+  if (QRES && bbeg < bend) loadb(bbeg);
   for (long bb = bbeg; bb < bend; bb += a.bstep) {
     const long brow = min(bb + r, a.nb - 1);
     const float *bp = a.b + brow * ld + h * 8;
@@ -238,13 +251,7 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
         // conditional prefetch joins paths with different outstanding-load
         // counts, and the wait before the MFMAs would then also wait for the
         // loads just issued
-        const long nxt = bb + a.bstep < bend ? bb + a.bstep : bb;
-        const float *np = a.b + min(nxt + r, a.nb - 1) * ld + h * 8;
-#pragma unroll
-        for (int c = 0; c < NCH; c++) {
-          anext[2 * c] = *(const float4 *)(np + c * 16);
-          anext[2 * c + 1] = *(const float4 *)(np + c * 16 + 4);
-        }
+        loadb(bb + a.bstep < bend ? bb + a.bstep : bb);
       }
 #pragma unroll
       for (int c = 0; c < NCH; c++)
@@ -527,6 +534,9 @@ static YkPlan yk_plan(long nq, long nb, long sample, long min_splits, long max_s
   if (want > nblocks) want = nblocks;
   if (want < 1) want = 1;
   long per = (nblocks + want - 1) / want;  // processed blocks per split
+  // an item's rows stay within 2^30 bytes (buffer-load offsets are 32-bit)
+  const long max_per = (1L << 30) / (32L * 128 * 4 * sample);
+  if (per > max_per) per = max_per;
   p.brange = per * p.bstep;
   p.nsplit = (int)((nb + p.brange - 1) / p.brange);
   return p;
