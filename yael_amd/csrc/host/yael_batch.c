@@ -125,7 +125,8 @@ static int yh_batch_bind(yh_bstate *s, int P, yh_allocs *al, yk_batch **out) {
     for (long u = uf[i]; u < uf[i + 1]; u++) ui[u] = i;
   int *d_uf = (int *)yh_dmalloc(al, sizeof(int) * (ninst + 1));
   int *d_ui = (int *)yh_dmalloc(al, sizeof(int) * nu);
-  double *psum = (double *)yh_dmalloc(al, sizeof(double) * (size_t)nu * s->k * s->d);
+  /* partial sums, then one objective partial per chunk */
+  double *psum = (double *)yh_dmalloc(al, sizeof(double) * ((size_t)nu * s->k * s->d + nu));
   int *pcnt = (int *)yh_dmalloc(al, sizeof(int) * (size_t)nu * s->k);
   int rc = (!d_uf || !d_ui || !psum || !pcnt) ? YK_ENOMEM : YK_OK;
   if (rc == YK_OK) rc = yk_memcpy_h2d(d_uf, uf, sizeof(int) * (ninst + 1));

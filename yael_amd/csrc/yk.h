@@ -130,7 +130,9 @@ int yk_gather_rows(const float *src, const int *idx, long n, int d, float *dst);
 /* Batched form (yk_batch.hip): many k-means instances advanced together.
  * yk_batch_assign: the FP32 assign step over work items (instance, 128-point
  * tile), centroid sets kpad rows apart.  yk_batch_create binds device
- * arrays laid out by the caller (see host/yael_batch.c); yk_batch_run queues
+ * arrays laid out by the caller (see host/yael_batch.c; psum holds
+ * nuitems * (k * d + 1) doubles: the chunks' partial sums, then their
+ * objective partials); yk_batch_run queues
  * niter iterations, replaying one captured hipGraph when use_graph. */
 typedef struct yk_batch yk_batch;
 int yk_batch_assign(long nitems, const int *item_inst, const int *item_tile, const long *inst_off,

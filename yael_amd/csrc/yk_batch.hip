@@ -9,11 +9,11 @@
 //   assign                            yk_batch_assign (yk_select.hip): the
 //                                     FP32 scan kernel with a work-item ->
 //                                     (instance, 128-point tile) table
-//   update, partial sums              yk_bupd_partial_kernel: one workgroup
-//                                     per (instance, chunk of P points); one
-//                                     thread per dimension adds the chunk's
-//                                     points in point order into an LDS
-//                                     FP64 table [k][d]
+//   update, partial sums              yk_bupd_partial_kernel: one wave per
+//                                     (instance, chunk of P points) adds the
+//                                     chunk's points in point order into an
+//                                     LDS FP64 table [k][d] (ds_add_f64),
+//                                     and sums the chunk's objective
 //   update, reduce + scale            yk_bupd_reduce_kernel: chunks summed in
 //                                     chunk order, (float)(sum * (1/count))
 //   empty-cluster split               yk_bsplit_kernel: kmeans.c:166-209 on
@@ -21,7 +21,8 @@
 //                                     host-precomputed per-(instance,
 //                                     iteration) tables (the streams depend
 //                                     only on the seed, kmeans.c:296)
-//   objective, abandon and stop rules yk_bobj_kernel (kmeans.c:300-313)
+//   objective, abandon and stop rules yk_bobj_kernel (kmeans.c:300-313):
+//                                     chunk objectives in chunk order
 //   iteration counter                 yk_bstep_kernel
 // Finished instances are skipped by every kernel (done flags), so their
 // outputs are those of the iteration at which the reference would stop.
@@ -47,6 +48,7 @@ struct BatchDev {
   long nuitems;
   const int *uitem_inst, *uitem_first;  // update chunks; first chunk of each instance
   double *psum;  // nuitems x k x d
+  double *pobj;  // nuitems (chunk objective partials, after psum)
   int *pcnt;     // nuitems x k
   int *nassign;  // ninst x k
   float *pbuf;   // ninst x k (split probabilities)
@@ -57,53 +59,86 @@ struct BatchDev {
   int *it;  // iteration counter
 };
 
-// One workgroup per update chunk; thread t < d owns dimension t.
+// One workgroup per update chunk adds the chunk's points into an LDS FP64
+// table [k][d].  d >= 64: one thread per dimension (up to 4 waves), each
+// (cluster, dimension) sum owned by one thread and accumulated in point
+// order.  d < 64: one wave whose lanes split into 64/d groups of d, 64/d
+// points per pass, with ds_add_f64: LDS atomics of one wave apply in
+// instruction order and, inside one instruction, in lane order, so the
+// sums are again in point order.  Deterministic, whatever the grid.  The
+// chunk's objective (its points' distances, FP64) goes to pobj[u].
 __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
   extern __shared__ double lsum[];  // k x d
   int *lcnt = (int *)(lsum + (size_t)b.k * b.d);
+  const int tid = threadIdx.x, nthr = blockDim.x;
   const long u = blockIdx.x;
   const int inst = b.uitem_inst[u];
   if (b.done[inst]) return;
   const long off = b.inst_off[inst], n = b.inst_off[inst + 1] - off;
   const long c0 = (u - b.uitem_first[inst]) * (long)b.chunk;
   const long p0 = off + c0, p1 = off + (c0 + b.chunk < n ? c0 + b.chunk : n);
-  for (int i = threadIdx.x; i < b.k * b.d; i += blockDim.x) lsum[i] = 0.0;
-  for (int i = threadIdx.x; i < b.k; i += blockDim.x) lcnt[i] = 0;
-  __syncthreads();
+  for (int i = tid; i < b.k * b.d; i += nthr) lsum[i] = 0.0;
+  for (int i = tid; i < b.k; i += nthr) lcnt[i] = 0;
   // a point without a nearest centroid (NaN input) has assign -1: the
   // instance is handed to the single-instance path, which reports it
-  for (long p = p0 + threadIdx.x; p < p1; p += blockDim.x)
-    if ((unsigned)b.assign[p] >= (unsigned)b.k) b.overflow[inst] = 1;
-  __syncthreads();
-  if (b.overflow[inst]) return;
-  for (int t = threadIdx.x; t < b.d; t += blockDim.x) {
-    long p = p0;
-    for (; p + 4 <= p1; p += 4) {
-      const int a0 = b.assign[p], a1 = b.assign[p + 1], a2 = b.assign[p + 2],
-                a3 = b.assign[p + 3];
-      const float x0 = b.v[p * b.ld + t], x1 = b.v[(p + 1) * b.ld + t],
-                  x2 = b.v[(p + 2) * b.ld + t], x3 = b.v[(p + 3) * b.ld + t];
-      lsum[a0 * b.d + t] += (double)x0;
-      lsum[a1 * b.d + t] += (double)x1;
-      lsum[a2 * b.d + t] += (double)x2;
-      lsum[a3 * b.d + t] += (double)x3;
-      if (t == 0) {
-        lcnt[a0]++;
-        lcnt[a1]++;
-        lcnt[a2]++;
-        lcnt[a3]++;
+  bool bad = false;
+  for (long p = p0 + tid; p < p1; p += nthr) bad |= (unsigned)b.assign[p] >= (unsigned)b.k;
+  if (__syncthreads_or(bad)) {
+    if (tid == 0) b.overflow[inst] = 1;
+    return;
+  }
+  if (tid < 64) {  // the chunk objective, one wave, fixed order
+    double obj = 0.0;
+    for (long p = p0 + tid; p < p1; p += 64) obj += (double)b.dis[p];
+    for (int o = 32; o > 0; o >>= 1) obj += __shfl_xor(obj, o);
+    if (tid == 0) b.pobj[u] = obj;
+  }
+  const int d = b.d, ld = b.ld;
+  if (d >= 64) {
+    // a thread owns its dimensions: plain read-add-write, four points in
+    // flight
+    for (int t = tid; t < d; t += nthr) {
+      long p = p0;
+      for (; p + 4 <= p1; p += 4) {
+        const int a0 = b.assign[p], a1 = b.assign[p + 1], a2 = b.assign[p + 2],
+                  a3 = b.assign[p + 3];
+        const float x0 = b.v[p * ld + t], x1 = b.v[(p + 1) * ld + t], x2 = b.v[(p + 2) * ld + t],
+                    x3 = b.v[(p + 3) * ld + t];
+        lsum[a0 * d + t] += (double)x0;
+        lsum[a1 * d + t] += (double)x1;
+        lsum[a2 * d + t] += (double)x2;
+        lsum[a3 * d + t] += (double)x3;
+        if (t == 0) {
+          lcnt[a0]++;
+          lcnt[a1]++;
+          lcnt[a2]++;
+          lcnt[a3]++;
+        }
+      }
+      for (; p < p1; p++) {
+        const int a = b.assign[p];
+        lsum[a * d + t] += (double)b.v[p * ld + t];
+        if (t == 0) lcnt[a]++;
       }
     }
-    for (; p < p1; p++) {
-      const int a = b.assign[p];
-      lsum[a * b.d + t] += (double)b.v[p * b.ld + t];
-      if (t == 0) lcnt[a]++;
+  } else {
+    // one wave: lanes in 64/d groups of d, LDS atomics (same address from
+    // two groups of one instruction: applied in lane order)
+    const int ppp = 64 / d, g = tid / d, t = tid - g * d;
+    if (g < ppp) {
+      for (long p = p0 + g; p < p1; p += ppp) {
+        const int a = b.assign[p];
+        __hip_atomic_fetch_add(&lsum[a * d + t], (double)b.v[p * ld + t], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (t == 0)
+          __hip_atomic_fetch_add(&lcnt[a], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
     }
   }
   __syncthreads();
   double *ps = b.psum + u * (size_t)b.k * b.d;
-  for (int i = threadIdx.x; i < b.k * b.d; i += blockDim.x) ps[i] = lsum[i];
-  for (int i = threadIdx.x; i < b.k; i += blockDim.x) b.pcnt[u * b.k + i] = lcnt[i];
+  for (int i = tid; i < b.k * b.d; i += nthr) ps[i] = lsum[i];
+  for (int i = tid; i < b.k; i += nthr) b.pcnt[u * b.k + i] = lcnt[i];
 }
 
 // One wave per (instance, cluster): chunk sums in chunk order, then
@@ -193,9 +228,12 @@ __global__ __launch_bounds__(256) void yk_bobj_kernel(BatchDev b) {
   __shared__ double s[256];
   const int inst = blockIdx.x;
   if (b.done[inst]) return;
-  const long off = b.inst_off[inst], n = b.inst_off[inst + 1] - off;
+  const long n = b.inst_off[inst + 1] - b.inst_off[inst];
+  // the chunks' objective partials, in chunk order per thread, then a
+  // fixed tree
   double acc = 0.0;
-  for (long i = threadIdx.x; i < n; i += 256) acc += (double)b.dis[off + i];
+  for (long u = b.uitem_first[inst] + threadIdx.x; u < b.uitem_first[inst + 1]; u += 256)
+    acc += b.pobj[u];
   s[threadIdx.x] = acc;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -275,7 +313,7 @@ extern "C" int yk_batch_create(yk_batch **out, int ninst, int d, int k, int kpad
   b.v = v; b.vn = vn; b.cent = cent; b.cn = cn; b.assign = assign; b.dis = dis;
   b.inst_off = inst_off; b.nitems = nitems; b.item_inst = item_inst; b.item_tile = item_tile;
   b.nuitems = nuitems; b.uitem_inst = uitem_inst; b.uitem_first = uitem_first;
-  b.psum = psum; b.pcnt = pcnt; b.nassign = nassign; b.pbuf = pbuf; b.rd = rd; b.eps = eps;
+  b.psum = psum; b.pobj = psum + (size_t)nuitems * k * d; b.pcnt = pcnt; b.nassign = nassign; b.pbuf = pbuf; b.rd = rd; b.eps = eps;
   // flags: 7 int arrays of ninst + the counter
   b.done = flags; b.overflow = flags + ninst; b.nre = flags + 2L * ninst;
   b.tot = flags + 3L * ninst; b.iters = flags + 4L * ninst; b.abandoned = flags + 5L * ninst;
@@ -305,9 +343,9 @@ static int yk_batch_enqueue(yk_batch *bt) {
   YK_TRY(yk_norms(b.cent, (long)b.ninst * b.kpad, b.d, b.cn, nullptr));
   YK_TRY(yk_batch_assign(b.nitems, b.item_inst, b.item_tile, b.inst_off, b.d, b.k, b.kpad, b.v,
                          b.vn, b.cent, b.cn, b.done, b.assign, b.dis));
-  int bt_threads = ((b.d + 63) / 64) * 64;
-  if (bt_threads > 256) bt_threads = 256;
-  hipLaunchKernelGGL(yk_bupd_partial_kernel, dim3((unsigned)b.nuitems), dim3(bt_threads),
+  // d < 64: one wave; else one thread per dimension (at most 256)
+  const int pthreads = b.d < 64 ? 64 : (b.d > 256 ? 256 : ((b.d + 63) / 64) * 64);
+  hipLaunchKernelGGL(yk_bupd_partial_kernel, dim3((unsigned)b.nuitems), dim3(pthreads),
                      yk_batch_lds_bytes(b.k, b.d), st, b);
   hipLaunchKernelGGL(yk_bupd_reduce_kernel, dim3((unsigned)(((long)b.ninst * b.k + 3) / 4)),
                      dim3(256), 0, st, b);

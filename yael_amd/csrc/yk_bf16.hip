@@ -72,15 +72,9 @@ extern "C" long yk_bf16_bytes(long n, int d, int type, int is_query) {
   return rows * yk_bf16_ld(d, type) * (long)sizeof(uint16_t);
 }
 
-// element (i, j) of a BF16 copy: row-major for queries, fragment-major for
-// the base
-__device__ __forceinline__ long yk_bf_at(long i, int j, int ldh, bool frag) {
-  if (!frag) return i * ldh + j;
-  const int nch = ldh >> 4;
-  return (((i >> 5) * nch + (j >> 4)) * 64 + ((j >> 3) & 1) * 32 + (i & 31)) * 8 + (j & 7);
-}
 
-// One wave per vector (coalesced): BF16 row (augmented for L2) + error
+// Query copies (row-major), one wave per vector (coalesced): BF16 row
+// (augmented for L2) + error
 // statistics stats[i] = {|x|, |bf(x)|, |x - bf(x)|, |x|^2} (float); the
 // per-set maxima go to smax (uint bit patterns of non-negative floats, whose
 // integer order is their float order) by atomicMax.
@@ -93,21 +87,16 @@ __global__ __launch_bounds__(256) void yk_to_bf16_kernel(const float *__restrict
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __shared__ float4 wmax[4];
   float4 mx = make_float4(0.f, 0.f, 0.f, 0.f);
-  const bool frag = !is_query;
-  const long nrows = frag ? (n + 31) & ~31L : n;
-  for (long i = blockIdx.x * 4L + wave; i < nrows; i += gridDim.x * 4L) {
-  if (i >= n) {  // zero pad rows of the fragment-major base copy
-    for (int j = lane; j < ldh; j += 64) out[yk_bf_at(i, j, ldh, true)] = 0;
-    continue;
-  }
+  for (long i = blockIdx.x * 4L + wave; i < n; i += gridDim.x * 4L) {
   const float *row = x + i * ld;
+  uint16_t *o = out + i * ldh;
   double s2 = 0, sb = 0, se = 0;
   const float scale = (type != YK_DOT && is_query) ? -2.0f : 1.0f;
   for (int j = lane; j < d; j += 64) {
     float v = row[j];
     uint16_t hb = yk_bf16_rne(v * scale);
     float vb = yk_bf16_f(hb) / scale;  // exact: scale is a power of two
-    out[yk_bf_at(i, j, ldh, frag)] = hb;
+    o[j] = hb;
     s2 += (double)v * v;
     sb += (double)vb * vb;
     se += ((double)v - vb) * ((double)v - vb);
@@ -134,8 +123,7 @@ __global__ __launch_bounds__(256) void yk_to_bf16_kernel(const float *__restrict
     }
     extra = 6;
   }
-  for (int j = d + lane; j < ldh; j += 64)
-    out[yk_bf_at(i, j, ldh, frag)] = (j - d) < extra ? tail[j - d] : 0;
+  for (int j = d + lane; j < ldh; j += 64) o[j] = (j - d) < extra ? tail[j - d] : 0;
   float4 s = make_float4((float)sqrt(s2), (float)sqrt(sb), (float)sqrt(se), (float)s2);
   if (lane == 0) stats[i] = s;
   mx = make_float4(fmaxf(mx.x, s.x), fmaxf(mx.y, s.y), fmaxf(mx.z, s.z), fmaxf(mx.w, s.w));
@@ -157,6 +145,108 @@ __global__ void yk_zero4_kernel(unsigned *p) {
   if (threadIdx.x < 4) p[threadIdx.x] = 0;
 }
 
+// The base copy (fragment-major): one wave per 32-row block; lane h*32 + r
+// owns row 32b + r, columns 16c + 8h .. +7 of every chunk c, so each chunk is
+// one coalesced 1 KB store.  Pass 1 sums the row statistics (the lane pair
+// combines its halves by a shuffle), pass 2 converts again (the rows are
+// cached) and stores, appending the norm pieces and ones after column d.
+// Conversion, norm pieces and statistics are those of yk_to_bf16_kernel.
+__global__ __launch_bounds__(256) void yk_to_bf16_frag_kernel(const float *__restrict__ x, long n,
+                                                              int d, int ld, int ldh, int type,
+                                                              uint16_t *__restrict__ out,
+                                                              float4 *__restrict__ stats,
+                                                              unsigned *smax) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 31, h = lane >> 5;
+  const int nch = ldh >> 4;
+  const long nblk = (n + 31) >> 5;
+  __shared__ float4 wmax[4];
+  float4 mx = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (long b = blockIdx.x * 4L + wave; b < nblk; b += gridDim.x * 4L) {
+    const long row = b * 32 + r;
+    const bool valid = row < n;
+    const float *xr = x + (valid ? row : 0) * ld;
+    uint4 *ob = (uint4 *)(out + (b * nch * 64 + lane) * 8);
+    // the lane's 8 values of chunk c (columns past d read as zero; the FP32
+    // rows are padded to ld, a multiple of 16, so full 8-column runs below
+    // d are two aligned float4 loads)
+    auto vals = [&](int c, float (&v)[8]) {
+      const int c0 = 16 * c + 8 * h;
+      if (valid && c0 + 8 <= d) {
+        const float4 p = *(const float4 *)(xr + c0), q = *(const float4 *)(xr + c0 + 4);
+        v[0] = p.x; v[1] = p.y; v[2] = p.z; v[3] = p.w;
+        v[4] = q.x; v[5] = q.y; v[6] = q.z; v[7] = q.w;
+      } else {
+#pragma unroll
+        for (int t = 0; t < 8; t++) v[t] = (valid && c0 + t < d) ? xr[c0 + t] : 0.f;
+      }
+    };
+    double s2 = 0, sb = 0, se = 0;
+    for (int c = 0; c < nch; c++) {
+      float v[8];
+      vals(c, v);
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        const float vb = yk_bf16_f(yk_bf16_rne(v[t]));
+        s2 += (double)v[t] * v[t];
+        sb += (double)vb * vb;
+        se += ((double)v[t] - vb) * ((double)v[t] - vb);
+      }
+    }
+    s2 += __shfl_xor(s2, 32);
+    sb += __shfl_xor(sb, 32);
+    se += __shfl_xor(se, 32);
+    const uint16_t one = 0x3f80;
+    uint16_t tail[6] = {0, 0, 0, 0, 0, 0};
+    if (type != YK_DOT && valid) {
+      const float nf = (float)s2;
+      const uint16_t h1 = yk_bf16_rne(nf);
+      const float r1 = nf - yk_bf16_f(h1);
+      const uint16_t h2 = yk_bf16_rne(r1);
+      const float r2 = r1 - yk_bf16_f(h2);
+      const uint16_t h3 = yk_bf16_rne(r2);
+      tail[0] = h1; tail[1] = h2; tail[2] = h3; tail[3] = one; tail[4] = one; tail[5] = one;
+    }
+    for (int c = 0; c < nch; c++) {
+      const int c0 = 16 * c + 8 * h;
+      float v[8];
+      vals(c, v);
+      uint16_t e[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        const int col = c0 + t;
+        e[t] = col < d ? yk_bf16_rne(v[t]) : ((col - d) < 6 ? tail[col - d] : 0);
+      }
+      ob[c * 64] = make_uint4((uint32_t)e[0] | ((uint32_t)e[1] << 16),
+                              (uint32_t)e[2] | ((uint32_t)e[3] << 16),
+                              (uint32_t)e[4] | ((uint32_t)e[5] << 16),
+                              (uint32_t)e[6] | ((uint32_t)e[7] << 16));
+    }
+    if (valid) {
+      const float4 st = make_float4((float)sqrt(s2), (float)sqrt(sb), (float)sqrt(se), (float)s2);
+      if (h == 0) stats[row] = st;
+      mx = make_float4(fmaxf(mx.x, st.x), fmaxf(mx.y, st.y), fmaxf(mx.z, st.z), fmaxf(mx.w, st.w));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    mx.x = fmaxf(mx.x, __shfl_xor(mx.x, o));
+    mx.y = fmaxf(mx.y, __shfl_xor(mx.y, o));
+    mx.z = fmaxf(mx.z, __shfl_xor(mx.z, o));
+    mx.w = fmaxf(mx.w, __shfl_xor(mx.w, o));
+  }
+  if (lane == 0) wmax[wave] = mx;
+  __syncthreads();
+  if (smax && threadIdx.x == 0) {
+    for (int w = 1; w < 4; w++)
+      mx = make_float4(fmaxf(mx.x, wmax[w].x), fmaxf(mx.y, wmax[w].y), fmaxf(mx.z, wmax[w].z),
+                       fmaxf(mx.w, wmax[w].w));
+    atomicMax(smax + 0, __float_as_uint(mx.x));
+    atomicMax(smax + 1, __float_as_uint(mx.y));
+    atomicMax(smax + 2, __float_as_uint(mx.z));
+    atomicMax(smax + 3, __float_as_uint(mx.w));
+  }
+}
+
 extern "C" int yk_to_bf16(const float *x, long n, int d, int type, int is_query, uint16_t *out,
                           float *stats, float *stats_max) {
   YK_TRY(yk_init());
@@ -165,11 +255,17 @@ extern "C" int yk_to_bf16(const float *x, long n, int d, int type, int is_query,
   const int ld = yk_ld(d), ldh = yk_bf16_ld(d, type);
   if (stats_max) hipLaunchKernelGGL(yk_zero4_kernel, dim3(1), dim3(64), 0, yk_stream(),
                                     (unsigned *)stats_max);
-  long blocks = (n + 31 + 3) / 4;
-  if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(yk_to_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, yk_stream(),
-                     x, n, d, ld, ldh, type, is_query, out, (float4 *)stats,
-                     (unsigned *)stats_max);
+  if (is_query) {
+    long blocks = (n + 3) / 4;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(yk_to_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, yk_stream(), x, n,
+                       d, ld, ldh, type, is_query, out, (float4 *)stats, (unsigned *)stats_max);
+  } else {
+    long blocks = ((n + 31) / 32 + 3) / 4;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(yk_to_bf16_frag_kernel, dim3((unsigned)blocks), dim3(256), 0, yk_stream(),
+                       x, n, d, ld, ldh, type, out, (float4 *)stats, (unsigned *)stats_max);
+  }
   YK_HIP(hipGetLastError());
   return YK_OK;
 }
@@ -589,6 +685,7 @@ __global__ void yk_bf_zero_kernel(int *p, long n) {
 // FP32 re-rank (bit-identical to the FP32 form) + certificate.
 // One wave per query; MFMA rows = 32 candidates, every column the query.
 
+template <int NCH>
 __global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
     const float *__restrict__ q, const double *__restrict__ qn, const float *__restrict__ b,
     const float *__restrict__ bn, long nb, int ld, int type, const uint64_t *__restrict__ cand,
@@ -597,47 +694,104 @@ __global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
   __shared__ uint64_t ex[YK_KMAX];
   const long qi = blockIdx.x;
   const int lane = threadIdx.x, r = lane & 31, h = lane >> 5;
-  const int nch = ld >> 4;
+  const int nch = NCH > 0 ? NCH : ld >> 4;
   const bool dot = type == YK_DOT;
   const uint64_t *cq = cand + qi * kp;
   const float *qp = q + qi * ld + h * 8;
   const double qnd = dot ? 0.0 : qn[qi];
-  for (int t0 = 0; t0 < kp; t0 += 32) {
-    const int ci = t0 + r;
-    const uint64_t key = ci < kp ? cq[ci] : YK_KEY_NONE;
-    const long row = key == YK_KEY_NONE ? 0 : (long)(uint32_t)key;
-    const float *bp = b + row * ld + h * 8;
-    f32x16 acc;
+  // NCH > 0: the query's chunks held in registers for all candidate tiles
+  float4 qr[NCH > 0 ? 2 * NCH : 1];
+  if (NCH > 0) {
 #pragma unroll
-    for (int e = 0; e < 16; e++) acc[e] = 0.f;
-    for (int c = 0; c < nch; c++) {
-      float4 a0 = *(const float4 *)(bp + c * 16), a1 = *(const float4 *)(bp + c * 16 + 4);
-      float4 b0 = *(const float4 *)(qp + c * 16), b1 = *(const float4 *)(qp + c * 16 + 4);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc, 0, 0, 0);
+    for (int c = 0; c < NCH; c++) {
+      qr[2 * c] = *(const float4 *)(qp + c * 16);
+      qr[2 * c + 1] = *(const float4 *)(qp + c * 16 + 4);
     }
-    // column 0 (lanes 0 and 32) holds the 32 candidate rows 8g + 4h + e
+  }
+  // Two 32-candidate tiles at a time, their MFMA chains interleaved (each
+  // chain is serial: the FP32 form's instruction sequence per pair).
+  for (int t0 = 0; t0 < kp; t0 += 64) {
+    long row[2];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int ci = t0 + 32 * u + r;
+      const uint64_t key = ci < kp ? cq[ci] : YK_KEY_NONE;
+      row[u] = key == YK_KEY_NONE ? 0 : (long)(uint32_t)key;
+    }
+    const float *bp0 = b + row[0] * ld + h * 8, *bp1 = b + row[1] * ld + h * 8;
+    f32x16 acc0, acc1;
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc0[e] = acc1[e] = 0.f;
+    auto chunk = [&](int c, float4 b0, float4 b1) {
+      const float4 a0 = *(const float4 *)(bp0 + c * 16), a1 = *(const float4 *)(bp0 + c * 16 + 4);
+      const float4 e0 = *(const float4 *)(bp1 + c * 16), e1 = *(const float4 *)(bp1 + c * 16 + 4);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e0.x, b0.x, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e0.y, b0.y, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e0.z, b0.z, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e0.w, b0.w, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e1.x, b1.x, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e1.y, b1.y, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e1.z, b1.z, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e1.w, b1.w, acc1, 0, 0, 0);
+    };
+    if (t0 + 32 >= kp) {
+      // one tile left: its chain alone
+      auto chunk1 = [&](int c, float4 b0, float4 b1) {
+        const float4 a0 = *(const float4 *)(bp0 + c * 16),
+                     a1 = *(const float4 *)(bp0 + c * 16 + 4);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc0, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc0, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc0, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc0, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc0, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc0, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc0, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc0, 0, 0, 0);
+      };
+      if (NCH > 0) {
+#pragma unroll
+        for (int c = 0; c < NCH; c++) chunk1(c, qr[2 * c], qr[2 * c + 1]);
+      } else {
+        for (int c = 0; c < nch; c++)
+          chunk1(c, *(const float4 *)(qp + c * 16), *(const float4 *)(qp + c * 16 + 4));
+      }
+    } else if (NCH > 0) {
+#pragma unroll
+      for (int c = 0; c < NCH; c++) chunk(c, qr[2 * c], qr[2 * c + 1]);
+    } else {
+      for (int c = 0; c < nch; c++)
+        chunk(c, *(const float4 *)(qp + c * 16), *(const float4 *)(qp + c * 16 + 4));
+    }
+    // column 0 (lanes 0 and 32) holds the 32 candidate rows 8g + 4h + e of
+    // each tile
     if (r == 0) {
 #pragma unroll
-      for (int i = 0; i < 16; i++) {
-        const int m = 8 * (i >> 2) + 4 * h + (i & 3);
-        const int cidx = t0 + m;
-        if (cidx >= kp) continue;
-        const uint64_t ck = cq[cidx];
-        uint64_t xk = YK_KEY_NONE;
-        if (ck != YK_KEY_NONE) {
-          const int bi = (int)(uint32_t)ck;
-          float v = dot ? acc[i] : (float)(qnd + (double)bn[bi]) - 2.0f * acc[i];
-          // k = 1 follows nn.c:404-440: only values below 1e30 are taken
-          if (v == v && (k > 1 || v < 1e30f)) xk = yk_key(v, bi);
+      for (int u = 0; u < 2; u++) {
+        if (u == 1 && t0 + 32 >= kp) break;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+          const int m = 8 * (i >> 2) + 4 * h + (i & 3);
+          const int cidx = t0 + 32 * u + m;
+          if (cidx >= kp) continue;
+          const uint64_t ck = cq[cidx];
+          uint64_t xk = YK_KEY_NONE;
+          if (ck != YK_KEY_NONE) {
+            const int bi = (int)(uint32_t)ck;
+            const float dv = u ? acc1[i] : acc0[i];
+            float v = dot ? dv : (float)(qnd + (double)bn[bi]) - 2.0f * dv;
+            // k = 1 follows nn.c:404-440: only values below 1e30 are taken
+            if (v == v && (k > 1 || v < 1e30f)) xk = yk_key(v, bi);
+          }
+          ex[cidx] = xk;
         }
-        ex[cidx] = xk;
       }
     }
   }
@@ -906,7 +1060,21 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
     YK_TRY(yk_bf16_candidates(a, kp, (uint64_t *)vcand));
     hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), nunc, 1L);
     int tok = yk_prof_begin(YK_PROF_RERANK);
-    hipLaunchKernelGGL(yk_rerank_cert_kernel, dim3((unsigned)n), dim3(64), 0, yk_stream(),
+    void (*rk)(const float *, const double *, const float *, const float *, long, int, int,
+               const uint64_t *, int, int, const float4 *, const float4 *, float, float, int *,
+               float *, int *, int *);
+    switch (ld >> 4) {
+      case 1: rk = yk_rerank_cert_kernel<1>; break;
+      case 2: rk = yk_rerank_cert_kernel<2>; break;
+      case 3: rk = yk_rerank_cert_kernel<3>; break;
+      case 4: rk = yk_rerank_cert_kernel<4>; break;
+      case 5: rk = yk_rerank_cert_kernel<5>; break;
+      case 6: rk = yk_rerank_cert_kernel<6>; break;
+      case 7: rk = yk_rerank_cert_kernel<7>; break;
+      case 8: rk = yk_rerank_cert_kernel<8>; break;
+      default: rk = yk_rerank_cert_kernel<0>; break;
+    }
+    hipLaunchKernelGGL(rk, dim3((unsigned)n), dim3(64), 0, yk_stream(),
                        q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, nb, ld, type,
                        (const uint64_t *)vcand, kp, k, (const float4 *)qstats + q0,
                        (const float4 *)bstats_max, gam_bf, gam_32, assign + q0 * k, dis + q0 * k,
