@@ -261,8 +261,10 @@ extern "C" int yk_to_bf16(const float *x, long n, int d, int type, int is_query,
     hipLaunchKernelGGL(yk_to_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, yk_stream(), x, n,
                        d, ld, ldh, type, is_query, out, (float4 *)stats, (unsigned *)stats_max);
   } else {
+    // a grid-stride grid of at most 512 workgroups: enough to stream the set,
+    // few enough that the per-workgroup maxima do not queue on 4 atomics
     long blocks = ((n + 31) / 32 + 3) / 4;
-    if (blocks > 4096) blocks = 4096;
+    if (blocks > 512) blocks = 512;
     hipLaunchKernelGGL(yk_to_bf16_frag_kernel, dim3((unsigned)blocks), dim3(256), 0, yk_stream(),
                        x, n, d, ld, ldh, type, out, (float4 *)stats, (unsigned *)stats_max);
   }
@@ -708,9 +710,38 @@ __global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
       qr[2 * c + 1] = *(const float4 *)(qp + c * 16 + 4);
     }
   }
+  // Bound on |FP32-form value - first-pass value| of any pair of this query
+  // (also the certificate's margin): E = E_bf + E_32.
+  const float4 qs = qstats[qi], bs = *bmax;  // {|x|, |bf x|, |e|, |x|^2}
+  double ebf, e32;
+  if (dot) {
+    ebf = (double)qs.z * bs.x + (double)qs.y * bs.z + gam_bf * (double)qs.y * bs.y;
+    e32 = gam_32 * (double)qs.x * bs.x;
+  } else {
+    ebf = 2.0 * ((double)qs.z * bs.x + (double)qs.y * bs.z) +
+          gam_bf * (2.0 * qs.y * bs.y + qs.w + bs.w);
+    e32 = gam_32 * (2.0 * qs.x * bs.x + qs.w + bs.w);
+  }
+  // Only candidates with first-pass value <= v_k + 2E can be among the FP32
+  // form's k nearest (the k first-pass-smallest have FP32 values <= v_k + E,
+  // any candidate above has FP32 value > v_k + E); the candidates are in
+  // ascending first-pass order, so the re-rank stops at the first tile that
+  // starts above the bound.
+  const uint64_t kkey = cq[k - 1];
+  const double bound = kkey == YK_KEY_NONE
+                           ? INFINITY
+                           : (double)yk_fkey_inv((uint32_t)(kkey >> 32)) + 2.0 * (ebf + e32);
+  auto needed = [&](int t) {
+    if (t >= kp) return false;
+    const uint64_t kk = cq[t];
+    return kk != YK_KEY_NONE && (double)yk_fkey_inv((uint32_t)(kk >> 32)) <= bound;
+  };
+  int nproc = 0;  // candidates re-ranked
   // Two 32-candidate tiles at a time, their MFMA chains interleaved (each
   // chain is serial: the FP32 form's instruction sequence per pair).
-  for (int t0 = 0; t0 < kp; t0 += 64) {
+  for (int t0 = 0; t0 < kp && needed(t0); t0 += 64) {
+    const bool two = needed(t0 + 32);
+    nproc = min(kp, t0 + (two ? 64 : 32));
     long row[2];
 #pragma unroll
     for (int u = 0; u < 2; u++) {
@@ -742,7 +773,7 @@ __global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
       acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc0, 0, 0, 0);
       acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e1.w, b1.w, acc1, 0, 0, 0);
     };
-    if (t0 + 32 >= kp) {
+    if (!two) {
       // one tile left: its chain alone
       auto chunk1 = [&](int c, float4 b0, float4 b1) {
         const float4 a0 = *(const float4 *)(bp0 + c * 16),
@@ -775,7 +806,7 @@ __global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
     if (r == 0) {
 #pragma unroll
       for (int u = 0; u < 2; u++) {
-        if (u == 1 && t0 + 32 >= kp) break;
+        if (u == 1 && !two) break;
 #pragma unroll
         for (int i = 0; i < 16; i++) {
           const int m = 8 * (i >> 2) + 4 * h + (i & 3);
@@ -797,8 +828,8 @@ __global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
   }
   __syncthreads();
   int p2 = 1;
-  while (p2 < kp) p2 <<= 1;
-  for (int i = kp + lane; i < p2; i += 64) ex[i] = YK_KEY_NONE;
+  while (p2 < nproc || p2 < k) p2 <<= 1;
+  for (int i = nproc + lane; i < p2; i += 64) ex[i] = YK_KEY_NONE;
   __syncthreads();
   yk_bitonic_b(ex, p2);
   // no result: (-1, 1e30) for k = 1 (nn.c:404), (-1, NaN bits) padding for
@@ -821,16 +852,6 @@ __global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
     } else {
       const float Tb = yk_fkey_inv((uint32_t)(last >> 32));
       const float Fk = yk_fkey_inv((uint32_t)(ex[k - 1] >> 32));
-      const float4 qs = qstats[qi], bs = *bmax;  // {|x|, |bf x|, |e|, |x|^2}
-      double ebf, e32;
-      if (dot) {
-        ebf = (double)qs.z * bs.x + (double)qs.y * bs.z + gam_bf * (double)qs.y * bs.y;
-        e32 = gam_32 * (double)qs.x * bs.x;
-      } else {
-        ebf = 2.0 * ((double)qs.z * bs.x + (double)qs.y * bs.z) +
-              gam_bf * (2.0 * qs.y * bs.y + qs.w + bs.w);
-        e32 = gam_32 * (2.0 * qs.x * bs.x + qs.w + bs.w);
-      }
       ok = (double)Fk < (double)Tb - ebf - e32;
     }
     if (!ok) unc_list[atomicAdd(n_unc, 1)] = (int)qi;

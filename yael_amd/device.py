@@ -83,8 +83,10 @@ class DeviceSet:
             nbytes = lib().yk_bf16_bytes(self.n, self.d, typ, int(is_query))
             self.bf[key] = (DeviceBuffer(nbytes), DeviceBuffer(self.n * 16), DeviceBuffer(16))
         h, st, mx = self.bf[key]
+        # the set maxima are needed for the base only (the certificate bounds
+        # use each query's own statistics)
         check(lib().yk_to_bf16(self.vec.ptr, self.n, self.d, typ, int(is_query), h.ptr, st.ptr,
-                               mx.ptr), "to_bf16")
+                               None if is_query else mx.ptr), "to_bf16")
         return h, st, mx
 
     def free(self):
