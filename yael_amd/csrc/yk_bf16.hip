@@ -283,7 +283,7 @@ struct Bf16Args {
   int ldh;
   long brange, bstep;
   int nsplit, tiles_x, total, per_xcd;
-  uint64_t *part;       // MODE 2: [nq][nsplit][2][R]
+  uint64_t *part;       // MODE 1: [nq][nsplit][2][2]; MODE 2: [nq][nsplit][2][R]
   const uint64_t *thr;  // MODE 3
   int *cnt;
   uint64_t *cand;       // MODE 3: candidate lists; MODE 2 with kp_direct: [nq][kp]
@@ -307,7 +307,7 @@ struct BfSel {
   long qrow[NB];
   bool qok[NB];
   uint64_t top[NB][MODE == 2 ? R : 1];
-  float bv[NB];
+  float bv[NB][MODE == 1 ? 2 : 1];  // MODE 1: minima of the even and odd blocks
   uint64_t T[NB];
   float Tv[NB];
   // MODE 3: candidate keys are staged in LDS, YK_CSTAGE per (query tile,
@@ -329,7 +329,8 @@ struct BfSel {
       const long qi = qbase + j * 32 + r;
       qok[j] = qi < a.nq;
       qrow[j] = qok[j] ? qi : a.nq - 1;
-      bv[j] = INFINITY;
+      bv[j][0] = INFINITY;
+      if (MODE == 1) bv[j][MODE == 1 ? 1 : 0] = INFINITY;
 #pragma unroll
       for (int s = 0; s < (MODE == 2 ? R : 1); s++) top[j][s] = YK_KEY_NONE;
       T[j] = MODE == 3 ? a.thr[qrow[j]] : 0;
@@ -354,30 +355,31 @@ struct BfSel {
   // fragment-major copy), masked with one integer compare against
   // rl = bend - bb - 4h.
   __device__ __forceinline__ void block(const Bf16Args &a, const f32x16 (&acc)[NB], long bb,
-                                        long bend, int h) {
+                                        long bend, int h, int sub) {
     // a uniform branch: the per-row mask exists only in the (rare) partial
     // block, so the common path carries no per-value row compares (MODE 2
     // keeps one masked copy: its insertion network is large enough that a
     // second copy costs the register budget of two waves per SIMD)
     const int rl = (int)min(bend - bb, 64L) - 4 * h;
     if (MODE != 2 && bb + 32 <= bend)
-      block_impl<true>(a, acc, bb, rl, h);
+      block_impl<true>(a, acc, bb, rl, h, sub);
     else
-      block_impl<false>(a, acc, bb, rl, h);
+      block_impl<false>(a, acc, bb, rl, h, sub);
   }
 
   template <bool FULL>
   __device__ __forceinline__ void block_impl(const Bf16Args &a, const f32x16 (&acc)[NB], long bb,
-                                             int rl, int h) {
+                                             int rl, int h, int sub) {
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       if (MODE == 1) {
-        // sketch: the lane's minimum value only (see finish)
-        float m = bv[j];
+        // sketch: the lane's minimum value only, kept apart for the even and
+        // odd blocks of the item (sub, a constant after inlining; see finish)
+        float m = bv[j][MODE == 1 ? sub : 0];
 #pragma unroll
         for (int i = 0; i < 16; i++)
           if (FULL || 8 * (i >> 2) + (i & 3) < rl) m = fminf(m, acc[j][i]);
-        bv[j] = m;
+        bv[j][MODE == 1 ? sub : 0] = m;
         continue;
       }
       float m = acc[j][0];
@@ -443,17 +445,22 @@ struct BfSel {
       for (int j = 0; j < NB; j++)
         if (ncs[j] > 0) flush(a, j, ncs[j]);
     } else if (MODE == 1) {
-      // Each lane's minimum over its rows of the item: the item's two lane
-      // halves and the splits cover disjoint row sets, so the k-th smallest
-      // of these values is attained by k distinct rows; the key (value,
-      // 0xffffffff) therefore bounds the k-th smallest key from above
-      // without the row index being known.
+      // Each lane's minima over its rows of the item's even and odd blocks:
+      // the (split, lane half, parity) subsets are disjoint row sets, so the
+      // k-th smallest of these values is attained by k distinct rows; the
+      // key (value, 0xffffffff) therefore bounds the k-th smallest key from
+      // above without the row index being known.  (More, smaller subsets
+      // bring the bound closer to the sample's own k-th value.)
 #pragma unroll
       for (int j = 0; j < NB; j++) {
         const long qi = qbase + j * 32 + r;
-        if (qok[j])
-          a.part[(qi * a.nsplit + split) * 2 + h] =
-              bv[j] < INFINITY ? (((uint64_t)yk_fkey(bv[j]) << 32) | 0xffffffffu) : YK_KEY_NONE;
+        if (!qok[j]) continue;
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const float v = bv[j][MODE == 1 ? u : 0];
+          a.part[((qi * a.nsplit + split) * 2 + h) * 2 + u] =
+              v < INFINITY ? (((uint64_t)yk_fkey(v) << 32) | 0xffffffffu) : YK_KEY_NONE;
+        }
       }
     } else if (MODE == 2 && a.kp_direct) {
       // lanes r and r + 32 hold sorted top-R lists of disjoint rows of the
@@ -561,7 +568,7 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
           buf[c] = __builtin_bit_cast(bf16x8, v);
         }
       };
-      auto step = [&](const bf16x8(&buf)[QRES ? NCH : 1], long bb) {
+      auto step = [&](const bf16x8(&buf)[QRES ? NCH : 1], long bb, int sub) {
         f32x16 acc[NB];
 #pragma unroll
         for (int j = 0; j < NB; j++)
@@ -573,7 +580,7 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
           for (int j = 0; j < NB; j++)
             acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(buf[c], qreg[j][QRES ? c : 0], acc[j],
                                                              0, 0, 0);
-        sel.block(a, acc, bb, bend, h);
+        sel.block(a, acc, bb, bend, h, sub);
       };
       // The prefetches are unconditional (past the range they reload the
       // current block): a conditional load would join two paths with
@@ -583,10 +590,10 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
       for (long bb = bbeg; bb < bend; bb += 2 * a.bstep) {
         const long b1 = bb + a.bstep, b2 = b1 + a.bstep;
         load(bufb, b1 < bend ? b1 : bb);
-        step(bufa, bb);
+        step(bufa, bb, 0);
         if (b1 >= bend) break;
         load(bufa, b2 < bend ? b2 : b1);
-        step(bufb, b1);
+        step(bufb, b1, 1);
       }
     } else {
       for (long bb = bbeg; bb < bend; bb += a.bstep) {
@@ -604,7 +611,7 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
             acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bq, acc[j], 0, 0, 0);
           }
         }
-        sel.block(a, acc, bb, bend, h);
+        sel.block(a, acc, bb, bend, h, 0);
       }
     }
     sel.finish(a, qbase, split, r, h);
@@ -1016,10 +1023,10 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   const long nblocks = (a.nb + 31) / 32;
   while (stride > 1 && nblocks / stride < 2L * kp) stride >>= 1;
   BPlan pa = yk_bplan(nq, a.nb, 2, stride, kp, kp > 256 ? kp : 256);
-  YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * pa.nsplit * 2, &vpart));
+  YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * pa.nsplit * 4, &vpart));
   a.part = (uint64_t *)vpart;
   YK_TRY((yk_launch_bscan<1, 1>(pa, a)));
-  int n = pa.nsplit * 2, p2 = 1;
+  int n = pa.nsplit * 4, p2 = 1;
   while (p2 < n) p2 <<= 1;
   hipLaunchKernelGGL(yk_bf_sortsel_kernel, dim3((unsigned)nq), dim3(256), sizeof(uint64_t) * p2,
                      yk_stream(), a.part, n, kp, p2, (uint64_t *)nullptr, thr);
