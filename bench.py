@@ -574,9 +574,16 @@ def main():
         cb = cpu_baseline_knn(base, queries, k)
         g_idx = r["i16"][:cb["nq"]]
         agree = float(np.mean(np.sort(g_idx, 1) == np.sort(cb["idx"], 1)))
+        # integer-valued data: every distance is exact in both, so the
+        # distance lists must be identical; indices differ only where equal
+        # distances tie at the k-th place (the reference's heap breaks such
+        # ties in its own order)
+        dagree = float(np.mean(np.all(r["d16"][:cb["nq"]].view(np.int32)
+                                      == cb["dis"].view(np.int32), axis=1)))
         out["cpu_baseline"] = {"value": cb["value"], "unit": cb["unit"], "cores": cb["cores"],
                                "kind": cb["kind"], "sample": cb["sample"],
-                               "index_agreement_with_gpu": agree}
+                               "index_agreement_with_gpu": agree,
+                               "distance_lists_identical_to_gpu": dagree}
         print("# cpu baseline: %.1f queries/s on %d cores" % (cb["value"], cb["cores"]),
               file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)

@@ -865,6 +865,123 @@ __global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
   }
 }
 
+// k' <= 8 (the k-means assign): four queries per wave.  Row r of the MFMA
+// tile is candidate r % 8 of query r / 8, column s < 4 is query s, so the
+// useful outputs (query s's candidates x column s) sit in lanes s and s + 32
+// (values i = 4s .. 4s + 3): a quarter of the MFMA chains of one query per
+// wave.  Each output element is still the FP32 form's instruction sequence
+// for its pair.  Results, conventions and certificate as in
+// yk_rerank_cert_kernel.
+template <int NCH>
+__global__ __launch_bounds__(64) void yk_rerank4_kernel(
+    const float *__restrict__ q, const double *__restrict__ qn, const float *__restrict__ b,
+    const float *__restrict__ bn, long nb, int ld, int type, const uint64_t *__restrict__ cand,
+    long nq, int kp, int k, const float4 *__restrict__ qstats, const float4 *__restrict__ bmax,
+    float gam_bf, float gam_32, int *out_i, float *out_d, int *unc_list, int *n_unc) {
+  __shared__ uint64_t ex[4][8];
+  const long q0 = blockIdx.x * 4L;
+  const int lane = threadIdx.x, r = lane & 31, h = lane >> 5;
+  const int nch = NCH > 0 ? NCH : ld >> 4;
+  const bool dot = type == YK_DOT;
+  // A operand: row r = candidate r % 8 of query q0 + r / 8
+  const long qa = q0 + (r >> 3);
+  const int ca = r & 7;
+  const uint64_t akey = (qa < nq && ca < kp) ? cand[qa * kp + ca] : YK_KEY_NONE;
+  const long arow = akey == YK_KEY_NONE ? 0 : (long)(uint32_t)akey;
+  const float *bp = b + arow * ld + h * 8;
+  // B operand: column r = query q0 + r for r < 4 (others repeat query q0)
+  const long qb = (r < 4 && q0 + r < nq) ? q0 + r : q0;
+  const float *qp = q + qb * ld + h * 8;
+  f32x16 acc;
+#pragma unroll
+  for (int e = 0; e < 16; e++) acc[e] = 0.f;
+  for (int c = 0; c < nch; c++) {
+    const float4 a0 = *(const float4 *)(bp + c * 16), a1 = *(const float4 *)(bp + c * 16 + 4);
+    const float4 b0 = *(const float4 *)(qp + c * 16), b1 = *(const float4 *)(qp + c * 16 + 4);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc, 0, 0, 0);
+  }
+  // lane (column s, half h) holds rows 8s + 4h + e, e < 4, in acc[4s + e]:
+  // candidates 4h + e of query q0 + s
+  if (r < 4 && q0 + r < nq) {
+    const long qi = q0 + r;
+    const double qnd = dot ? 0.0 : qn[qi];
+#pragma unroll
+    for (int s4 = 0; s4 < 4; s4++) {
+      if (s4 != r) continue;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int ci = 4 * h + e;
+        uint64_t xk = YK_KEY_NONE;
+        const uint64_t ck = ci < kp ? cand[qi * kp + ci] : YK_KEY_NONE;
+        if (ck != YK_KEY_NONE) {
+          const int bi = (int)(uint32_t)ck;
+          const float dv = acc[4 * s4 + e];
+          float v = dot ? dv : (float)(qnd + (double)bn[bi]) - 2.0f * dv;
+          // k = 1 follows nn.c:404-440: only values below 1e30 are taken
+          if (v == v && (k > 1 || v < 1e30f)) xk = yk_key(v, bi);
+        }
+        ex[r][ci] = xk;
+      }
+    }
+  }
+  __syncthreads();
+  if (lane >= 4 || q0 + lane >= nq) return;
+  const int s4 = lane;
+  const long qi = q0 + s4;
+  uint64_t t[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) t[i] = ex[s4][i];
+  // sort the 8 keys (insertion network)
+#pragma unroll
+  for (int i = 1; i < 8; i++)
+#pragma unroll
+    for (int j2 = i; j2 > 0; j2--) {
+      const uint64_t x = t[j2 - 1], y = t[j2];
+      t[j2 - 1] = x < y ? x : y;
+      t[j2] = x < y ? y : x;
+    }
+  const float none_d = k == 1 ? 1e30f : __uint_as_float(0xffffffffu);
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    if (i < k) {
+      out_i[qi * k + i] = t[i] == YK_KEY_NONE ? -1 : (int)(uint32_t)t[i];
+      out_d[qi * k + i] = t[i] == YK_KEY_NONE ? none_d : yk_fkey_inv((uint32_t)(t[i] >> 32));
+    }
+  const uint64_t last = cand[qi * kp + kp - 1];
+  uint64_t kth = YK_KEY_NONE;
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+    if (i == k - 1) kth = t[i];
+  bool ok;
+  if (last == YK_KEY_NONE || kp >= nb) {
+    ok = true;
+  } else if (kth == YK_KEY_NONE) {
+    ok = false;
+  } else {
+    const float Tb = yk_fkey_inv((uint32_t)(last >> 32));
+    const float Fk = yk_fkey_inv((uint32_t)(kth >> 32));
+    const float4 qs = qstats[qi], bs = *bmax;  // {|x|, |bf x|, |e|, |x|^2}
+    double ebf, e32;
+    if (dot) {
+      ebf = (double)qs.z * bs.x + (double)qs.y * bs.z + gam_bf * (double)qs.y * bs.y;
+      e32 = gam_32 * (double)qs.x * bs.x;
+    } else {
+      ebf = 2.0 * ((double)qs.z * bs.x + (double)qs.y * bs.z) +
+            gam_bf * (2.0 * qs.y * bs.y + qs.w + bs.w);
+      e32 = gam_32 * (2.0 * qs.x * bs.x + qs.w + bs.w);
+    }
+    ok = (double)Fk < (double)Tb - ebf - e32;
+  }
+  if (!ok) unc_list[atomicAdd(n_unc, 1)] = (int)qi;
+}
+
 // scatter FP32-fallback rows back: out[list[i]] = src[i]
 __global__ void yk_scatter_rows_kernel(const int *__restrict__ list, int n, int k,
                                        const int *__restrict__ si, const float *__restrict__ sd,
@@ -1102,11 +1219,33 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
       case 8: rk = yk_rerank_cert_kernel<8>; break;
       default: rk = yk_rerank_cert_kernel<0>; break;
     }
-    hipLaunchKernelGGL(rk, dim3((unsigned)n), dim3(64), 0, yk_stream(),
-                       q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, nb, ld, type,
-                       (const uint64_t *)vcand, kp, k, (const float4 *)qstats + q0,
-                       (const float4 *)bstats_max, gam_bf, gam_32, assign + q0 * k, dis + q0 * k,
-                       unc, nunc);
+    if (kp <= 8) {
+      void (*rk4)(const float *, const double *, const float *, const float *, long, int, int,
+                  const uint64_t *, long, int, int, const float4 *, const float4 *, float, float,
+                  int *, float *, int *, int *);
+      switch (ld >> 4) {
+        case 1: rk4 = yk_rerank4_kernel<1>; break;
+        case 2: rk4 = yk_rerank4_kernel<2>; break;
+        case 3: rk4 = yk_rerank4_kernel<3>; break;
+        case 4: rk4 = yk_rerank4_kernel<4>; break;
+        case 5: rk4 = yk_rerank4_kernel<5>; break;
+        case 6: rk4 = yk_rerank4_kernel<6>; break;
+        case 7: rk4 = yk_rerank4_kernel<7>; break;
+        case 8: rk4 = yk_rerank4_kernel<8>; break;
+        default: rk4 = yk_rerank4_kernel<0>; break;
+      }
+      hipLaunchKernelGGL(rk4, dim3((unsigned)((n + 3) / 4)), dim3(64), 0, yk_stream(),
+                         q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, nb, ld, type,
+                         (const uint64_t *)vcand, n, kp, k, (const float4 *)qstats + q0,
+                         (const float4 *)bstats_max, gam_bf, gam_32, assign + q0 * k,
+                         dis + q0 * k, unc, nunc);
+    } else {
+      hipLaunchKernelGGL(rk, dim3((unsigned)n), dim3(64), 0, yk_stream(),
+                         q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, nb, ld, type,
+                         (const uint64_t *)vcand, kp, k, (const float4 *)qstats + q0,
+                         (const float4 *)bstats_max, gam_bf, gam_32, assign + q0 * k,
+                         dis + q0 * k, unc, nunc);
+    }
     yk_prof_end(tok);
     YK_HIP(hipGetLastError());
     int hn = 0;
