@@ -544,10 +544,6 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
     // fragment c of the block starting at row bb: bfrag(bb) + c * 512
     auto bfrag = [&](long bb) { return a.b + ((bb >> 5) * nch * 64 + lane) * 8; };
     if (QRES) {
-      // two register buffers used in turn (the loop is unrolled by two so
-      // neither is copied): block bb is multiplied from one while the next
-      // block's fragments load into the other
-      bf16x8 bufa[QRES ? NCH : 1], bufb[QRES ? NCH : 1];
       // Buffer loads: the item's fragments through a descriptor based at its
       // first block, the block/chunk offset in a scalar register and the
       // lane's 16 bytes in one constant vector register -- no per-block
@@ -560,40 +556,45 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
       const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
           (void *)ibase, 0, (int)min(rbytes, 0x7fffffffL), YK_BUF_DWORD3);
       const int voff = lane * 16;
-      auto load = [&](bf16x8(&buf)[QRES ? NCH : 1], long bb) {
-        const int soff = (int)(((bb - bbeg) >> 5) * NCH * 1024);
+      auto soff_of = [&](long bb) { return (int)(((bb - bbeg) >> 5) * NCH * 1024); };
+      // One rolling register buffer: as soon as the MFMAs of chunk c of block
+      // bb have been issued, chunk c of the next block loads into the same
+      // registers, so every load has a whole block of MFMAs to arrive in and
+      // the buffer costs NCH registers rather than two blocks' worth (which
+      // is what lets three query tiles fit at two waves per SIMD).  The
+      // scheduling barriers keep each load after the MFMAs that read its
+      // registers.  The prefetch is unconditional (past the range it reloads
+      // the current block): a conditional load would join two paths with
+      // different outstanding-load counts, and the wait before the next MFMAs
+      // would then also wait for the loads just issued.
+      bf16x8 buf[QRES ? NCH : 1];
+      if (bbeg < bend) {
+        const int s0 = soff_of(bbeg);
 #pragma unroll
-        for (int c = 0; c < NCH; c++) {
-          const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff + c * 1024, 0);
-          buf[c] = __builtin_bit_cast(bf16x8, v);
-        }
-      };
-      auto step = [&](const bf16x8(&buf)[QRES ? NCH : 1], long bb, int sub) {
+        for (int c = 0; c < NCH; c++)
+          buf[c] = __builtin_bit_cast(
+              bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, s0 + c * 1024, 0));
+      }
+      int sub = 0;
+      for (long bb = bbeg; bb < bend; bb += a.bstep, sub ^= 1) {
+        const int sn = soff_of(bb + a.bstep < bend ? bb + a.bstep : bb);
         f32x16 acc[NB];
 #pragma unroll
         for (int j = 0; j < NB; j++)
 #pragma unroll
           for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
 #pragma unroll
-        for (int c = 0; c < NCH; c++)
+        for (int c = 0; c < NCH; c++) {
 #pragma unroll
           for (int j = 0; j < NB; j++)
             acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(buf[c], qreg[j][QRES ? c : 0], acc[j],
                                                              0, 0, 0);
+          __builtin_amdgcn_sched_barrier(0);
+          buf[c] = __builtin_bit_cast(
+              bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, sn + c * 1024, 0));
+          __builtin_amdgcn_sched_barrier(0);
+        }
         sel.block(a, acc, bb, bend, h, sub);
-      };
-      // The prefetches are unconditional (past the range they reload the
-      // current block): a conditional load would join two paths with
-      // different outstanding-load counts, and the wait before the next
-      // MFMAs would then also wait for the loads just issued.
-      if (bbeg < bend) load(bufa, bbeg);
-      for (long bb = bbeg; bb < bend; bb += 2 * a.bstep) {
-        const long b1 = bb + a.bstep, b2 = b1 + a.bstep;
-        load(bufb, b1 < bend ? b1 : bb);
-        step(bufa, bb, 0);
-        if (b1 >= bend) break;
-        load(bufa, b2 < bend ? b2 : b1);
-        step(bufb, b1, 1);
       }
     } else {
       for (long bb = bbeg; bb < bend; bb += a.bstep) {
@@ -1047,38 +1048,44 @@ static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, 
   return p;
 }
 
+// Query tiles (32 queries each) per wave: three where the kernel still fits
+// two waves per SIMD (256 VGPRs: query fragments 12 * nch, accumulators 48,
+// base buffer 4 * nch, selection state), else two.  Each base fragment
+// loaded then feeds three MFMAs instead of two.
+static int yk_bnb(int mode, int nch) { return mode != 2 && nch >= 1 && nch <= 9 ? 3 : 2; }
+
 // streaming kernel with exactly nch register-resident chunks (nch <= 16),
 // else the one streaming both operands
-template <int NB, int MODE, int R>
+template <int MODE, int R>
 static void (*yk_bscan_pick(int nch))(Bf16Args) {
+  constexpr int N3 = MODE != 2 ? 3 : 2;
   switch (nch) {
-    case 1: return yk_scan_bf16<NB, 1, MODE, R>;
-    case 2: return yk_scan_bf16<NB, 2, MODE, R>;
-    case 3: return yk_scan_bf16<NB, 3, MODE, R>;
-    case 4: return yk_scan_bf16<NB, 4, MODE, R>;
-    case 5: return yk_scan_bf16<NB, 5, MODE, R>;
-    case 6: return yk_scan_bf16<NB, 6, MODE, R>;
-    case 7: return yk_scan_bf16<NB, 7, MODE, R>;
-    case 8: return yk_scan_bf16<NB, 8, MODE, R>;
-    case 9: return yk_scan_bf16<NB, 9, MODE, R>;
-    case 10: return yk_scan_bf16<NB, 10, MODE, R>;
-    case 11: return yk_scan_bf16<NB, 11, MODE, R>;
-    case 12: return yk_scan_bf16<NB, 12, MODE, R>;
-    case 13: return yk_scan_bf16<NB, 13, MODE, R>;
-    case 14: return yk_scan_bf16<NB, 14, MODE, R>;
-    case 15: return yk_scan_bf16<NB, 15, MODE, R>;
-    case 16: return yk_scan_bf16<NB, 16, MODE, R>;
-    default: return yk_scan_bf16<NB, 0, MODE, R>;
+    case 1: return yk_scan_bf16<N3, 1, MODE, R>;
+    case 2: return yk_scan_bf16<N3, 2, MODE, R>;
+    case 3: return yk_scan_bf16<N3, 3, MODE, R>;
+    case 4: return yk_scan_bf16<N3, 4, MODE, R>;
+    case 5: return yk_scan_bf16<N3, 5, MODE, R>;
+    case 6: return yk_scan_bf16<N3, 6, MODE, R>;
+    case 7: return yk_scan_bf16<N3, 7, MODE, R>;
+    case 8: return yk_scan_bf16<N3, 8, MODE, R>;
+    case 9: return yk_scan_bf16<N3, 9, MODE, R>;
+    case 10: return yk_scan_bf16<2, 10, MODE, R>;
+    case 11: return yk_scan_bf16<2, 11, MODE, R>;
+    case 12: return yk_scan_bf16<2, 12, MODE, R>;
+    case 13: return yk_scan_bf16<2, 13, MODE, R>;
+    case 14: return yk_scan_bf16<2, 14, MODE, R>;
+    case 15: return yk_scan_bf16<2, 15, MODE, R>;
+    case 16: return yk_scan_bf16<2, 16, MODE, R>;
+    default: return yk_scan_bf16<2, 0, MODE, R>;
   }
 }
 
 template <int MODE, int R>
 static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
-  // 64 queries per wave (two 32-column MFMA tiles): with the base fragments
-  // prefetched in registers one block ahead this fits two waves per SIMD,
-  // which measured faster than 128 queries per wave at one wave per SIMD
-  const int nbq = 2;
-  void (*kern)(Bf16Args) = yk_bscan_pick<2, MODE, R>(a.ldh >> 4);
+  // 64 or 96 queries per wave at two waves per SIMD (measured faster than
+  // 128 queries per wave at one wave per SIMD)
+  const int nbq = yk_bnb(MODE, a.ldh >> 4);
+  void (*kern)(Bf16Args) = yk_bscan_pick<MODE, R>(a.ldh >> 4);
   a.brange = p.brange;
   a.bstep = p.bstep;
   a.nsplit = p.nsplit;
@@ -1139,7 +1146,7 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   if (stride < 1) stride = 1;
   const long nblocks = (a.nb + 31) / 32;
   while (stride > 1 && nblocks / stride < 2L * kp) stride >>= 1;
-  BPlan pa = yk_bplan(nq, a.nb, 2, stride, kp, kp > 256 ? kp : 256);
+  BPlan pa = yk_bplan(nq, a.nb, yk_bnb(1, a.ldh >> 4), stride, kp, kp > 256 ? kp : 256);
   YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * pa.nsplit * 4, &vpart));
   a.part = (uint64_t *)vpart;
   YK_TRY((yk_launch_bscan<1, 1>(pa, a)));
@@ -1150,7 +1157,7 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   hipLaunchKernelGGL(yk_bf_zero_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
                      yk_stream(), cnt, nq);
   hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), flag, 1L);
-  BPlan pb = yk_bplan(nq, a.nb, 2, 1, 1, 4096);
+  BPlan pb = yk_bplan(nq, a.nb, yk_bnb(3, a.ldh >> 4), 1, 1, 4096);
   a.thr = thr;
   a.cnt = cnt;
   a.cand = (uint64_t *)vcand;
