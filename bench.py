@@ -114,10 +114,10 @@ def bench_knn(ya, dev, args, base, queries, k, kp):
                                 None, oi.ptr, od.ptr))
 
     def step_bf16():
+        # the base's float norms come out of its BF16 conversion pass
         Q.compute_norms()
-        B.compute_norms()
         qh, qst, _ = Q.compute_bf16(2, True)
-        bh, _, bmx = B.compute_bf16(2, False)
+        bh, _, bmx = B.compute_bf16(2, False, with_norms=True)
         ya.check(lib.yk_knn_bf16(2, nq, nb, d, k, kp, Q.vec.ptr, Q.dnorm.ptr, qh.ptr, qst.ptr,
                                  B.vec.ptr, B.fnorm.ptr, bh.ptr, bmx.ptr, oi16.ptr, od16.ptr,
                                  ctypes.byref(nfb)))
@@ -295,9 +295,8 @@ def bench_precision_d960(ya, dev, nb=1_000_000, nq=10_000, d=960, k=10, kp=64, s
 
     def b16():
         Q.compute_norms()
-        B.compute_norms()
         qh, qst, _ = Q.compute_bf16(2, True)
-        bh, _, bmx = B.compute_bf16(2, False)
+        bh, _, bmx = B.compute_bf16(2, False, with_norms=True)
         ya.check(lib.yk_knn_bf16(2, nq, nb, d, k, kp, Q.vec.ptr, Q.dnorm.ptr, qh.ptr, qst.ptr,
                                  B.vec.ptr, B.fnorm.ptr, bh.ptr, bmx.ptr, oi16.ptr, od16.ptr,
                                  ctypes.byref(nfb)))

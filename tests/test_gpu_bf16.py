@@ -118,3 +118,82 @@ def test_k1_no_value_below_1e30(dev):
     B.free()
     assert np.array_equal(i16, i32) and np.array_equal(d16.view(np.int32), d32.view(np.int32))
     assert (i32[:, 0] == -1).any()
+
+
+def _bf16_rne(x):
+    """float32 -> bf16 bit patterns, round to nearest even (finite values)."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def _bf16_f(h):
+    return (h.astype(np.uint32) << 16).view(np.float32)
+
+
+def _norm_pieces(s2):
+    """three bf16 pieces of float(s2), each the rounding of what is left"""
+    nf = np.float32(s2)
+    h1 = _bf16_rne(np.array([nf]))[0]
+    r1 = np.float32(nf - _bf16_f(np.array([h1]))[0])
+    h2 = _bf16_rne(np.array([r1]))[0]
+    r2 = np.float32(r1 - _bf16_f(np.array([h2]))[0])
+    h3 = _bf16_rne(np.array([r2]))[0]
+    return [h1, h2, h3]
+
+
+@pytest.mark.parametrize("d,typ", [(20, 2), (128, 2), (200, 2), (960, 2), (128, 16)])
+def test_base_conversion_layout_stats_norms(dev, d, typ):
+    """The fragment-major BF16 base copy decoded in numpy: block b, chunk c is
+    the 1 KB fragment whose lane h*32 + r holds row 32b + r, columns 16c + 8h
+    .. +7; columns past d hold the norm pieces and ones (L2), then zeros; rows
+    past n are zero.  Statistics exact on integer data; the fused float norms
+    carry the same bits as yk_norms'."""
+    from yael_amd import lib
+    rng = np.random.default_rng(d + typ)
+    n = 70  # two full 32-row blocks and a partial one
+    x = rng.integers(0, 256, (n, d)).astype(np.float32)
+    S = dev.DeviceSet(x)
+    S.compute_norms()
+    ref_fn = S.fnorm.to_host(np.float32, (n,)).copy()
+    h, st, mx = S.compute_bf16(typ, False, with_norms=True)
+    fn = S.fnorm.to_host(np.float32, (n,))
+    assert np.array_equal(fn.view(np.int32), ref_fn.view(np.int32)), "fused float norms differ"
+    ldh = lib().yk_bf16_ld(d, typ)
+    nch, nblk = ldh // 16, (n + 31) // 32
+    raw = h.to_host(np.uint16, (nblk, nch, 2, 32, 8))
+    got = raw.transpose(0, 3, 1, 2, 4).reshape(nblk * 32, ldh)
+    want = np.zeros((nblk * 32, ldh), np.uint16)
+    want[:n, :d] = _bf16_rne(x)
+    s2 = (x.astype(np.float64) ** 2).sum(1)
+    if typ == 2:
+        for i in range(n):
+            want[i, d:d + 6] = _norm_pieces(s2[i]) + [0x3F80] * 3
+    assert np.array_equal(got, want), "BF16 base copy differs"
+    stats = st.to_host(np.float32, (n, 4))
+    exp = np.stack([np.sqrt(s2), np.sqrt(s2), np.zeros(n), s2], 1).astype(np.float32)
+    assert np.array_equal(stats, exp)
+    assert np.array_equal(mx.to_host(np.float32, (4,)), exp.max(0))
+    S.free()
+
+
+@pytest.mark.parametrize("d,typ", [(20, 2), (128, 2), (128, 16)])
+def test_query_conversion_layout(dev, d, typ):
+    """Queries are row-major BF16 rows of -2q (L2; q for dot product) with
+    three ones and the norm pieces after column d."""
+    from yael_amd import lib
+    rng = np.random.default_rng(7 * d + typ)
+    n = 45
+    x = rng.integers(0, 256, (n, d)).astype(np.float32)
+    S = dev.DeviceSet(x)
+    h, st, _ = S.compute_bf16(typ, True)
+    ldh = lib().yk_bf16_ld(d, typ)
+    got = h.to_host(np.uint16, (n, ldh))
+    want = np.zeros((n, ldh), np.uint16)
+    want[:, :d] = _bf16_rne(x * (np.float32(-2) if typ == 2 else np.float32(1)))
+    if typ == 2:
+        s2 = (x.astype(np.float64) ** 2).sum(1)
+        for i in range(n):
+            want[i, d:d + 6] = [0x3F80] * 3 + _norm_pieces(s2[i])
+    assert np.array_equal(got, want), "BF16 query copy differs"
+    S.free()

@@ -67,7 +67,7 @@ PROTOS = {
     "yk_knn_f32": (I, [I, L, L, I, I, vp, vp, vp, vp, vp, vp, vp]),
     "yk_cross_distances": (I, [I, L, L, I, vp, vp, vp, vp, vp, L]),
     "yk_bf16_ld": (I, [I, I]),
-    "yk_to_bf16": (I, [vp, L, I, I, I, vp, vp, vp]),
+    "yk_to_bf16": (I, [vp, L, I, I, I, vp, vp, vp, vp]),
     "yk_knn_bf16": (I, [I, L, L, I, I, I, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i32p]),
     "yk_pair_dist_f64": (I, [I, L, I, I, vp, vp, vp, vp]),
     "yk_kmeans_workspace_size": (C.c_size_t, [L, I, I]),

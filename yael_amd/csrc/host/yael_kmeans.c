@@ -160,10 +160,11 @@ static int yh_kmeans_core(int d, int n, int k, int niter, int flags, int verbose
   int rc;
   for (int iter = 1; iter <= niter; iter++) {
     (*iter_tot)++;
-    if ((rc = yk_norms(m->c, k, d, m->cn, NULL)) != YK_OK) return rc;
+    if (!m->bf16 && (rc = yk_norms(m->c, k, d, m->cn, NULL)) != YK_OK) return rc;
     if (m->bf16) {
       int nfb = 0;
-      if ((rc = yk_to_bf16(m->c, k, d, 2, 0, m->ch, m->cs, m->csmax)) != YK_OK) return rc;
+      /* the conversion also writes the centroids' float norms */
+      if ((rc = yk_to_bf16(m->c, k, d, 2, 0, m->ch, m->cs, m->csmax, m->cn)) != YK_OK) return rc;
       if ((rc = yk_knn_bf16(2, n, k, d, 1, YH_KM_KPRIME, m->v, m->qn, m->vh, m->vs, m->c, m->cn,
                             m->ch, m->csmax, m->assign, m->dis, &nfb)) != YK_OK)
         return rc;
@@ -236,7 +237,7 @@ int yh_kmeans(int d, int n, int k, int niter, const float *v, int v_on_device, i
   if (m.bf16) {
     /* the points' BF16 copy is made once per call */
     if ((rc = yh_km_alloc_bf16(&m, n, k, d)) != YK_OK) return rc;
-    if ((rc = yk_to_bf16(m.v, n, d, 2, 1, m.vh, m.vs, NULL)) != YK_OK) return rc;
+    if ((rc = yk_to_bf16(m.v, n, d, 2, 1, m.vh, m.vs, NULL, NULL)) != YK_OK) return rc;
   }
 
   int *hcounts = (int *)malloc(sizeof(int) * k);

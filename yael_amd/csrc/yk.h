@@ -73,7 +73,8 @@ int yk_knn_f32(int type, long nq, long nb, int d, int k, const float *q,
  * (row stride yk_bf16_ld(d, type) uint16; for L2 the squared norm is appended
  * in three BF16 pieces, queries are stored as -2q) and per-vector statistics
  * stats[4*i..] = {|x|, |bf(x)|, |x - bf(x)|, |x|^2}; stats_max (4 floats,
- * optional) receives their maxima over the set.
+ * optional) receives their maxima over the set; fnorm (base sets only,
+ * optional) the float norms exactly as yk_norms writes them.
  * yk_knn_bf16: BF16 MFMA first pass keeps the kp best candidates per query,
  * re-ranks them with the FP32 form's arithmetic (bit-identical values) and
  * certifies the result; uncertified queries are searched with the FP32 form
@@ -85,7 +86,7 @@ int yk_bf16_ld(int d, int type);
  * elements, lane h*32 + r holding row 32b + r, columns 16c + 8h .. +7) */
 long yk_bf16_bytes(long n, int d, int type, int is_query);
 int yk_to_bf16(const float *x, long n, int d, int type, int is_query, uint16_t *out,
-               float *stats, float *stats_max);
+               float *stats, float *stats_max, float *fnorm);
 int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, const float *q,
                 const double *qn, const uint16_t *qh, const float *qstats, const float *b,
                 const float *bn, const uint16_t *bh, const float *bstats_max, int *assign,

@@ -147,18 +147,25 @@ __global__ void yk_zero4_kernel(unsigned *p) {
 
 // The base copy (fragment-major): one wave per 32-row block; lane h*32 + r
 // owns row 32b + r, columns 16c + 8h .. +7 of every chunk c, so each chunk is
-// one coalesced 1 KB store.  Pass 1 sums the row statistics (the lane pair
-// combines its halves by a shuffle), pass 2 converts again (the rows are
-// cached) and stores, appending the norm pieces and ones after column d.
-// Conversion, norm pieces and statistics are those of yk_to_bf16_kernel.
+// one coalesced 1 KB store.  One read of the rows: the chunks holding only
+// columns below d are stored as they are converted; the chunk holding column
+// d (its data part kept in registers) and the chunks after it follow once the
+// row statistics are summed (the lane pair combines its halves by a
+// shuffle), with the norm pieces and ones after column d.  Conversion, norm
+// pieces and statistics are those of yk_to_bf16_kernel.  With fnorm, the
+// float norm of each row is summed in column order exactly as yk_norms does
+// (float squares, float sums): both lanes of a pair run the same sequence,
+// taking the other lane's 8 values of each chunk by a shuffle.
 __global__ __launch_bounds__(256) void yk_to_bf16_frag_kernel(const float *__restrict__ x, long n,
                                                               int d, int ld, int ldh, int type,
                                                               uint16_t *__restrict__ out,
                                                               float4 *__restrict__ stats,
-                                                              unsigned *smax) {
+                                                              unsigned *smax,
+                                                              float *__restrict__ fnorm) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int r = lane & 31, h = lane >> 5;
   const int nch = ldh >> 4;
+  const int cd = d >> 4;  // first chunk holding a column >= d
   const long nblk = (n + 31) >> 5;
   __shared__ float4 wmax[4];
   float4 mx = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -181,16 +188,43 @@ __global__ __launch_bounds__(256) void yk_to_bf16_frag_kernel(const float *__res
         for (int t = 0; t < 8; t++) v[t] = (valid && c0 + t < d) ? xr[c0 + t] : 0.f;
       }
     };
+    auto pack = [](const uint16_t (&e)[8]) {
+      return make_uint4((uint32_t)e[0] | ((uint32_t)e[1] << 16),
+                        (uint32_t)e[2] | ((uint32_t)e[3] << 16),
+                        (uint32_t)e[4] | ((uint32_t)e[5] << 16),
+                        (uint32_t)e[6] | ((uint32_t)e[7] << 16));
+    };
     double s2 = 0, sb = 0, se = 0;
-    for (int c = 0; c < nch; c++) {
+    float sf = 0.f;
+    uint16_t keep[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < nch && 16 * c < d; c++) {
       float v[8];
       vals(c, v);
+      uint16_t e[8];
 #pragma unroll
       for (int t = 0; t < 8; t++) {
-        const float vb = yk_bf16_f(yk_bf16_rne(v[t]));
+        e[t] = yk_bf16_rne(v[t]);  // zero past d
+        const float vb = yk_bf16_f(e[t]);
         s2 += (double)v[t] * v[t];
         sb += (double)vb * vb;
         se += ((double)v[t] - vb) * ((double)v[t] - vb);
+      }
+      if (fnorm) {
+        float o[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) o[t] = __shfl_xor(v[t], 32);
+        const int cmax = min(16, d - 16 * c);
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+          const float u = (t < 8) == (h == 0) ? v[t & 7] : o[t & 7];
+          if (t < cmax) sf = __fadd_rn(sf, __fmul_rn(u, u));
+        }
+      }
+      if (c < cd) {
+        ob[c * 64] = pack(e);
+      } else {
+#pragma unroll
+        for (int t = 0; t < 8; t++) keep[t] = e[t];
       }
     }
     s2 += __shfl_xor(s2, 32);
@@ -207,24 +241,22 @@ __global__ __launch_bounds__(256) void yk_to_bf16_frag_kernel(const float *__res
       const uint16_t h3 = yk_bf16_rne(r2);
       tail[0] = h1; tail[1] = h2; tail[2] = h3; tail[3] = one; tail[4] = one; tail[5] = one;
     }
-    for (int c = 0; c < nch; c++) {
+    for (int c = cd; c < nch; c++) {
       const int c0 = 16 * c + 8 * h;
-      float v[8];
-      vals(c, v);
       uint16_t e[8];
 #pragma unroll
       for (int t = 0; t < 8; t++) {
         const int col = c0 + t;
-        e[t] = col < d ? yk_bf16_rne(v[t]) : ((col - d) < 6 ? tail[col - d] : 0);
+        e[t] = col < d ? keep[t] : ((col - d) < 6 ? tail[col - d] : 0);
       }
-      ob[c * 64] = make_uint4((uint32_t)e[0] | ((uint32_t)e[1] << 16),
-                              (uint32_t)e[2] | ((uint32_t)e[3] << 16),
-                              (uint32_t)e[4] | ((uint32_t)e[5] << 16),
-                              (uint32_t)e[6] | ((uint32_t)e[7] << 16));
+      ob[c * 64] = pack(e);
     }
     if (valid) {
       const float4 st = make_float4((float)sqrt(s2), (float)sqrt(sb), (float)sqrt(se), (float)s2);
-      if (h == 0) stats[row] = st;
+      if (h == 0) {
+        stats[row] = st;
+        if (fnorm) fnorm[row] = sf;
+      }
       mx = make_float4(fmaxf(mx.x, st.x), fmaxf(mx.y, st.y), fmaxf(mx.z, st.z), fmaxf(mx.w, st.w));
     }
   }
@@ -248,10 +280,12 @@ __global__ __launch_bounds__(256) void yk_to_bf16_frag_kernel(const float *__res
 }
 
 extern "C" int yk_to_bf16(const float *x, long n, int d, int type, int is_query, uint16_t *out,
-                          float *stats, float *stats_max) {
+                          float *stats, float *stats_max, float *fnorm) {
   YK_TRY(yk_init());
   if (type == 12) type = YK_L2;
   if (n <= 0) return YK_OK;
+  if (is_query && fnorm)
+    return yk_set_error(YK_EINVAL, "yk_to_bf16: float norms are written for base sets only");
   const int ld = yk_ld(d), ldh = yk_bf16_ld(d, type);
   if (stats_max) hipLaunchKernelGGL(yk_zero4_kernel, dim3(1), dim3(64), 0, yk_stream(),
                                     (unsigned *)stats_max);
@@ -266,7 +300,7 @@ extern "C" int yk_to_bf16(const float *x, long n, int d, int type, int is_query,
     long blocks = ((n + 31) / 32 + 3) / 4;
     if (blocks > 512) blocks = 512;
     hipLaunchKernelGGL(yk_to_bf16_frag_kernel, dim3((unsigned)blocks), dim3(256), 0, yk_stream(),
-                       x, n, d, ld, ldh, type, out, (float4 *)stats, (unsigned *)stats_max);
+                       x, n, d, ld, ldh, type, out, (float4 *)stats, (unsigned *)stats_max, fnorm);
   }
   YK_HIP(hipGetLastError());
   return YK_OK;

@@ -76,8 +76,9 @@ class DeviceSet:
     def compute_norms(self):
         check(lib().yk_norms(self.vec.ptr, self.n, self.d, self.fnorm.ptr, self.dnorm.ptr), "norms")
 
-    def compute_bf16(self, typ, is_query):
-        """BF16 copy (+ error statistics) for the precision form."""
+    def compute_bf16(self, typ, is_query, with_norms=False):
+        """BF16 copy (+ error statistics) for the precision form; with_norms
+        (base sets) also writes the float norms, as compute_norms does."""
         key = (typ, bool(is_query))
         if key not in self.bf:
             nbytes = lib().yk_bf16_bytes(self.n, self.d, typ, int(is_query))
@@ -86,7 +87,8 @@ class DeviceSet:
         # the set maxima are needed for the base only (the certificate bounds
         # use each query's own statistics)
         check(lib().yk_to_bf16(self.vec.ptr, self.n, self.d, typ, int(is_query), h.ptr, st.ptr,
-                               None if is_query else mx.ptr), "to_bf16")
+                               None if is_query else mx.ptr,
+                               self.fnorm.ptr if with_norms else None), "to_bf16")
         return h, st, mx
 
     def free(self):
@@ -115,9 +117,10 @@ def knn_bf16(queries, base, k, kprime, typ=2, prepare=True):
     L = lib()
     if prepare:
         queries.compute_norms()
-        base.compute_norms()
     qh, qst, _ = queries.compute_bf16(typ, True) if prepare else queries.bf[(typ, True)]
-    bh, _, bmx = base.compute_bf16(typ, False) if prepare else base.bf[(typ, False)]
+    # the base's float norms come out of its conversion pass
+    bh, _, bmx = (base.compute_bf16(typ, False, with_norms=True) if prepare
+                  else base.bf[(typ, False)])
     oi, od = DeviceBuffer(queries.n * k * 4), DeviceBuffer(queries.n * k * 4)
     nfb = ctypes.c_int(0)
     check(L.yk_knn_bf16(typ, queries.n, base.n, base.d, k, kprime, queries.vec.ptr,
