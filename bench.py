@@ -164,6 +164,48 @@ def bench_kmeans(ya, dev, v, k, niter, seed):
     return dict(sec=t1 - t0, iters=niter, obj=float(ret.value))
 
 
+def bench_kmeanspp(ya, dev, v, k, seed, cpu):
+    """k-means++ seeding: one k-means step after k-means++ seeding minus one
+    step after random seeding, device-resident; with cpu, the same two runs
+    of the compiled reference on the host (OpenMP distance steps)."""
+    lib = ya.lib()
+    V = dev.DeviceSet(v)
+    n, d = v.shape
+    cent = np.empty((k, d), np.float32)
+    asg = np.empty(n, np.int32)
+    ret = ctypes.c_float()
+
+    def run(flags):
+        lib.yk_sync()
+        t0 = time.perf_counter()
+        ya.check(lib.yael_kmeans_device(d, n, k, 1, V.vec.ptr, flags, seed, 1,
+                                        cent.ctypes.data_as(ctypes.c_void_p), None,
+                                        asg.ctypes.data_as(ctypes.c_void_p), None,
+                                        ctypes.byref(ret)))
+        lib.yk_sync()
+        return time.perf_counter() - t0
+    run(0x10000 | 0x20000)
+    t_rand = min(run(0x10000) for _ in range(2))
+    t_kpp = min(run(0x10000 | 0x20000) for _ in range(2))
+    V.free()
+    rec = dict(init_s=t_kpp - t_rand, obj=float(ret.value))
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bind import Reference, have_ref
+        if have_ref():
+            cores = min(len(os.sched_getaffinity(0)), 16)
+            ref = Reference()
+            t0 = time.perf_counter()
+            ref.kmeans(v, k, 1, flags=0x10000 | cores, seed=seed)
+            t1 = time.perf_counter()
+            r, _, _, a_ref, _ = ref.kmeans(v, k, 1, flags=0x10000 | 0x20000 | cores, seed=seed)
+            t2 = time.perf_counter()
+            rec.update(cpu_reference_init_s=(t2 - t1) - (t1 - t0), cpu_threads=cores,
+                       same_result_as_reference=bool(np.float32(r) == np.float32(rec["obj"])
+                                                     and np.array_equal(a_ref, asg)))
+    return rec
+
+
 def bench_kmeans_large(ya, dev, n=10_000_000, d=128, k=65536, niter=20, seed=1, nsub=8,
                        ksub=256):
     """n=1e7, d=128, k=65,536 k-means with the BF16 assign form (BF16 first
@@ -537,6 +579,14 @@ def main():
                                           "device-resident",
                          "kmeans_obj": km["obj"]}
         print("# kmeans: %.2f iter/s" % (km["iters"] / km["sec"]), file=sys.stderr, flush=True)
+        kpp = bench_kmeanspp(ya, dev, v, 1024, 1, cpu=not args.no_cpu)
+        out["extras"]["kmeanspp"] = dict(
+            config="k-means++ seeding (KMEANS_INIT_BERKELEY), n=100000 d=128 k=1024: draws "
+                   "among the first 8k = 8192 points; one k-means step with it minus one with "
+                   "random seeding, device-resident", **kpp)
+        print("# kmeans++ seeding: %.1f ms (%s)" % (kpp["init_s"] * 1e3, "cpu reference %.1f ms"
+              % (kpp["cpu_reference_init_s"] * 1e3) if "cpu_reference_init_s" in kpp else "no cpu"),
+              file=sys.stderr, flush=True)
         bt = bench_batched(ya, dev)
         out["extras"]["batched"] = {
             "config": "256 k-means instances (hkm level 2), Zipf(1.1) sizes sum 2e6 in [256, "

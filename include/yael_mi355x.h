@@ -26,7 +26,7 @@ extern "C" {
 
 /* kmeans flags, kmeans.h:11-18 */
 #define KMEANS_QUIET 0x10000
-#define KMEANS_INIT_BERKELEY 0x20000 /* not served on the GPU: error */
+#define KMEANS_INIT_BERKELEY 0x20000 /* k-means++ (kmeans.c:27-82) */
 #define KMEANS_NORMALIZE_CENTS 0x40000
 #define KMEANS_INIT_RANDOM 0x80000
 #define KMEANS_INIT_USER 0x100000

@@ -18,7 +18,9 @@
  *   - centroid update as float sums in point order scaled by a double
  *     reciprocal (kmeans.c:278-288);
  *   - empty-cluster splitting driven by glibc rand_r (kmeans.c:166-209,
- *     vector.c:135-189).
+ *     vector.c:135-189);
+ *   - k-means++ seeding with the reference's double-precision L1
+ *     normalisation and prefix walk (kmeans.c:27-82, 393-400).
  *
  * The GEMM is pluggable: by default a plain loop (float accumulation in
  * ascending k, no fused multiply-add; this file is built with
@@ -624,7 +626,36 @@ static int oy_kmeans_core(int d, int n, int k, int niter, int nt, int flags,
   return 0;
 }
 
-/* kmeans.c:332-447 (random or user init; kmeans++ is not restated) */
+/* kmeans.c:27-82: k-means++ seeding among the first n points.  The first
+ * centre is rand_r % k (as the reference draws it); each next one keeps the
+ * running minimum of the squared distances to the centres chosen so far
+ * (nn.c:132-154 against the newest centre), scales a copy to unit L1 norm in
+ * place (vector.c:2016-2026, 1792-1797) and walks a draw rd in [0, 1) down it
+ * in double: the first j with rd < 0, else n - 1. */
+void oy_kmeanspp_init(int d, int n, int k, const float *v, int *sel, unsigned seed) {
+  float *best = (float *)malloc(sizeof(float) * (size_t)n);
+  float *tmp = (float *)malloc(sizeof(float) * (size_t)n);
+  for (long j = 0; j < n; j++) best[j] = HUGE_VAL;
+  sel[0] = rand_r(&seed) % k;
+  for (int i = 1; i < k; i++) {
+    oy_distances_1_nonpacked(d, n, v + (long)d * sel[i - 1], v, d, tmp);
+    for (long j = 0; j < n; j++)
+      if (tmp[j] < best[j]) best[j] = tmp[j];
+    memcpy(tmp, best, sizeof(float) * (size_t)n);
+    oy_fvec_normalize(tmp, n, 1);
+    double rd = oy_drand_r(&seed);
+    long j;
+    for (j = 0; j < n - 1; j++) {
+      rd -= tmp[j];
+      if (rd < 0) break;
+    }
+    sel[i] = (int)j;
+  }
+  free(best);
+  free(tmp);
+}
+
+/* kmeans.c:332-447 (random, k-means++ or user init) */
 float oy_kmeans_traced(int d, int n, int k, int niter, const float *v, int flags,
                        long seed_in, int redo, float *cent_out, float *dis_out,
                        int *assign_out, int *nassign_out, oy_kmeans_trace *tr) {
@@ -636,7 +667,7 @@ float oy_kmeans_traced(int d, int n, int k, int niter, const float *v, int flags
     assert(cent_out);
     redo = 1;
   }
-  if (flags & (OY_INIT_BERKELEY | OY_L1 | OY_CHI2)) {
+  if (flags & (OY_L1 | OY_CHI2)) {
     fprintf(stderr, "oracle: kmeans flag not restated\n");
     abort();
   }
@@ -655,7 +686,13 @@ float oy_kmeans_traced(int d, int n, int k, int niter, const float *v, int flags
     if (user) {
       memcpy(cent, cent_out, sizeof(float) * (size_t)k * d);
     } else {
-      oy_random_init(n, k, sel, (unsigned)rand_r(&seed));
+      if (flags & OY_INIT_BERKELEY) {
+        /* kmeans.c:393-400: at most 8k points when n > 8k and n > 8192 */
+        const int nsub = (n > k * 8 && n > 8192) ? k * 8 : n;
+        oy_kmeanspp_init(d, nsub, k, v, sel, (unsigned)rand_r(&seed));
+      } else {
+        oy_random_init(n, k, sel, (unsigned)rand_r(&seed));
+      }
       for (long i = 0; i < k; i++)
         memcpy(cent + i * d, v + (long)sel[i] * d, sizeof(float) * d);
     }

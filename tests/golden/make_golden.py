@@ -4,7 +4,7 @@ Run in a tree where oracle/_ref/libyael_ref.so was built from yael's own
 sources (oracle/Makefile, target 'ref').  Every fixture stores its inputs
 and the reference's outputs; nothing of the reference's code is kept.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [fixture names]
 """
 import ctypes
 import os
@@ -19,6 +19,7 @@ import numpy as np  # noqa: E402
 from oracle_bind import Reference, have_ref  # noqa: E402
 
 QUIET = 0x10000
+BERKELEY = 0x20000
 NORMALIZE = 0x40000
 INIT_USER = 0x100000
 
@@ -127,6 +128,28 @@ def kmeans_cases(ref, rng):
     return out
 
 
+def kpp_cases(ref):
+    """k-means++ seeding (own generator, so the other fixtures keep their
+    inputs): a float-data run with redo, and one step on integer data from
+    n <= 8192 points and from a subset of 8k points (n > 8k and n > 8192)."""
+    rng = np.random.default_rng(20261203)
+    out = {}
+    v = rng.standard_normal((2000, 16)).astype(np.float32)
+    r, c, dis, asg, nas = ref.kmeans(v, 20, 10, flags=QUIET | BERKELEY, seed=5, redo=2)
+    out["km_kpp"] = dict(v=v, k=np.int32(20), niter=np.int32(10), flags=np.int32(QUIET | BERKELEY),
+                         seed=np.int64(5), redo=np.int32(2), ret=np.float32(r), cent=c, dis=dis,
+                         assign=asg, nassign=nas)
+    cen = rng.uniform(0, 255, (50, 32))
+    for name, n, k, seed in (("km_kpp_step", 3000, 30, 17), ("km_kpp_sub_step", 20000, 40, 19)):
+        vi = np.clip(np.rint(cen[rng.integers(0, 50, n)] + 20 * rng.standard_normal((n, 32))), 0, 255)
+        vi = vi.astype(np.float32)
+        r, c, dis, asg, nas = ref.kmeans(vi, k, 1, flags=QUIET | BERKELEY, seed=seed)
+        out[name] = dict(v=vi, k=np.int32(k), niter=np.int32(1), flags=np.int32(QUIET | BERKELEY),
+                         seed=np.int64(seed), redo=np.int32(1), ret=np.float32(r), cent=c, dis=dis,
+                         assign=asg, nassign=nas)
+    return out
+
+
 def rng_cases(ref):
     lib = ref.lib
     out = {}
@@ -160,9 +183,15 @@ def main():
     cases.update(kmeans_cases(ref, rng))
     cases.update(rng_cases(ref))
     cases.update(shortlist_cases(ref, rng))
+    cases.update(kpp_cases(ref))
+    only = set(sys.argv[1:])  # fixture names to (re)write; none: all
+    wrote = 0
     for name, arrs in cases.items():
+        if only and name not in only:
+            continue
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrs)
-    print("wrote %d fixtures" % len(cases))
+        wrote += 1
+    print("wrote %d fixtures" % wrote)
 
 
 if __name__ == "__main__":

@@ -216,3 +216,28 @@ def test_bf16_assign_same_run_as_fp32(ya, integer):
     assert np.array_equal(a[2].view(np.int32), b[2].view(np.int32))
     assert a[1] == b[1]
     assert st16[2] < st16[1]
+
+
+@pytest.mark.parametrize("case", ["km_kpp_step", "km_kpp_sub_step"])
+def test_kmeanspp_step_bitexact_golden(ya, case):
+    """k-means++ seeding (kmeans.c:27-82; the second case seeds from the
+    first 8k points) then one step, on integer data: the reference's bits."""
+    gl = load(case)
+    print("golden", case, flush=True)
+    c, q, dis, asg, nas = ya.kmeans(gl["v"], int(gl["k"]), niter=1, seed=int(gl["seed"]),
+                                    verbose=False, init="kmeans++", output="all")
+    assert np.array_equal(asg, gl["assign"]) and np.array_equal(nas, gl["nassign"])
+    assert np.array_equal(dis.view(np.int32), gl["dis"].view(np.int32))
+    assert np.array_equal(c.view(np.int32), gl["cent"].view(np.int32))
+    assert np.float32(q) == gl["ret"]
+
+
+def test_kmeanspp_float_runs_golden(ya):
+    """Float data, two runs: the same seeds are drawn (the draws sit far from
+    the rounding of the distances), the trajectories may part at near-ties."""
+    gl = load("km_kpp")
+    c, q, dis, asg, nas = ya.kmeans(gl["v"], int(gl["k"]), niter=int(gl["niter"]),
+                                    seed=int(gl["seed"]), redo=int(gl["redo"]), verbose=False,
+                                    init="kmeans++", output="all")
+    assert abs(q - float(gl["ret"])) <= 2e-3 * abs(float(gl["ret"])), (q, gl["ret"])
+    assert (asg == gl["assign"]).mean() > 0.98

@@ -95,7 +95,8 @@ def test_cross_golden_plain_gemm(oracle, case):
     assert np.all(np.abs(out.astype(np.float64) - g["dist"]) <= bound)
 
 
-KM_CASES = ["km_plain", "km_redo", "km_norm", "km_split", "km_step", "km_overflow", "km_split_step"]
+KM_CASES = ["km_plain", "km_redo", "km_norm", "km_split", "km_step", "km_overflow", "km_split_step",
+            "km_kpp", "km_kpp_step", "km_kpp_sub_step"]
 
 
 @pytest.mark.parametrize("case", KM_CASES)
@@ -109,6 +110,17 @@ def test_kmeans_golden_bitexact_with_reference_blas(oracle_blas, case):
     assert np.array_equal(nas, g["nassign"])
     assert np.array_equal(bits(c), bits(g["cent"]))
     assert np.array_equal(bits(dis), bits(g["dis"]))
+
+
+@pytest.mark.parametrize("case", ["km_kpp_step", "km_kpp_sub_step"])
+def test_kmeanspp_integer_data_plain_gemm(oracle, case):
+    """On integer-valued data every k-means++ distance is exact, so the
+    restatement's own GEMV loop gives the reference's seeds and first step."""
+    g = load(case)
+    r, c, dis, asg, nas = oracle.kmeans(g["v"], int(g["k"]), int(g["niter"]), flags=int(g["flags"]),
+                                        seed=int(g["seed"]))
+    assert np.float32(r) == g["ret"]
+    assert np.array_equal(asg, g["assign"]) and np.array_equal(bits(c), bits(g["cent"]))
 
 
 def test_kmeans_golden_abandon(oracle_blas):

@@ -59,6 +59,9 @@ PROTOS = {
     "yk_upload_vectors": (I, [vp, vp, L, I]),
     "yk_download_vectors": (I, [vp, vp, L, I]),
     "yk_memcpy_h2d": (I, [vp, vp, C.c_size_t]),
+    "yk_host_alloc": (I, [C.POINTER(vp), C.c_size_t]),
+    "yk_host_free": (I, [vp]),
+    "yk_kpp_update": (I, [vp, L, I, vp, L, vp, I, vp]),
     "yk_memcpy_d2h": (I, [vp, vp, C.c_size_t]),
     "yk_memcpy_d2d": (I, [vp, vp, C.c_size_t]),
     "yk_memset": (I, [vp, I, C.c_size_t]),

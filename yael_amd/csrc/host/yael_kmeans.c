@@ -105,6 +105,47 @@ void yh_random_init(int n, int k, int *sel, unsigned seed) {
   free(idx);
 }
 
+/* k-means++ seeding (kmeans.c:27-82) among the first nsub points of the
+ * device set v (double norms vn).  Per new centre the GPU updates every
+ * point's distance to its nearest chosen centre (yk_kpp_update); the two
+ * passes that turn those distances into the next draw stay on the host, in
+ * the reference's order: the L1 normalisation (a sequential double sum, then
+ * each value scaled by the double reciprocal, vector.c:2016-2026, 1792-1797)
+ * and the walk of rd down the normalised values -- their rounding depends on
+ * that order, which a parallel reduction or scan would not keep. */
+static int yh_kmeanspp_init(const float *v, const double *vn, int d, int nsub, int k, int *sel,
+                            unsigned seed) {
+  float *best = NULL, *p = NULL;
+  int rc = yk_host_alloc((void **)&p, sizeof(float) * (size_t)nsub);
+  if (rc != YK_OK) return rc;
+  if ((rc = yk_malloc((void **)&best, sizeof(float) * (size_t)nsub)) != YK_OK) {
+    yk_host_free(p);
+    return rc;
+  }
+  sel[0] = rand_r(&seed) % k;
+  for (int i = 1; i < k && rc == YK_OK; i++) {
+    /* the step kernel also writes the distances into the page-locked p */
+    rc = yk_kpp_update(v, nsub, d, vn, sel[i - 1], best, i == 1, p);
+    if (rc == YK_OK) rc = yk_sync();
+    if (rc != YK_OK) break;
+    double nr = 0;
+    for (long j = 0; j < nsub; j++) nr += fabs(p[j]);
+    const double scal = 1. / nr;
+    double rd = rand_r(&seed) / ((double)RAND_MAX + 1.0);
+    long j;
+    /* each value scaled as the normalisation stores it, only as far as the
+     * walk goes */
+    for (j = 0; j < nsub - 1; j++) {
+      rd -= (float)(p[j] * scal);
+      if (rd < 0) break;
+    }
+    sel[i] = (int)j;
+  }
+  yk_free(best);
+  yk_host_free(p);
+  return rc;
+}
+
 typedef struct {
   float *v, *c, *cn, *dis;
   double *qn, *obj;
@@ -215,8 +256,6 @@ int yh_kmeans(int d, int n, int k, int niter, const float *v, int v_on_device, i
   const int user = (flags & KMEANS_INIT_USER) != 0;
   if (flags & (KMEANS_L1 | KMEANS_CHI2))
     return yk_set_error_c(YK_EINVAL, "KMEANS_L1/KMEANS_CHI2 have no GPU path");
-  if (flags & KMEANS_INIT_BERKELEY)
-    return yk_set_error_c(YK_EINVAL, "KMEANS_INIT_BERKELEY (k-means++) has no GPU path");
   if (user && !cent_out) return yk_set_error_c(YK_EINVAL, "KMEANS_INIT_USER needs centroids");
   if (user) redo = 1;
   if (k > n || k < 1 || d < 1)
@@ -258,7 +297,14 @@ int yh_kmeans(int d, int n, int k, int niter, const float *v, int v_on_device, i
       if ((rc = yk_upload_vectors(m.c, cent_out, k, d)) != YK_OK) break;
     } else {
       /* the selected points are copied row by row on the device */
-      yh_random_init(n, k, sel, (unsigned)rand_r(&seed));
+      if (flags & KMEANS_INIT_BERKELEY) {
+        /* kmeans.c:393-400: at most 8k points when n > 8k and n > 8192 */
+        const int nsub = (n > k * 8 && n > 8192) ? k * 8 : n;
+        if ((rc = yh_kmeanspp_init(m.v, m.qn, d, nsub, k, sel, (unsigned)rand_r(&seed))) != YK_OK)
+          break;
+      } else {
+        yh_random_init(n, k, sel, (unsigned)rand_r(&seed));
+      }
       if ((rc = yk_memcpy_h2d(m.sel, sel, sizeof(int) * k)) != YK_OK) break;
       if ((rc = yk_gather_rows(m.v, m.sel, k, d, m.c)) != YK_OK) break;
     }

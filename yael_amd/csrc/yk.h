@@ -46,6 +46,10 @@ int yk_malloc(void **p, size_t bytes);
 int yk_free(void *p);
 int yk_upload_vectors(float *dst, const float *src, long n, int d);  /* host n x d -> padded */
 int yk_download_vectors(float *dst, const float *src, long n, int d); /* padded -> host n x d */
+/* page-locked, coherent, mapped host memory (fast device-to-host copies;
+ * kernels may write it directly) */
+int yk_host_alloc(void **p, size_t bytes);
+int yk_host_free(void *p);
 int yk_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int yk_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int yk_memcpy_d2d(void *dst, const void *src, size_t bytes);
@@ -125,6 +129,12 @@ size_t yk_kmeans_workspace_size(long n, int k, int d);
 int yk_kmeans_update(long n, int k, int d, const float *v, const int *assign,
                      float *cent, int *counts, int normalize, void *ws);
 int yk_sum_f64(const float *x, long n, double *out_host, void *ws);
+/* k-means++ step: best[j] = min(best[j], squared distance of point j to
+ * point s) for j < n (first: best starts at HUGE_VAL); vn: the points'
+ * double norms (yk_norms out_d); host_copy (optional, from yk_host_alloc)
+ * receives best as well, valid after yk_sync */
+int yk_kpp_update(const float *v, long n, int d, const double *vn, long s, float *best, int first,
+                  float *host_copy);
 /* dst row i = src row idx[i] (padded rows; idx is a device array) */
 int yk_gather_rows(const float *src, const int *idx, long n, int d, float *dst);
 

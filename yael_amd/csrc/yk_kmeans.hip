@@ -265,6 +265,41 @@ extern "C" int yk_kmeans_update(long n, int k, int d, const float *v, const int 
 double *yk_ws_obj(void *ws, long n, int k) { return yk_ws_layout(ws, n, k).obj; }
 
 // dst[i] = src[idx[i]] for padded rows (initial centroids from selected points)
+// k-means++ step (kmeans.c:51-62 with nn.c:132-154): the squared distance
+// of every point j < n to point s -- float(||v_j||^2 + ||v_s||^2) from the
+// double norms (double sums of float squares), then the cross term added
+// as -2 * (float dot product) -- and the running minimum kept in best
+// (first: best starts at HUGE_VAL; a NaN distance never replaces it).
+__global__ __launch_bounds__(256) void yk_kpp_kernel(const float *__restrict__ v, int ld, int d,
+                                                     const double *__restrict__ vn, long n, long s,
+                                                     float *__restrict__ best, int first,
+                                                     float *__restrict__ host_copy) {
+  const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const float *a = v + s * ld, *b = v + j * ld;
+  float dot = 0.f;
+  for (int l = 0; l < d; l++) dot = dot + b[l] * a[l];
+  float dist = (float)(vn[j] + vn[s]);
+  dist = dist + -2.0f * dot;
+  float cur = first ? HUGE_VALF : best[j];
+  if (dist < cur) cur = dist;
+  best[j] = cur;
+  if (host_copy) host_copy[j] = cur;
+}
+
+extern "C" int yk_kpp_update(const float *v, long n, int d, const double *vn, long s, float *best,
+                             int first, float *host_copy) {
+  YK_TRY(yk_init());
+  if (n <= 0) return YK_OK;
+  if (s < 0 || s >= n) return yk_set_error(YK_EINVAL, "yk_kpp_update: centre %ld of %ld", s, n);
+  float *hdev = nullptr;
+  if (host_copy) YK_HIP(hipHostGetDevicePointer((void **)&hdev, host_copy, 0));
+  hipLaunchKernelGGL(yk_kpp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, yk_stream(),
+                     v, yk_ld(d), d, vn, n, s, best, first, hdev);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
 __global__ void yk_gather_kernel(const float *__restrict__ src, const int *__restrict__ idx,
                                  long n, int ld, float *__restrict__ dst) {
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;

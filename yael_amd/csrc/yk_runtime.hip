@@ -92,6 +92,23 @@ extern "C" int yk_memcpy_h2d(void *dst, const void *src, size_t bytes) {
   return YK_OK;
 }
 
+extern "C" int yk_host_alloc(void **p, size_t bytes) {
+  YK_TRY(yk_init());
+  *p = nullptr;
+  // coherent and mapped: kernels may also write it directly
+  if (hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocCoherent | hipHostMallocMapped) !=
+      hipSuccess) {
+    (void)hipGetLastError();
+    return yk_set_error(YK_ENOMEM, "hipHostMalloc(%zu) failed", bytes);
+  }
+  return YK_OK;
+}
+
+extern "C" int yk_host_free(void *p) {
+  if (p) YK_HIP(hipHostFree(p));
+  return YK_OK;
+}
+
 extern "C" int yk_memcpy_d2h(void *dst, const void *src, size_t bytes) {
   YK_TRY(yk_init());
   if (!bytes) return YK_OK;
