@@ -206,6 +206,60 @@ def bench_kmeanspp(ya, dev, v, k, seed, cpu):
     return rec
 
 
+def bench_hkm(ya, base, cpu, n=200_000, nlevel=2, bf=32, niter=10, nq_cpu=20_000):
+    """Hierarchical k-means: hkm_learn on n mixture points (every level one
+    batched k-means) and hkm_quantize of the whole base; with cpu, the
+    reference's own hkm_learn on the same points and hkm_quantize on a
+    sample, on the host (YAEL_COUNT_CPU threads)."""
+    from yael_amd import hkm
+    libc = ctypes.CDLL(None)
+    v = np.ascontiguousarray(base[:n])
+    libc.srand48(ctypes.c_long(1))
+    t, _ = hkm.hkm_learn(v[:20000], 2, 4, niter=2)  # warm-up (builds, caches)
+    libc.srand48(ctypes.c_long(1))
+    t0 = time.perf_counter()
+    t, labels = hkm.hkm_learn(v, nlevel, bf, niter=niter)
+    t1 = time.perf_counter()
+    idx = t.quantize(base)
+    t2 = time.perf_counter()
+    idx = t.quantize(base)
+    t3 = time.perf_counter()
+    rec = dict(config="hkm_learn n=%d d=%d nlevel=%d bf=%d (%d leaves) niter=%d, mixture data; "
+                      "hkm_quantize of %d points" % (n, v.shape[1], nlevel, bf, bf ** nlevel, niter,
+                                                     len(base)),
+               learn_s=t1 - t0, quantize_points_per_s=len(base) / (t3 - t2))
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bind import Reference, have_ref, make_hkm
+        if have_ref():
+            cores = min(len(os.sched_getaffinity(0)), 16)
+            os.environ["YAEL_COUNT_CPU"] = str(cores)
+            ref = Reference()
+            libc.srand48(ctypes.c_long(1))
+            t0 = time.perf_counter()
+            lv, lab_ref, h = ref.hkm_learn(v, nlevel, bf, niter)
+            t1 = time.perf_counter()
+            q = np.ascontiguousarray(base[:nq_cpu])
+            t2 = time.perf_counter()
+            qi_ref = ref.hkm_quantize(h, q)
+            t3 = time.perf_counter()
+            def qerr(leaves, lab):
+                # mean squared distance of the points to their leaf centroid (FP64)
+                c = leaves[lab].astype(np.float64)
+                return float(((v.astype(np.float64) - c) ** 2).sum(1).mean())
+            # leaf-level agreement is low once a level-0 near-tie moves one
+            # point: the children's random seeding is by position in the node
+            rec.update(cpu_reference_learn_s=t1 - t0, cpu_threads=cores,
+                       cpu_reference_quantize_points_per_s=nq_cpu / (t3 - t2),
+                       cpu_quantize_sample=nq_cpu,
+                       level0_agreement_with_reference=float(
+                           (lab_ref // bf ** (nlevel - 1) == labels // bf ** (nlevel - 1)).mean()),
+                       leaf_agreement_with_reference=float((lab_ref == labels).mean()),
+                       mean_sq_dist_to_leaf=qerr(t.centroids(nlevel - 1), labels),
+                       reference_mean_sq_dist_to_leaf=qerr(lv[nlevel - 1], lab_ref))
+    return rec
+
+
 def bench_kmeans_large(ya, dev, n=10_000_000, d=128, k=65536, niter=20, seed=1, nsub=8,
                        ksub=256):
     """n=1e7, d=128, k=65,536 k-means with the BF16 assign form (BF16 first
@@ -587,6 +641,13 @@ def main():
         print("# kmeans++ seeding: %.1f ms (%s)" % (kpp["init_s"] * 1e3, "cpu reference %.1f ms"
               % (kpp["cpu_reference_init_s"] * 1e3) if "cpu_reference_init_s" in kpp else "no cpu"),
               file=sys.stderr, flush=True)
+        hk = bench_hkm(ya, base, cpu=not args.no_cpu)
+        out["extras"]["hkm"] = hk
+        print("# hkm: learn %.3f s, quantize %.3g points/s%s" % (
+            hk["learn_s"], hk["quantize_points_per_s"],
+            (" (cpu reference: learn %.2f s, quantize %.3g points/s)"
+             % (hk["cpu_reference_learn_s"], hk["cpu_reference_quantize_points_per_s"]))
+            if "cpu_reference_learn_s" in hk else ""), file=sys.stderr, flush=True)
         bt = bench_batched(ya, dev)
         out["extras"]["batched"] = {
             "config": "256 k-means instances (hkm level 2), Zipf(1.1) sizes sum 2e6 in [256, "

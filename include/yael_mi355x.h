@@ -92,7 +92,38 @@ void compute_cross_distances_alt_nonpacked(int distance_type, int d, int na, int
 void compute_cross_distances_alt_thread(int distance_type, int d, int na, int nb,
                                         const float *a, const float *b, float *dist2, int nt);
 
+/* hierarchical k-means: the tree, same layout as hkm.h:10-16.  Level l
+ * holds bf^(l+1) centroids (d floats each); the children of node p of
+ * level l are its rows p*bf .. p*bf + bf - 1. */
+typedef struct hkm_s {
+  int nlevel;        /* number of levels */
+  int bf;            /* branching factor */
+  int k;             /* number of leaves, bf^nlevel */
+  int d;             /* dimension of the vectors */
+  float **centroids; /* nlevel arrays */
+} hkm_t;
+
+/* replaces hkm.h:21-23 (hkm.c:36-116); every level is one batched k-means
+ * on the GPU (node seeds drawn from lrand48() in node order, as the
+ * reference's kmeans(seed 0) calls draw them) */
+hkm_t *hkm_learn(int n, int d, int nlevel, int bf, const float *v, int nb_iter_max, int nt,
+                 int verbose, int **clust_assign_out);
+/* replaces hkm.h:25 (hkm.c:119-126) */
+void hkm_delete(hkm_t *hkm);
+/* replaces hkm.h:28 (hkm.c:131-150): tree descent on the GPU */
+void hkm_quantize(const hkm_t *hkm, int n, const float *v, int *idx);
+/* replace hkm.h:31-32 (hkm.c:181-232): the reference's file format */
+void hkm_write(const char *filename, const hkm_t *hkm);
+hkm_t *hkm_read(const char *filename);
+/* replaces hkm.h:35 (hkm.c:154-157) */
+float *hkm_get_centroids(const hkm_t *hkm, int l, int no);
+
 /* ---------------- part 2: status-returning twins (0 = ok, < 0 = YK_E*) ---------------- */
+
+/* hkm_learn / hkm_quantize returning an error code (the tree in *out) */
+int yael_hkm_learn_checked(int n, int d, int nlevel, int bf, const float *v, int nb_iter_max,
+                           int nt, int verbose, int **clust_assign_out, hkm_t **out);
+int yael_hkm_quantize_checked(const hkm_t *hkm, int n, const float *v, int *idx);
 
 int yael_knn_full_checked(int distance_type, int nq, int nb, int d, int k, const float *b,
                           const float *q, const float *b_weights, int *assign, float *dis);

@@ -723,6 +723,80 @@ float oy_kmeans(int d, int n, int k, int niter, const float *v, int flags,
                           nassign, NULL);
 }
 
+/* ------------------------------------------------------------------------ */
+/* hierarchical k-means (hkm.c)                                              */
+
+static __thread const int *oy_lab_sort;
+static int oy_cmp_lab(const void *x, const void *y) { /* sorting.c:324-338 */
+  const int a = *(const int *)x, b = *(const int *)y;
+  const int dt = oy_lab_sort[a] - oy_lab_sort[b];
+  return dt ? dt : a - b;
+}
+
+/* hkm.c:36-116.  Level l clusters each of its bf^l nodes into bf children
+ * with kmeans(d, nassign, bf, niter, v_node, KMEANS_INIT_RANDOM |
+ * KMEANS_QUIET, 0, 1, ...) (seed 0: lrand48() per call), the node's points
+ * taken in (label, index) order.  cent: the levels stacked (level l:
+ * bf^(l+1) rows of d); assign: final labels.  Returns -1 where the
+ * reference exits (an empty node). */
+int oy_hkm_learn(int n, int d, int nlevel, int bf, const float *points, int niter, float *cent,
+                 int *assign) {
+  int *label = (int *)calloc((size_t)n, sizeof(int));
+  int *perm = (int *)malloc(sizeof(int) * (size_t)n);
+  int *ca = (int *)malloc(sizeof(int) * (size_t)n);
+  float *v = (float *)malloc(sizeof(float) * (size_t)n * d);
+  long k = 1, off = 0;
+  int ret = 0;
+  for (int l = 0; l < nlevel && ret == 0; l++) {
+    for (int i = 0; i < n; i++) perm[i] = i;
+    oy_lab_sort = label;
+    qsort(perm, n, sizeof(int), oy_cmp_lab);
+    for (int i = 0; i < n; i++) memcpy(v + (long)d * i, points + (long)d * perm[i], sizeof(float) * d);
+    int pos = 0;
+    for (long parent = 0; parent < k; parent++) {
+      int nassign = 0;
+      while (pos + nassign < n && label[perm[pos + nassign]] == parent) nassign++;
+      if (nassign == 0) {
+        ret = -1;
+        break;
+      }
+      oy_kmeans(d, nassign, bf, niter, v + (long)d * pos, OY_INIT_RANDOM | OY_QUIET, 0, 1,
+                cent + (off + parent * bf) * d, NULL, ca, NULL);
+      for (int i = 0; i < nassign; i++) {
+        const int t = perm[pos + i];
+        label[t] = label[t] * bf + ca[i];
+      }
+      pos += nassign;
+    }
+    off += k * bf;
+    k *= bf;
+  }
+  if (ret == 0) memcpy(assign, label, sizeof(int) * (size_t)n);
+  free(label);
+  free(perm);
+  free(ca);
+  free(v);
+  return ret;
+}
+
+/* hkm.c:131-150: per point, nn() among the children of its node, level by
+ * level (cent stacked as in oy_hkm_learn) */
+void oy_hkm_quantize(int nlevel, int bf, int d, const float *cent, int n, const float *v,
+                     int *idx) {
+  for (int i = 0; i < n; i++) {
+    long vw = 0, off = 0, k = 1;
+    for (int l = 0; l < nlevel; l++) {
+      int t;
+      float dd;
+      oy_knn_full(2, 1, bf, d, 1, cent + (off + vw * bf) * d, v + (long)d * i, NULL, &t, &dd);
+      vw = vw * bf + t;
+      off += k * bf;
+      k *= bf;
+    }
+    idx[i] = (int)vw;
+  }
+}
+
 /* Size of the trace struct, so Python can allocate it. */
 int oy_trace_sizeof(void) { return (int)sizeof(oy_kmeans_trace); }
 

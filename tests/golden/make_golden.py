@@ -150,6 +150,44 @@ def kpp_cases(ref):
     return out
 
 
+def hkm_cases(ref):
+    """hierarchical k-means (own generator): a tree learned with one k-means
+    step per node on integer data (exact everywhere) with its quantisation
+    of new points and the bytes hkm_write gives; a float-data tree; and the
+    descent through a tree of integer centroids (exact distances).  The
+    node seeds come from lrand48(), seeded here with srand48."""
+    import tempfile
+    from oracle_bind import libc_srand48, make_hkm
+    rng = np.random.default_rng(20261204)
+    out = {}
+    cen = rng.uniform(0, 255, (30, 16))
+    v = np.clip(np.rint(cen[rng.integers(0, 30, 6000)] + 20 * rng.standard_normal((6000, 16))), 0, 255)
+    v = v.astype(np.float32)
+    q = np.clip(np.rint(cen[rng.integers(0, 30, 500)] + 20 * rng.standard_normal((500, 16))), 0, 255)
+    q = q.astype(np.float32)
+    libc_srand48(12345)
+    levels, labels, h = ref.hkm_learn(v, 3, 4, 1)
+    qidx = ref.hkm_quantize(h, q)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "tree.hkm")
+        ref.lib.hkm_write(path.encode(), h)
+        raw = np.frombuffer(open(path, "rb").read(), np.uint8).copy()
+    out["hkm_step"] = dict(v=v, nlevel=np.int32(3), bf=np.int32(4), niter=np.int32(1),
+                           srand48=np.int64(12345), c0=levels[0], c1=levels[1], c2=levels[2],
+                           labels=labels, q=q, qidx=qidx, hkm_file=raw)
+    vf = rng.standard_normal((3000, 8)).astype(np.float32)
+    libc_srand48(777)
+    levels, labels, h = ref.hkm_learn(vf, 2, 5, 6)
+    out["hkm_float"] = dict(v=vf, nlevel=np.int32(2), bf=np.int32(5), niter=np.int32(6),
+                            srand48=np.int64(777), c0=levels[0], c1=levels[1], labels=labels)
+    tl = [rng.integers(0, 64, (6 ** (l + 1), 12)).astype(np.float32) for l in range(3)]
+    hq, _arr, _keep = make_hkm(tl, 6)
+    qi = rng.integers(0, 64, (2000, 12)).astype(np.float32)
+    qidx = ref.hkm_quantize(ctypes.pointer(hq), qi)
+    out["hkm_quant"] = dict(c0=tl[0], c1=tl[1], c2=tl[2], bf=np.int32(6), q=qi, qidx=qidx)
+    return out
+
+
 def rng_cases(ref):
     lib = ref.lib
     out = {}
@@ -184,6 +222,7 @@ def main():
     cases.update(rng_cases(ref))
     cases.update(shortlist_cases(ref, rng))
     cases.update(kpp_cases(ref))
+    cases.update(hkm_cases(ref))
     only = set(sys.argv[1:])  # fixture names to (re)write; none: all
     wrote = 0
     for name, arrs in cases.items():

@@ -144,6 +144,30 @@ class Oracle(_Api):
         return float(r), cent, dis, assign, nassign, dict(niter=int(niter_done), qerr=qerr,
                                                          nreassign=nre, nreassign_tot=int(tot))
 
+    def hkm_learn(self, v, nlevel, bf, niter):
+        """(levels, leaf labels) of the restatement (None where it fails)"""
+        v = _f32(v)
+        n, d = v.shape
+        rows = sum(bf ** (l + 1) for l in range(nlevel))
+        cent = np.zeros((rows, d), np.float32)
+        labels = np.empty(n, np.int32)
+        if self.lib.oy_hkm_learn(n, d, nlevel, bf, _p(v, _f32p), niter, _p(cent, _f32p),
+                                 _p(labels, _i32p)) != 0:
+            return None
+        levels, off = [], 0
+        for l in range(nlevel):
+            levels.append(cent[off:off + bf ** (l + 1)].copy())
+            off += bf ** (l + 1)
+        return levels, labels
+
+    def hkm_quantize(self, levels, bf, v):
+        v = _f32(v)
+        cent = np.ascontiguousarray(np.concatenate(levels), np.float32)
+        idx = np.empty(v.shape[0], np.int32)
+        self.lib.oy_hkm_quantize(len(levels), bf, cent.shape[1], _p(cent, _f32p), v.shape[0],
+                                 _p(v, _f32p), _p(idx, _i32p))
+        return idx
+
     def reassign_empty(self, cent, nassign, seed):
         cent = _f32(cent).copy()
         k, d = cent.shape
@@ -174,10 +198,50 @@ def _openblas_single_thread():
         lib.scipy_openblas_set_num_threads(1)
 
 
+class HkmT(ctypes.Structure):
+    """hkm_t (hkm.h:10-16)."""
+    _fields_ = [("nlevel", ctypes.c_int), ("bf", ctypes.c_int), ("k", ctypes.c_int),
+                ("d", ctypes.c_int), ("centroids", ctypes.POINTER(ctypes.POINTER(ctypes.c_float)))]
+
+
+def make_hkm(levels, bf):
+    """An hkm_t over numpy level arrays (kept alive by the returned tuple)."""
+    levels = [_f32(a) for a in levels]
+    arr = (ctypes.POINTER(ctypes.c_float) * len(levels))(*[_p(a, _f32p) for a in levels])
+    h = HkmT(len(levels), bf, bf ** len(levels), levels[0].shape[1], arr)
+    return h, arr, levels
+
+
+def libc_srand48(seed):
+    """Seed the process's lrand48 stream (hkm's kmeans calls draw from it)."""
+    ctypes.CDLL(None).srand48(ctypes.c_long(seed))
+
+
 class Reference(_Api):
     def __init__(self):
         _openblas_single_thread()
         super().__init__(REF_SO, "")
+
+    def hkm_learn(self, v, nlevel, bf, niter):
+        """(levels: list of (bf^(l+1), d) arrays, leaf labels, hkm_t pointer)"""
+        v = _f32(v)
+        n, d = v.shape
+        f = self.lib.hkm_learn
+        f.restype = ctypes.POINTER(HkmT)
+        asg = ctypes.POINTER(ctypes.c_int)()
+        h = f(n, d, nlevel, bf, _p(v, _f32p), niter, 1, 0, ctypes.byref(asg))
+        hc = h.contents
+        levels = [np.ctypeslib.as_array(hc.centroids[l], shape=(bf ** (l + 1), d)).copy()
+                  for l in range(nlevel)]
+        labels = np.ctypeslib.as_array(asg, shape=(n,)).copy()
+        ctypes.CDLL(None).free(asg)
+        return levels, labels, h
+
+    def hkm_quantize(self, hptr, v):
+        v = _f32(v)
+        idx = np.empty(v.shape[0], np.int32)
+        self.lib.hkm_quantize(hptr, v.shape[0], _p(v, _f32p), _p(idx, _i32p))
+        return idx
 
 
 def openblas_path():

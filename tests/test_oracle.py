@@ -237,3 +237,41 @@ def test_planted_mistake_is_caught():
 
 def test_golden_files_present():
     assert len(glob.glob(os.path.join(GOLD, "*.npz"))) >= 20
+
+
+@pytest.mark.parametrize("case", ["hkm_step", "hkm_float"])
+def test_hkm_learn_golden_bitexact_with_reference_blas(oracle_blas, case):
+    """The restated tree driver (label sort, kmeans(seed 0) per node with
+    lrand48 seeds, label update) gives the reference's tree and labels."""
+    from oracle_bind import libc_srand48
+    g = load(case)
+    nlevel, bf = int(g["nlevel"]), int(g["bf"])
+    libc_srand48(int(g["srand48"]))
+    levels, labels = oracle_blas.hkm_learn(g["v"], nlevel, bf, int(g["niter"]))
+    for l in range(nlevel):
+        assert np.array_equal(bits(levels[l]), bits(g["c%d" % l])), l
+    assert np.array_equal(labels, g["labels"])
+
+
+def test_hkm_quantize_golden(oracle, oracle_blas):
+    """Descent through a tree of integer centroids (exact distances, any
+    GEMM) and through the learned tree (reference BLAS)."""
+    g = load("hkm_quant")
+    levels = [g["c0"], g["c1"], g["c2"]]
+    assert np.array_equal(oracle.hkm_quantize(levels, int(g["bf"]), g["q"]), g["qidx"])
+    s = load("hkm_step")
+    idx = oracle_blas.hkm_quantize([s["c0"], s["c1"], s["c2"]], int(s["bf"]), s["q"])
+    assert np.array_equal(idx, s["qidx"])
+
+
+def test_hkm_invariants(oracle):
+    """Labels are leaves (< bf^nlevel); the level-0 centroids are the k-means
+    of the whole set; each leaf label's parent is the point's level-1 node."""
+    g = load("hkm_step")
+    from oracle_bind import libc_srand48
+    libc_srand48(int(g["srand48"]))
+    levels, labels = oracle.hkm_learn(g["v"], 3, 4, 1)
+    assert labels.min() >= 0 and labels.max() < 4 ** 3
+    # one step on integer data from the same seeds: the restatement's own GEMM
+    # loop gives the reference's tree too (exact arithmetic)
+    assert np.array_equal(labels, g["labels"])

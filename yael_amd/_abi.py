@@ -18,6 +18,15 @@ PROTOS = {
     "nn_thread": (C.c_double, [I, I, I, vp, vp, vp, I]),
     "knn": (vp, [I, I, I, I, vp, vp, vp]),
     "knn_thread": (vp, [I, I, I, I, vp, vp, vp, I]),
+    # hkm.h
+    "hkm_learn": (vp, [I, I, I, I, vp, I, I, I, vp]),
+    "hkm_delete": (None, [vp]),
+    "hkm_quantize": (None, [vp, I, vp, vp]),
+    "hkm_write": (None, [C.c_char_p, vp]),
+    "hkm_read": (vp, [C.c_char_p]),
+    "hkm_get_centroids": (vp, [vp, I, I]),
+    "yael_hkm_learn_checked": (I, [I, I, I, I, vp, I, I, I, vp, vp]),
+    "yael_hkm_quantize_checked": (I, [vp, I, vp, vp]),
     "compute_cross_distances": (None, [I, I, I, vp, vp, vp]),
     "compute_cross_distances_nonpacked": (None, [I, I, I, vp, I, vp, I, vp, I]),
     "compute_cross_distances_thread": (None, [I, I, I, vp, vp, vp, I]),
@@ -62,6 +71,7 @@ PROTOS = {
     "yk_host_alloc": (I, [C.POINTER(vp), C.c_size_t]),
     "yk_host_free": (I, [vp]),
     "yk_kpp_update": (I, [vp, L, I, vp, L, vp, I, vp]),
+    "yk_hkm_quantize": (I, [vp, vp, L, I, I, I, vp, vp, vp]),
     "yk_memcpy_d2h": (I, [vp, vp, C.c_size_t]),
     "yk_memcpy_d2d": (I, [vp, vp, C.c_size_t]),
     "yk_memset": (I, [vp, I, C.c_size_t]),

@@ -43,6 +43,12 @@ int yh_kmeans(int d, int n, int k, int niter, const float *v, int v_on_device, i
 #define YH_KM_BF16_MIN_K 4096
 #define YH_KM_KPRIME 8
 
+/* the batched form (host/yael_batch.c) without the library lock */
+int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, const float *v,
+                      int v_on_device, int flags, const long *seeds, float *cent_out,
+                      float *dis_out, int *assign_out, int *nassign_out, float *ret_out,
+                      int *iters_out, int use_graph, int *chunk_used);
+
 /* empty-cluster split (kmeans.c:166-209) on host copies of the centroids */
 int yh_reassign_empty(int d, int k, float *cent, long ldc, const int *nassign, unsigned seed);
 /* first k entries of the partial Fisher-Yates of kmeans.c:15-20 */

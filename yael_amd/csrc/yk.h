@@ -135,6 +135,11 @@ int yk_sum_f64(const float *x, long n, double *out_host, void *ws);
  * receives best as well, valid after yk_sync */
 int yk_kpp_update(const float *v, long n, int d, const double *vn, long s, float *best, int first,
                   float *host_copy);
+/* hkm tree descent: idx[i] = leaf of point i (v padded, vn double norms);
+ * cent/cn: the levels' centroids stacked (level l: bf^(l+1) padded rows)
+ * and their float norms */
+int yk_hkm_quantize(const float *v, const double *vn, long n, int d, int nlevel, int bf,
+                    const float *cent, const float *cn, int *idx);
 /* dst row i = src row idx[i] (padded rows; idx is a device array) */
 int yk_gather_rows(const float *src, const int *idx, long n, int d, float *dst);
 

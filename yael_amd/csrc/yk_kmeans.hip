@@ -300,6 +300,57 @@ extern "C" int yk_kpp_update(const float *v, long n, int d, const double *vn, lo
   return YK_OK;
 }
 
+// hkm tree descent (hkm.c:131-150), one thread per point: at each level the
+// bf children of the point's node compared as nn() does (nn.c:404-440):
+// float(centroid float norm + point double norm), then -2 * dot added in
+// float, strict '<' from (-1, 1e30).  Levels are stacked in cent/cn (level
+// l: bf^(l+1) padded rows).  A point with no child below 1e30 at some level
+// gets -1 (the reference would index outside the level).
+__global__ __launch_bounds__(256) void yk_hkm_quantize_kernel(
+    const float *__restrict__ v, const double *__restrict__ vn, long n, int ld, int d, int nlevel,
+    int bf, const float *__restrict__ cent, const float *__restrict__ cn, int *__restrict__ idx) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float *x = v + i * ld;
+  const double xn = vn[i];
+  long node = 0, lvl = 0, width = 1;  // node of the point, first row of the level, nodes
+  for (int l = 0; l < nlevel; l++) {
+    const long row0 = lvl + node * bf;
+    int best = -1;
+    float bv = 1e30f;
+    for (int c = 0; c < bf; c++) {
+      const float *cc = cent + (row0 + c) * ld;
+      float dot = 0.f;
+      for (int j = 0; j < d; j++) dot = dot + cc[j] * x[j];
+      float dist = (float)((double)cn[row0 + c] + xn);
+      dist = dist + -2.0f * dot;
+      if (dist < bv) {
+        bv = dist;
+        best = c;
+      }
+    }
+    if (best < 0) {
+      node = -1;
+      break;
+    }
+    node = node * bf + best;
+    lvl += width * bf;
+    width *= bf;
+  }
+  idx[i] = (int)node;
+}
+
+extern "C" int yk_hkm_quantize(const float *v, const double *vn, long n, int d, int nlevel, int bf,
+                               const float *cent, const float *cn, int *idx) {
+  YK_TRY(yk_init());
+  if (n <= 0) return YK_OK;
+  if (nlevel < 1 || bf < 1) return yk_set_error(YK_EINVAL, "yk_hkm_quantize: bad tree");
+  hipLaunchKernelGGL(yk_hkm_quantize_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     yk_stream(), v, vn, n, yk_ld(d), d, nlevel, bf, cent, cn, idx);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
 __global__ void yk_gather_kernel(const float *__restrict__ src, const int *__restrict__ idx,
                                  long n, int ld, float *__restrict__ dst) {
   long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
