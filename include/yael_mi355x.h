@@ -18,6 +18,8 @@
 #ifndef YAEL_MI355X_H_INCLUDED
 #define YAEL_MI355X_H_INCLUDED
 
+#include <stdio.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -117,6 +119,36 @@ void hkm_write(const char *filename, const hkm_t *hkm);
 hkm_t *hkm_read(const char *filename);
 /* replaces hkm.h:35 (hkm.c:154-157) */
 float *hkm_get_centroids(const hkm_t *hkm, int l, int no);
+
+/* vector files (doc/file_format.rst; vector.h I/O): fvecs = per vector int
+ * d then d floats, ivecs = int d then d ints, bvecs = int d then d bytes,
+ * text = whitespace-separated numbers.  Host-side; same return values as
+ * vector.c (vectors read or written, -1 on error). */
+/* replace vector.c:628-641 */
+long fvecs_fsize(const char *fname, int *d_out, int *n_out);
+long ivecs_fsize(const char *fname, int *d_out, int *n_out);
+long bvecs_fsize(const char *fname, int *d_out, int *n_out);
+/* replaces vector.c:882-920 */
+int fvecs_read(const char *fname, int d, int n, float *v);
+/* replaces vector.c:923-936 (bvecs read as floats) */
+int b2fvecs_read(const char *fname, int d, int n, float *v);
+/* replaces vector.c:939-962 */
+int fvecs_read_txt(const char *fname, int d, int n, float *v);
+/* replace vector.c:651-666, 1307-1322 (malloc'd arrays) */
+int fvecs_new_read(const char *fname, int *d_out, float **vf_out);
+int ivecs_new_read(const char *fname, int *d_out, int **vi_out);
+/* replace vector.c:1369-1383, 1446-1456, 1459-1472, 1475-1492 */
+int fvec_fwrite(FILE *fo, const float *v, int d);
+int fvecs_fwrite(FILE *fo, int d, int n, const float *vf);
+int fvecs_write(const char *fname, int d, int n, const float *vf);
+int fvecs_write_txt(const char *fname, int d, int n, const float *vf);
+/* replace vector.c:1494-1508, 1513-1520, 1524-1537, 1279-1295 */
+int ivec_fwrite(FILE *f, const int *v, int d);
+int ivecs_fwrite(FILE *f, int d, int n, const int *v);
+int ivecs_write(const char *fname, int d, int n, const int *v);
+int ivecs_write_txt(const char *fname, int d, int n, const int *v);
+/* replaces vector.c:1955-1964 */
+long fvec_purge_nans(float *v, long n, float replace_value);
 
 /* ---------------- part 2: status-returning twins (0 = ok, < 0 = YK_E*) ---------------- */
 
