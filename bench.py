@@ -593,7 +593,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": ach16, "peak": PEAK_BF16_TFLOPS,
                      "unit": "TFLOP/s", "frac": ach16 / PEAK_BF16_TFLOPS,
                      "traffic": traffic,
-                     "kernel": "yk_scan_bf16<2,9,3,1> (BF16 filtered pass, MFMA 32x32x16)",
+                     "kernel": "yk_scan_bf16<3,9,3,1> (BF16 filtered pass, MFMA 32x32x16)",
                      "flop_per_launch": flops, "kernel_ms_per_launch": bf_ms_launch,
                      "launches_per_step": bf_n / args.steps,
                      "sketch_kernel_ms_per_launch": sk_ms / max(sk_n, 1)},
@@ -612,7 +612,7 @@ def main():
             "fp32": {"queries_per_s": qps32, "ms_per_step": r["sec32"] * 1e3,
                      "roofline": {"bound": "mfma", "achieved": ach32, "peak": PEAK_FP32_TFLOPS,
                                   "unit": "TFLOP/s", "frac": ach32 / PEAK_FP32_TFLOPS,
-                                  "kernel": "yk_scan_f32<1,true,3,1> (FP32 filtered pass)",
+                                  "kernel": "yk_scan_f32<1,8,3,1> (FP32 filtered pass)",
                                   "kernel_ms_per_launch": f32_ms_launch,
                                   "launches_per_step": f32_n / args.steps,
                                   "sketch_kernel_ms_per_launch":

@@ -836,174 +836,195 @@ __global__ void yk_bf_zero_kernel(int *p, long n) {
 }
 
 // ---------------------------------------------------------------------------
-// FP32 re-rank (bit-identical to the FP32 form) + certificate.
-// One wave per query; MFMA rows = 32 candidates, every column the query.
+// FP32 re-rank (bit-identical to the FP32 form) + certificate.  MFMA rows =
+// 32 candidates, every column the query.
+
+// The re-rank is split over waves: yk_rerank_tile_kernel, one wave per
+// (query, 32-candidate tile) -- the tiles of all queries run at once --
+// writes each candidate's FP32-form key to ex[q * kp + candidate] (the value
+// is the FP32 form's instruction sequence for the pair: the same MFMA chain
+// in the same order, then float(qn + bn) - 2 dot); yk_rerank_fin_kernel, one
+// wave per query, sorts the re-ranked keys in registers, writes the k
+// results and applies the certificate.
+//
+// E = E_bf + E_32 bounds |FP32-form value - first-pass value| for any pair
+// of the query (also the certificate's margin).  Only candidates whose
+// first-pass value is <= v_k + 2E (v_k: the k-th first-pass value) can be
+// among the FP32 form's k nearest: the k first-pass-smallest have FP32
+// values <= v_k + E, any candidate above the bound one > v_k + E.  The
+// candidates are in ascending first-pass order, so the re-ranked ones are
+// the leading tiles that start within the bound.
+
+// E = E_bf + E_32 for query stats qs and base maxima bs, and the re-rank
+// bound v_k + 2E (INFINITY when the k-th key is missing)
+__device__ __forceinline__ void yk_rerank_bound(bool dot, float4 qs, float4 bs, float gam_bf,
+                                                float gam_32, uint64_t kkey, double *ebf,
+                                                double *e32, double *bound) {
+  if (dot) {
+    *ebf = (double)qs.z * bs.x + (double)qs.y * bs.z + gam_bf * (double)qs.y * bs.y;
+    *e32 = gam_32 * (double)qs.x * bs.x;
+  } else {
+    *ebf = 2.0 * ((double)qs.z * bs.x + (double)qs.y * bs.z) +
+           gam_bf * (2.0 * qs.y * bs.y + qs.w + bs.w);
+    *e32 = gam_32 * (2.0 * qs.x * bs.x + qs.w + bs.w);
+  }
+  *bound = kkey == YK_KEY_NONE ? INFINITY
+                               : (double)yk_fkey_inv((uint32_t)(kkey >> 32)) + 2.0 * (*ebf + *e32);
+}
+
+__device__ __forceinline__ bool yk_rerank_needed(const uint64_t *cq, int kp, int t, double bound) {
+  if (t >= kp) return false;
+  const uint64_t kk = cq[t];
+  return kk != YK_KEY_NONE && (double)yk_fkey_inv((uint32_t)(kk >> 32)) <= bound;
+}
 
 template <int NCH>
-__global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
+__global__ __launch_bounds__(64) void yk_rerank_tile_kernel(
     const float *__restrict__ q, const double *__restrict__ qn, const float *__restrict__ b,
-    const float *__restrict__ bn, long nb, int ld, int type, const uint64_t *__restrict__ cand,
-    int kp, int k, const float4 *__restrict__ qstats, const float4 *__restrict__ bmax, float gam_bf,
-    float gam_32, int *out_i, float *out_d, int *unc_list, int *n_unc) {
-  __shared__ uint64_t ex[YK_KMAX];
-  const long qi = blockIdx.x;
+    const float *__restrict__ bn, int ld, int type, const uint64_t *__restrict__ cand, int kp,
+    int k, const float4 *__restrict__ qstats, const float4 *__restrict__ bmax, float gam_bf,
+    float gam_32, uint64_t *__restrict__ ex) {
+  const long qi = blockIdx.y;
+  const int t0 = blockIdx.x * 32;
   const int lane = threadIdx.x, r = lane & 31, h = lane >> 5;
-  const int nch = NCH > 0 ? NCH : ld >> 4;
   const bool dot = type == YK_DOT;
   const uint64_t *cq = cand + qi * kp;
+  double ebf, e32, bound;
+  yk_rerank_bound(dot, qstats[qi], *bmax, gam_bf, gam_32, cq[k - 1], &ebf, &e32, &bound);
+  if (!yk_rerank_needed(cq, kp, t0, bound)) return;
+  const int ci = t0 + r;
+  const uint64_t key = ci < kp ? cq[ci] : YK_KEY_NONE;
+  const long row = key == YK_KEY_NONE ? 0 : (long)(uint32_t)key;
+  const float *bp = b + row * ld + h * 8;
   const float *qp = q + qi * ld + h * 8;
-  const double qnd = dot ? 0.0 : qn[qi];
-  // NCH > 0: the query's chunks held in registers for all candidate tiles
-  float4 qr[NCH > 0 ? 2 * NCH : 1];
+  const int nch = NCH > 0 ? NCH : ld >> 4;
+  f32x16 acc;
+#pragma unroll
+  for (int e = 0; e < 16; e++) acc[e] = 0.f;
+  auto chunk = [&](int c) {
+    const float4 a0 = *(const float4 *)(bp + c * 16), a1 = *(const float4 *)(bp + c * 16 + 4);
+    const float4 b0 = *(const float4 *)(qp + c * 16), b1 = *(const float4 *)(qp + c * 16 + 4);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc, 0, 0, 0);
+  };
   if (NCH > 0) {
+    // every row load issued before the chain starts (one memory round trip
+    // per wave instead of one per chunk)
+    float4 av[2 * (NCH > 0 ? NCH : 1)];
 #pragma unroll
     for (int c = 0; c < NCH; c++) {
-      qr[2 * c] = *(const float4 *)(qp + c * 16);
-      qr[2 * c + 1] = *(const float4 *)(qp + c * 16 + 4);
+      av[2 * c] = *(const float4 *)(bp + c * 16);
+      av[2 * c + 1] = *(const float4 *)(bp + c * 16 + 4);
     }
-  }
-  // Bound on |FP32-form value - first-pass value| of any pair of this query
-  // (also the certificate's margin): E = E_bf + E_32.
-  const float4 qs = qstats[qi], bs = *bmax;  // {|x|, |bf x|, |e|, |x|^2}
-  double ebf, e32;
-  if (dot) {
-    ebf = (double)qs.z * bs.x + (double)qs.y * bs.z + gam_bf * (double)qs.y * bs.y;
-    e32 = gam_32 * (double)qs.x * bs.x;
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int c = 0; c < NCH; c++) {
+      const float4 a0 = av[2 * c], a1 = av[2 * c + 1];
+      // the query's chunk: the same address for every lane of a half (an
+      // L1 hit); hoisting it too costs two waves per SIMD and measured slower
+      const float4 b0 = *(const float4 *)(qp + c * 16), b1 = *(const float4 *)(qp + c * 16 + 4);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc, 0, 0, 0);
+    }
   } else {
-    ebf = 2.0 * ((double)qs.z * bs.x + (double)qs.y * bs.z) +
-          gam_bf * (2.0 * qs.y * bs.y + qs.w + bs.w);
-    e32 = gam_32 * (2.0 * qs.x * bs.x + qs.w + bs.w);
+    for (int c = 0; c < nch; c++) chunk(c);
   }
-  // Only candidates with first-pass value <= v_k + 2E can be among the FP32
-  // form's k nearest (the k first-pass-smallest have FP32 values <= v_k + E,
-  // any candidate above has FP32 value > v_k + E); the candidates are in
-  // ascending first-pass order, so the re-rank stops at the first tile that
-  // starts above the bound.
-  const uint64_t kkey = cq[k - 1];
-  const double bound = kkey == YK_KEY_NONE
-                           ? INFINITY
-                           : (double)yk_fkey_inv((uint32_t)(kkey >> 32)) + 2.0 * (ebf + e32);
-  auto needed = [&](int t) {
-    if (t >= kp) return false;
-    const uint64_t kk = cq[t];
-    return kk != YK_KEY_NONE && (double)yk_fkey_inv((uint32_t)(kk >> 32)) <= bound;
-  };
-  int nproc = 0;  // candidates re-ranked
-  // Two 32-candidate tiles at a time, their MFMA chains interleaved (each
-  // chain is serial: the FP32 form's instruction sequence per pair).
-  for (int t0 = 0; t0 < kp && needed(t0); t0 += 64) {
-    const bool two = needed(t0 + 32);
-    nproc = min(kp, t0 + (two ? 64 : 32));
-    long row[2];
+  // column 0 (lanes 0 and 32) holds the tile's candidate rows 8g + 4h + e
+  if (r == 0) {
+    const double qnd = dot ? 0.0 : qn[qi];
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const int ci = t0 + 32 * u + r;
-      const uint64_t key = ci < kp ? cq[ci] : YK_KEY_NONE;
-      row[u] = key == YK_KEY_NONE ? 0 : (long)(uint32_t)key;
-    }
-    const float *bp0 = b + row[0] * ld + h * 8, *bp1 = b + row[1] * ld + h * 8;
-    f32x16 acc0, acc1;
-#pragma unroll
-    for (int e = 0; e < 16; e++) acc0[e] = acc1[e] = 0.f;
-    auto chunk = [&](int c, float4 b0, float4 b1) {
-      const float4 a0 = *(const float4 *)(bp0 + c * 16), a1 = *(const float4 *)(bp0 + c * 16 + 4);
-      const float4 e0 = *(const float4 *)(bp1 + c * 16), e1 = *(const float4 *)(bp1 + c * 16 + 4);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e0.x, b0.x, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e0.y, b0.y, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e0.z, b0.z, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e0.w, b0.w, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e1.x, b1.x, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e1.y, b1.y, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e1.z, b1.z, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(e1.w, b1.w, acc1, 0, 0, 0);
-    };
-    if (!two) {
-      // one tile left: its chain alone
-      auto chunk1 = [&](int c, float4 b0, float4 b1) {
-        const float4 a0 = *(const float4 *)(bp0 + c * 16),
-                     a1 = *(const float4 *)(bp0 + c * 16 + 4);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc0, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc0, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc0, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc0, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc0, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc0, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc0, 0, 0, 0);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc0, 0, 0, 0);
-      };
-      if (NCH > 0) {
-#pragma unroll
-        for (int c = 0; c < NCH; c++) chunk1(c, qr[2 * c], qr[2 * c + 1]);
-      } else {
-        for (int c = 0; c < nch; c++)
-          chunk1(c, *(const float4 *)(qp + c * 16), *(const float4 *)(qp + c * 16 + 4));
+    for (int i = 0; i < 16; i++) {
+      const int cidx = t0 + 8 * (i >> 2) + 4 * h + (i & 3);
+      if (cidx >= kp) continue;
+      const uint64_t ck = cq[cidx];
+      uint64_t xk = YK_KEY_NONE;
+      if (ck != YK_KEY_NONE) {
+        const int bi = (int)(uint32_t)ck;
+        float v = dot ? acc[i] : (float)(qnd + (double)bn[bi]) - 2.0f * acc[i];
+        // k = 1 follows nn.c:404-440: only values below 1e30 are taken
+        if (v == v && (k > 1 || v < 1e30f)) xk = yk_key(v, bi);
       }
-    } else if (NCH > 0) {
-#pragma unroll
-      for (int c = 0; c < NCH; c++) chunk(c, qr[2 * c], qr[2 * c + 1]);
-    } else {
-      for (int c = 0; c < nch; c++)
-        chunk(c, *(const float4 *)(qp + c * 16), *(const float4 *)(qp + c * 16 + 4));
-    }
-    // column 0 (lanes 0 and 32) holds the 32 candidate rows 8g + 4h + e of
-    // each tile
-    if (r == 0) {
-#pragma unroll
-      for (int u = 0; u < 2; u++) {
-        if (u == 1 && !two) break;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-          const int m = 8 * (i >> 2) + 4 * h + (i & 3);
-          const int cidx = t0 + 32 * u + m;
-          if (cidx >= kp) continue;
-          const uint64_t ck = cq[cidx];
-          uint64_t xk = YK_KEY_NONE;
-          if (ck != YK_KEY_NONE) {
-            const int bi = (int)(uint32_t)ck;
-            const float dv = u ? acc1[i] : acc0[i];
-            float v = dot ? dv : (float)(qnd + (double)bn[bi]) - 2.0f * dv;
-            // k = 1 follows nn.c:404-440: only values below 1e30 are taken
-            if (v == v && (k > 1 || v < 1e30f)) xk = yk_key(v, bi);
-          }
-          ex[cidx] = xk;
-        }
-      }
+      ex[qi * kp + cidx] = xk;
     }
   }
-  __syncthreads();
-  int p2 = 1;
-  while (p2 < nproc || p2 < k) p2 <<= 1;
-  for (int i = nproc + lane; i < p2; i += 64) ex[i] = YK_KEY_NONE;
-  __syncthreads();
-  yk_bitonic_b(ex, p2);
+}
+
+template <int E>
+__device__ __forceinline__ void yk_rerank_finish(const uint64_t *exq, int nproc, int k, int lane,
+                                                 int *oi, float *od, uint64_t *kth) {
+  uint64_t v[E];
+#pragma unroll
+  for (int i = 0; i < E; i++) {
+    const int e = lane * E + i;
+    v[i] = e < nproc ? exq[e] : YK_KEY_NONE;
+  }
+  yk_wave_sort<E>(v, lane);
   // no result: (-1, 1e30) for k = 1 (nn.c:404), (-1, NaN bits) padding for
   // k > 1 (nn.c:515-518)
   const float none_d = k == 1 ? 1e30f : __uint_as_float(0xffffffffu);
-  for (int i = lane; i < k; i += 64) {
-    uint64_t key = ex[i];
-    out_i[qi * k + i] = key == YK_KEY_NONE ? -1 : (int)(uint32_t)key;
-    out_d[qi * k + i] = key == YK_KEY_NONE ? none_d : yk_fkey_inv((uint32_t)(key >> 32));
+#pragma unroll
+  for (int i = 0; i < E; i++) {
+    const int e = lane * E + i;
+    if (e < k) {
+      oi[e] = v[i] == YK_KEY_NONE ? -1 : (int)(uint32_t)v[i];
+      od[e] = v[i] == YK_KEY_NONE ? none_d : yk_fkey_inv((uint32_t)(v[i] >> 32));
+    }
+    if (e == k - 1) *kth = v[i];
   }
+}
+
+__global__ __launch_bounds__(256) void yk_rerank_fin_kernel(
+    const uint64_t *__restrict__ cand, long n, long nb, int type, int kp, int k,
+    const float4 *__restrict__ qstats, const float4 *__restrict__ bmax, float gam_bf,
+    float gam_32, const uint64_t *__restrict__ ex, int *out_i, float *out_d, int *unc_list,
+    int *n_unc) {
+  __shared__ uint64_t kth_s[4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long qi = blockIdx.x * 4L + wave;
+  if (qi >= n) return;
+  const uint64_t *cq = cand + qi * kp;
+  double ebf, e32, bound;
+  yk_rerank_bound(type == YK_DOT, qstats[qi], *bmax, gam_bf, gam_32, cq[k - 1], &ebf, &e32,
+                  &bound);
+  int nproc = 0;  // candidates re-ranked: the leading tiles that start within the bound
+  for (int t0 = 0; yk_rerank_needed(cq, kp, t0, bound); t0 += 32) nproc = min(kp, t0 + 32);
+  const int m = max(nproc, k);
+  const uint64_t *exq = ex + qi * kp;
+  int *oi = out_i + qi * k;
+  float *od = out_d + qi * k;
+  uint64_t *kth = kth_s + wave;
+  if (m <= 256)
+    yk_rerank_finish<4>(exq, nproc, k, lane, oi, od, kth);
+  else if (m <= 512)
+    yk_rerank_finish<8>(exq, nproc, k, lane, oi, od, kth);
+  else
+    yk_rerank_finish<16>(exq, nproc, k, lane, oi, od, kth);
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   if (lane == 0) {
     // the first pass kept the kp smallest keys; fewer than kp real ones means
     // every base vector was a candidate
-    const uint64_t last = cq[kp - 1];
+    const uint64_t last = cq[kp - 1], fk = *kth;
     bool ok;
     if (last == YK_KEY_NONE || kp >= nb) {
       ok = true;
-    } else if (ex[k - 1] == YK_KEY_NONE) {
+    } else if (fk == YK_KEY_NONE) {
       ok = false;
     } else {
       const float Tb = yk_fkey_inv((uint32_t)(last >> 32));
-      const float Fk = yk_fkey_inv((uint32_t)(ex[k - 1] >> 32));
+      const float Fk = yk_fkey_inv((uint32_t)(fk >> 32));
       ok = (double)Fk < (double)Tb - ebf - e32;
     }
     if (!ok) unc_list[atomicAdd(n_unc, 1)] = (int)qi;
@@ -1016,7 +1037,7 @@ __global__ __launch_bounds__(64) void yk_rerank_cert_kernel(
 // (values i = 4s .. 4s + 3): a quarter of the MFMA chains of one query per
 // wave.  Each output element is still the FP32 form's instruction sequence
 // for its pair.  Results, conventions and certificate as in
-// yk_rerank_cert_kernel.
+// yk_rerank_tile_kernel + yk_rerank_fin_kernel.
 template <int NCH>
 __global__ __launch_bounds__(64) void yk_rerank4_kernel(
     const float *__restrict__ q, const double *__restrict__ qn, const float *__restrict__ b,
@@ -1148,8 +1169,8 @@ __global__ void yk_gather_f64_kernel(const double *__restrict__ src, const int *
 // ---------------------------------------------------------------------------
 // host orchestration
 
-static void *g_bs[8];
-static size_t g_bs_bytes[8];
+static void *g_bs[10];
+static size_t g_bs_bytes[10];
 
 static int yk_bscratch(int slot, size_t bytes, void **out) {
   if (bytes > g_bs_bytes[slot]) {
@@ -1368,20 +1389,6 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
     YK_TRY(yk_bf16_candidates(a, kp, (uint64_t *)vcand));
     hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), nunc, 1L);
     int tok = yk_prof_begin(YK_PROF_RERANK);
-    void (*rk)(const float *, const double *, const float *, const float *, long, int, int,
-               const uint64_t *, int, int, const float4 *, const float4 *, float, float, int *,
-               float *, int *, int *);
-    switch (ld >> 4) {
-      case 1: rk = yk_rerank_cert_kernel<1>; break;
-      case 2: rk = yk_rerank_cert_kernel<2>; break;
-      case 3: rk = yk_rerank_cert_kernel<3>; break;
-      case 4: rk = yk_rerank_cert_kernel<4>; break;
-      case 5: rk = yk_rerank_cert_kernel<5>; break;
-      case 6: rk = yk_rerank_cert_kernel<6>; break;
-      case 7: rk = yk_rerank_cert_kernel<7>; break;
-      case 8: rk = yk_rerank_cert_kernel<8>; break;
-      default: rk = yk_rerank_cert_kernel<0>; break;
-    }
     if (kp <= 8) {
       void (*rk4)(const float *, const double *, const float *, const float *, long, int, int,
                   const uint64_t *, long, int, int, const float4 *, const float4 *, float, float,
@@ -1403,11 +1410,31 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
                          (const float4 *)bstats_max, gam_bf, gam_32, assign + q0 * k,
                          dis + q0 * k, unc, nunc);
     } else {
-      hipLaunchKernelGGL(rk, dim3((unsigned)n), dim3(64), 0, yk_stream(),
-                         q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, nb, ld, type,
+      void (*rt)(const float *, const double *, const float *, const float *, int, int,
+                 const uint64_t *, int, int, const float4 *, const float4 *, float, float,
+                 uint64_t *);
+      switch (ld >> 4) {
+        case 1: rt = yk_rerank_tile_kernel<1>; break;
+        case 2: rt = yk_rerank_tile_kernel<2>; break;
+        case 3: rt = yk_rerank_tile_kernel<3>; break;
+        case 4: rt = yk_rerank_tile_kernel<4>; break;
+        case 5: rt = yk_rerank_tile_kernel<5>; break;
+        case 6: rt = yk_rerank_tile_kernel<6>; break;
+        case 7: rt = yk_rerank_tile_kernel<7>; break;
+        case 8: rt = yk_rerank_tile_kernel<8>; break;
+        default: rt = yk_rerank_tile_kernel<0>; break;
+      }
+      void *vex;
+      YK_TRY(yk_bscratch(8, sizeof(uint64_t) * (size_t)n * kp, &vex));
+      hipLaunchKernelGGL(rt, dim3((unsigned)((kp + 31) / 32), (unsigned)n), dim3(64), 0,
+                         yk_stream(), q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, ld, type,
                          (const uint64_t *)vcand, kp, k, (const float4 *)qstats + q0,
-                         (const float4 *)bstats_max, gam_bf, gam_32, assign + q0 * k,
-                         dis + q0 * k, unc, nunc);
+                         (const float4 *)bstats_max, gam_bf, gam_32, (uint64_t *)vex);
+      hipLaunchKernelGGL(yk_rerank_fin_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0,
+                         yk_stream(), (const uint64_t *)vcand, n, nb, type, kp, k,
+                         (const float4 *)qstats + q0, (const float4 *)bstats_max, gam_bf,
+                         gam_32, (const uint64_t *)vex, assign + q0 * k, dis + q0 * k, unc,
+                         nunc);
     }
     yk_prof_end(tok);
     YK_HIP(hipGetLastError());
