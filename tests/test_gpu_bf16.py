@@ -57,6 +57,18 @@ def test_unit_vectors_d960(dev):
     print("fallback queries", nfb, flush=True)
 
 
+@pytest.mark.parametrize("d,nq,nb", [(300, 77, 5003), (520, 300, 20011)])
+def test_long_vectors_ragged(dev, d, nq, nb):
+    """More than 16 chunks (the long-vector scan with fragment-major queries):
+    query counts off the 256-query workgroup span, a partial last base block,
+    integer data (ties) and both distance types."""
+    rng = np.random.default_rng(d)
+    b = mixture(rng, nb, d, 40, 20)
+    q = mixture(rng, nq, d, 40, 20)
+    same_as_fp32(dev, q, b, 20, 40, typ=2)
+    same_as_fp32(dev, q, b, 20, 40, typ=16)
+
+
 @pytest.mark.parametrize("d", [20, 200])
 def test_float_dims_and_dot(dev, d):
     rng = np.random.default_rng(d)

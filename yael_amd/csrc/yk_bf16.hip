@@ -322,6 +322,7 @@ struct Bf16Args {
   int *cnt;
   uint64_t *cand;       // MODE 3: candidate lists; MODE 2 with kp_direct: [nq][kp]
   int kp_direct;        // MODE 2, nsplit == 1: the lane pair's kp <= R smallest keys
+  const uint16_t *qf;   // long vectors: the queries fragment-major (as the base)
 };
 
 template <int R>
@@ -653,6 +654,112 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
   }  // work items
 }
 
+
+// Long vectors (d'/16 > 16 chunks): neither operand fits in registers.  Each
+// wave owns NB = 2 query tiles (64 queries) and takes the base in groups of
+// G = 2 blocks: the accumulators of all G x NB tiles stay in registers while
+// the chunk loop walks K, so each fragment loaded feeds two MFMAs (the
+// register-resident kernel's streaming at long d loaded three fragments per
+// two MFMAs).  The queries come from a fragment-major copy (qf), so every
+// load is one contiguous 1 KB fragment.  Chunk c + 1's fragments load while
+// chunk c's MFMAs run (two register sets used in turn, the loop unrolled by
+// two).  Selection per block as in yk_scan_bf16.  (Groups of 3 or 4 blocks
+// measured no faster and need one wave per SIMD.)
+template <int MODE, int R>
+__global__ __launch_bounds__(256) void yk_scan_bf16_long(Bf16Args a) {
+  constexpr int NB = 2, G = 2;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int r = lane & 31, h = lane >> 5;
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, wpx = gridDim.x >> 3;
+  const int ibeg = xcd * a.per_xcd, iend = min(a.total, ibeg + a.per_xcd);
+  const int nch = a.ldh >> 4;
+  __shared__ uint64_t cstage[MODE == 3 ? 4 * NB * 64 * YK_CSTAGE : 1];
+  __shared__ float vstage[MODE >= 2 ? 4 * 64 * 16 : 1];
+  for (int item = ibeg + local; item < iend; item += wpx) {
+    const int tile = item % a.tiles_x, split = item / a.tiles_x;
+    const long qbase = ((long)tile * nwave + wave) * (32 * NB);
+    const long bbeg = (long)split * a.brange;
+    const long bend = min(a.nb, bbeg + a.brange);
+    BfSel<NB, MODE, R> sel;
+    sel.init(a, qbase, r, lane, cstage + wave * (MODE == 3 ? NB * 64 * YK_CSTAGE : 0),
+             vstage + (MODE >= 2 ? (wave * 64 + lane) * 16 : 0));
+    if (MODE == 3 && !__any(sel.active())) continue;
+    // query tiles qbase/32 and qbase/32 + 1 of the fragment-major copy (its
+    // rows are padded to the launch's query span, the pad rows zero)
+    const uint16_t *qf0 = a.qf + ((qbase >> 5) * nch * 64 + lane) * 8;
+    const uint16_t *qf1 = qf0 + (long)nch * 512;
+    const long nbk = (a.nb + 31) >> 5;
+    const long rbytes = (nbk - (bbeg >> 5)) * (long)nch * 1024;
+    const uint16_t *ibase = a.b + (bbeg >> 5) * (long)nch * 512;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)ibase, 0, (int)min(rbytes, 0x7fffffffL), YK_BUF_DWORD3);
+    const int voff = lane * 16;
+    const int gstride = (int)((a.bstep >> 5) * nch * 1024);  // bytes between a group's blocks
+    int sub = 0;
+    for (long bb = bbeg; bb < bend; bb += G * a.bstep) {
+      const int sblk = (int)(((bb - bbeg) >> 5) * nch * 1024);
+      f32x16 acc[G][NB];
+#pragma unroll
+      for (int g = 0; g < G; g++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+#pragma unroll
+          for (int e = 0; e < 16; e++) acc[g][j][e] = 0.f;
+      bf16x8 qa0, qa1, ba[G], qb0, qb1, bv2[G];
+      // past the range, the base loads return zeros (buffer bounds)
+      auto ld = [&](int c, bf16x8 &q0, bf16x8 &q1, bf16x8(&bv)[G]) {
+        q0 = *(const bf16x8 *)(qf0 + c * 512);
+        q1 = *(const bf16x8 *)(qf1 + c * 512);
+#pragma unroll
+        for (int g = 0; g < G; g++)
+          bv[g] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
+                                                 rs, voff, sblk + g * gstride + c * 1024, 0));
+      };
+      auto mm = [&](const bf16x8 &q0, const bf16x8 &q1, const bf16x8(&bv)[G]) {
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+          acc[g][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv[g], q0, acc[g][0], 0, 0, 0);
+          acc[g][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv[g], q1, acc[g][1], 0, 0, 0);
+        }
+      };
+      ld(0, qa0, qa1, ba);
+      for (int c = 0; c < nch; c += 2) {
+        const int c1 = c + 1 < nch ? c + 1 : c, c2 = c + 2 < nch ? c + 2 : c1;
+        ld(c1, qb0, qb1, bv2);
+        mm(qa0, qa1, ba);
+        if (c + 1 >= nch) break;
+        ld(c2, qa0, qa1, ba);
+        mm(qb0, qb1, bv2);
+      }
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const long bg = bb + g * a.bstep;
+        if (bg < bend) sel.block(a, acc[g], bg, bend, h, sub);
+        sub ^= 1;
+      }
+    }
+    sel.finish(a, qbase, split, r, h);
+  }  // work items
+}
+
+// rows [0, n) of a row-major BF16 copy (row stride ldh) -> fragment-major
+// (block b, chunk c at ((b * nch + c) * 64 + lane) * 8, lane h*32 + r holding
+// row 32b + r, columns 16c + 8h .. +7), rows [n, nrows) zero; one wave per
+// (block, chunk)
+__global__ __launch_bounds__(256) void yk_frag_from_rows_kernel(const uint16_t *__restrict__ src,
+                                                                long n, long nrows, int ldh,
+                                                                uint16_t *__restrict__ dst) {
+  const int nch = ldh >> 4;
+  const long w = blockIdx.x * 4L + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+  if (w >= (nrows >> 5) * nch) return;
+  const long b = w / nch;
+  const int c = (int)(w - b * nch);
+  const long row = b * 32 + r;
+  uint4 v = make_uint4(0, 0, 0, 0);
+  if (row < n) v = *(const uint4 *)(src + row * ldh + c * 16 + h * 8);
+  *(uint4 *)(dst + (w * 64 + lane) * 8) = v;
+}
 
 // ---------------------------------------------------------------------------
 // per-query sorts (same contracts as in yk_select.hip)
@@ -1251,6 +1358,9 @@ static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
   // 128 queries per wave at one wave per SIMD)
   const int nbq = yk_bnb(MODE, a.ldh >> 4);
   void (*kern)(Bf16Args) = yk_bscan_pick<MODE, R>(a.ldh >> 4);
+  if constexpr (MODE != 2) {
+    if ((a.ldh >> 4) > 16) kern = yk_scan_bf16_long<MODE, R>;
+  }
   a.brange = p.brange;
   a.bstep = p.bstep;
   a.nsplit = p.nsplit;
@@ -1386,6 +1496,17 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
     a.b = bh;
     a.nb = nb;
     a.ldh = ldh;
+    if ((ldh >> 4) > 16 && kp > YK_REG_R) {
+      // long vectors: the batch's queries fragment-major for the long scan,
+      // rows padded to its 256-query workgroup span
+      const long rows = (n + 255) / 256 * 256, nw = (rows >> 5) * (ldh >> 4);
+      void *vqf;
+      YK_TRY(yk_bscratch(9, sizeof(uint16_t) * (size_t)rows * ldh, &vqf));
+      hipLaunchKernelGGL(yk_frag_from_rows_kernel, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0,
+                         yk_stream(), a.q, n, rows, ldh, (uint16_t *)vqf);
+      YK_HIP(hipGetLastError());
+      a.qf = (const uint16_t *)vqf;
+    }
     YK_TRY(yk_bf16_candidates(a, kp, (uint64_t *)vcand));
     hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), nunc, 1L);
     int tok = yk_prof_begin(YK_PROF_RERANK);
