@@ -78,12 +78,23 @@ def test_float_dims_and_dot(dev, d):
     same_as_fp32(dev, q, b, 8, 32, typ=16)
 
 
-def test_forced_fallback(dev):
-    """k' = k: the certificate can rarely hold, the FP32 fallback answers."""
+def test_escalation_certifies(dev):
+    """k' = k: the first level can rarely certify; the uncertified queries run
+    again with 8k' candidates, which certify them."""
     rng = np.random.default_rng(9)
     b = rng.standard_normal((5000, 64)).astype(np.float32)
     q = rng.standard_normal((100, 64)).astype(np.float32)
     nfb = same_as_fp32(dev, q, b, 20, 20)
+    assert nfb == 0
+
+
+def test_forced_fallback(dev):
+    """k' = k = 1024 (no room to escalate): the certificate can rarely hold,
+    the FP32 form answers the uncertified queries."""
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal((5000, 64)).astype(np.float32)
+    q = rng.standard_normal((40, 64)).astype(np.float32)
+    nfb = same_as_fp32(dev, q, b, 1024, 1024)
     assert nfb > 0
 
 

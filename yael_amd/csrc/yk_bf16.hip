@@ -1267,6 +1267,16 @@ __global__ void yk_scatter_rows_kernel(const int *__restrict__ list, int n, int 
   od[dst] = sd[t];
 }
 
+// dst row i = src row idx[i], rows of `units` 16-byte units
+__global__ void yk_gather16_kernel(const uint4 *__restrict__ src, const int *__restrict__ idx,
+                                   long n, int units, uint4 *__restrict__ dst) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n * units) return;
+  const long i = t / units;
+  const int u = (int)(t - i * units);
+  dst[t] = src[(long)idx[i] * units + u];
+}
+
 __global__ void yk_gather_f64_kernel(const double *__restrict__ src, const int *__restrict__ idx,
                                      long n, double *dst) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -1276,10 +1286,15 @@ __global__ void yk_gather_f64_kernel(const double *__restrict__ src, const int *
 // ---------------------------------------------------------------------------
 // host orchestration
 
-static void *g_bs[10];
-static size_t g_bs_bytes[10];
+// Scratch buffers kept across calls, slots 0..9 per nesting level (the
+// escalation of uncertified queries runs the precision form again, one level
+// down, while the outer level still holds its own buffers).
+static void *g_bs[20];
+static size_t g_bs_bytes[20];
+static int g_bs_level = 0;
 
 static int yk_bscratch(int slot, size_t bytes, void **out) {
+  slot += 10 * g_bs_level;
   if (bytes > g_bs_bytes[slot]) {
     if (g_bs[slot]) (void)hipFree(g_bs[slot]);
     g_bs[slot] = nullptr;
@@ -1562,7 +1577,47 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
     int hn = 0;
     YK_HIP(hipMemcpyAsync(&hn, nunc, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
     YK_HIP(hipStreamSynchronize(yk_stream()));
-    if (hn > 0) {
+    // Uncertified queries, first level: the precision form again on just
+    // those queries with 8x the candidates (the certificate's margin grows
+    // with the k'-th first-pass value); what that level cannot certify goes
+    // to the FP32 form there.  The result is the FP32 form's either way.
+    const int kp2 = (int)min((long)min(8 * kp, YK_KMAX), nb);
+    if (hn > 0 && g_bs_level == 0 && kp2 > kp) {
+      const size_t szq = sizeof(float) * (size_t)hn * ld, szn = sizeof(double) * hn,
+                   szh = sizeof(uint16_t) * (size_t)hn * ldh, szs = sizeof(float4) * hn;
+      void *vg, *voi;
+      YK_TRY(yk_bscratch(6, szq + szn + szh + szs + 256, &vg));
+      YK_TRY(yk_bscratch(7, (sizeof(int) + sizeof(float)) * (size_t)hn * k + 256, &voi));
+      float *gq = (float *)vg;
+      double *gqn = (double *)((char *)vg + szq);
+      uint16_t *gqh = (uint16_t *)((char *)gqn + szn);
+      float4 *gqs = (float4 *)((char *)gqh + szh);
+      int *foi = (int *)voi;
+      float *fod = (float *)(foi + (size_t)hn * k);
+      YK_TRY(yk_gather_rows(q + q0 * ld, unc, hn, d, gq));
+      if (type != YK_DOT)
+        hipLaunchKernelGGL(yk_gather_f64_kernel, dim3((hn + 255) / 256), dim3(256), 0, yk_stream(),
+                           qn + q0, unc, (long)hn, gqn);
+      const int hu = ldh / 8;  // 16-byte units of a BF16 row
+      hipLaunchKernelGGL(yk_gather16_kernel, dim3((unsigned)(((long)hn * hu + 255) / 256)),
+                         dim3(256), 0, yk_stream(), (const uint4 *)(qh + q0 * ldh), unc, (long)hn,
+                         hu, (uint4 *)gqh);
+      hipLaunchKernelGGL(yk_gather16_kernel, dim3((unsigned)((hn + 255) / 256)), dim3(256), 0,
+                         yk_stream(), (const uint4 *)(qstats + 4 * q0), unc, (long)hn, 1,
+                         (uint4 *)gqs);
+      YK_HIP(hipGetLastError());
+      int fb2 = 0;
+      g_bs_level = 1;
+      const int rc2 = yk_knn_bf16(type, hn, nb, d, k, kp2, gq, type == YK_DOT ? nullptr : gqn, gqh,
+                                  (const float *)gqs, b, bn, bh, bstats_max, foi, fod, &fb2);
+      g_bs_level = 0;
+      YK_TRY(rc2);
+      long tot = (long)hn * k;
+      hipLaunchKernelGGL(yk_scatter_rows_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                         yk_stream(), unc, hn, k, foi, fod, assign + q0 * k, dis + q0 * k);
+      YK_HIP(hipGetLastError());
+      total_fb += fb2;
+    } else if (hn > 0) {
       // uncertified queries: the FP32 form on their rows
       void *vq, *vqn, *voi, *vod;
       YK_TRY(yk_bscratch(6, sizeof(float) * (size_t)hn * ld + sizeof(double) * hn + 256, &vq));
