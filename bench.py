@@ -260,6 +260,31 @@ def bench_hkm(ya, base, cpu, n=200_000, nlevel=2, bf=32, niter=10, nq_cpu=20_000
     return rec
 
 
+def bench_dropin(ya, base, queries, k):
+    """knn_full through the C API on host arrays (the drop-in call: uploads,
+    norms, search, downloads), automatic form vs the FP32 form only."""
+    lib = ya.lib()
+
+    def rate(mode, reps):
+        old = lib.yael_knn_precision(mode)
+        try:
+            ya.knn(queries, base, k)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                idx, dis = ya.knn(queries, base, k)
+            sec = (time.perf_counter() - t0) / reps
+        finally:
+            lib.yael_knn_precision(old)
+        return len(queries) / sec, idx, dis
+    r0, i0, d0 = rate(0, 3)
+    r1, i1, d1 = rate(1, 2)
+    return dict(config="knn_full(2, nq=%d, nb=%d, d=%d, k=%d) on host arrays, PCIe copies "
+                       "included" % (len(queries), len(base), base.shape[1], k),
+                queries_per_s=r0, fp32_form_queries_per_s=r1,
+                same_bits_as_fp32_form=bool(np.array_equal(i0, i1) and
+                                            np.array_equal(d0.view(np.int32), d1.view(np.int32))))
+
+
 def bench_kmeans_large(ya, dev, n=10_000_000, d=128, k=65536, niter=20, seed=1, nsub=8,
                        ksub=256):
     """n=1e7, d=128, k=65,536 k-means with the BF16 assign form (BF16 first
@@ -641,6 +666,11 @@ def main():
         print("# kmeans++ seeding: %.1f ms (%s)" % (kpp["init_s"] * 1e3, "cpu reference %.1f ms"
               % (kpp["cpu_reference_init_s"] * 1e3) if "cpu_reference_init_s" in kpp else "no cpu"),
               file=sys.stderr, flush=True)
+        di = bench_dropin(ya, base, queries, k)
+        out["extras"]["drop_in_knn_full"] = di
+        print("# drop-in knn_full: %.0f queries/s (FP32 form %.0f), same bits %s" % (
+            di["queries_per_s"], di["fp32_form_queries_per_s"], di["same_bits_as_fp32_form"]),
+            file=sys.stderr, flush=True)
         hk = bench_hkm(ya, base, cpu=not args.no_cpu)
         out["extras"]["hkm"] = hk
         print("# hkm: learn %.3f s, quantize %.3g points/s%s" % (

@@ -157,6 +157,13 @@ int yael_hkm_learn_checked(int n, int d, int nlevel, int bf, const float *v, int
                            int nt, int verbose, int **clust_assign_out, hkm_t **out);
 int yael_hkm_quantize_checked(const hkm_t *hkm, int n, const float *v, int *idx);
 
+/* Form of knn_full / knn_full_thread / yael_knn_full_checked (process-wide;
+ * returns the previous mode): 0 automatic -- the precision form (BF16 first
+ * pass, exact FP32 re-rank, certificate, fallback; the FP32 form's results
+ * bit for bit) for unweighted searches of at least 256 queries and 2^26
+ * pairs on finite data, k <= 512 --, 1 the FP32 form only, 2 the precision
+ * form whenever it applies.  No counterpart in the reference. */
+int yael_knn_precision(int mode);
 int yael_knn_full_checked(int distance_type, int nq, int nb, int d, int k, const float *b,
                           const float *q, const float *b_weights, int *assign, float *dis);
 int yael_cross_distances_checked(int distance_type, int d, int na, int nb, const float *a,

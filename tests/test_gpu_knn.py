@@ -201,3 +201,44 @@ def test_pair_dist_f64_bitexact(ya, oracle):
     ref = oracle.pair_dist_f64(2, b, q, idx)
     assert np.array_equal(np.isnan(out), np.isnan(ref))
     assert np.array_equal(out[~np.isnan(out)], ref[~np.isnan(ref)])
+
+
+def _both_forms(ya, q, b, k, typ=2):
+    """knn_full with the precision form forced, then with the FP32 form only."""
+    lib = ya.lib()
+    old = lib.yael_knn_precision(2)
+    try:
+        i16, d16 = ya.knn(q, b, k, distance_type=typ)
+        lib.yael_knn_precision(1)
+        i32, d32 = ya.knn(q, b, k, distance_type=typ)
+    finally:
+        lib.yael_knn_precision(old)
+    return i16, d16, i32, d32
+
+
+@pytest.mark.parametrize("k,typ", [(1, 2), (10, 2), (100, 2), (7, 16)])
+def test_knn_full_precision_form_same_bits(ya, k, typ):
+    """knn_full through the precision form gives the FP32 form's bits."""
+    rng = np.random.default_rng(100 + k)
+    cen = rng.uniform(0, 255, (64, 48))
+    b = np.clip(np.rint(cen[rng.integers(0, 64, 30000)] + 20 * rng.standard_normal((30000, 48))),
+                0, 255).astype(np.float32)
+    q = rng.standard_normal((300, 48)).astype(np.float32) * 30 + 120
+    print("knn_full both forms k=%d type=%d" % (k, typ), flush=True)
+    i16, d16, i32, d32 = _both_forms(ya, q, b, k, typ)
+    assert np.array_equal(i16, i32)
+    assert np.array_equal(d16.view(np.int32), d32.view(np.int32))
+
+
+def test_knn_full_precision_form_nan_data(ya):
+    """NaN values: the set statistics show them and knn_full keeps the FP32
+    form (same bits, NaN rows never returned)."""
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal((5000, 24)).astype(np.float32)
+    b[17, 3] = np.nan
+    q = rng.standard_normal((300, 24)).astype(np.float32)
+    q[5, 0] = np.nan
+    i16, d16, i32, d32 = _both_forms(ya, q, b, 12)
+    assert np.array_equal(i16, i32)
+    assert np.array_equal(d16.view(np.int32), d32.view(np.int32))
+    assert not np.any(i16 == 17)

@@ -18,6 +18,7 @@ PROTOS = {
     "nn_thread": (C.c_double, [I, I, I, vp, vp, vp, I]),
     "knn": (vp, [I, I, I, I, vp, vp, vp]),
     "knn_thread": (vp, [I, I, I, I, vp, vp, vp, I]),
+    "yael_knn_precision": (I, [I]),
     # hkm.h
     "hkm_learn": (vp, [I, I, I, I, vp, I, I, I, vp]),
     "hkm_delete": (None, [vp]),

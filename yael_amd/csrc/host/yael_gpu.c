@@ -83,6 +83,51 @@ static float *yh_upload(int slot, const float *src, long n, int d, long lds) {
 /* ------------------------------------------------------------------------ */
 /* kNN                                                                       */
 
+/* kNN form: 0 = automatic (the precision form for large unweighted searches
+ * on finite data), 1 = FP32 form only, 2 = the precision form whenever it
+ * applies (unweighted, finite data, k <= YK_KMAX / 2). */
+static int g_knn_precision = 0;
+
+int yael_knn_precision(int mode) {
+  const int old = g_knn_precision;
+  if (mode >= 0 && mode <= 2) g_knn_precision = mode;
+  return old;
+}
+
+/* The precision form (BF16 first pass, FP32 re-rank, certificate,
+ * escalation, FP32 fallback) gives the FP32 form's results bit for bit; it
+ * is chosen where its fixed costs (BF16 copies, sketch) are small against
+ * the scan: at least 256 queries and 2^26 pairs. */
+#define YH_KNN_BF16_MIN_NQ 256
+#define YH_KNN_BF16_MIN_PAIRS (1L << 26)
+
+static int yh_knn_bf16(int type, int nq, int nb, int d, int k, float *db, float *dq, float *bn,
+                       double *qn, int *da, float *dd, int *used) {
+  *used = 0;
+  uint16_t *bh = (uint16_t *)yh_buf(YH_BH, (size_t)yk_bf16_bytes(nb, d, type, 0));
+  uint16_t *qh = (uint16_t *)yh_buf(YH_QH, (size_t)yk_bf16_bytes(nq, d, type, 1));
+  float *bst = (float *)yh_buf(YH_BST, 16 * (size_t)nb);
+  float *qst = (float *)yh_buf(YH_QST, 16 * (size_t)nq);
+  float *mx = (float *)yh_buf(YH_MX, 64);
+  if (!bh || !qh || !bst || !qst || !mx) return YK_ENOMEM;
+  /* the base's conversion also writes its float norms */
+  int rc = yk_to_bf16(db, nb, d, type, 0, bh, bst, mx, type == 2 ? bn : NULL);
+  if (rc == YK_OK) rc = yk_to_bf16(dq, nq, d, type, 1, qh, qst, mx + 4, NULL);
+  float hm[8];
+  if (rc == YK_OK) rc = yk_memcpy_d2h(hm, mx, sizeof hm);
+  if (rc != YK_OK) return rc;
+  for (int i = 0; i < 8; i++)
+    if (!isfinite(hm[i])) return YK_OK; /* non-finite values: the FP32 form */
+  if (type == 2 && (rc = yk_norms(dq, nq, d, NULL, qn)) != YK_OK) return rc;
+  int kp = 2 * k < 32 ? 32 : 2 * k;
+  if (kp > YK_KMAX) kp = YK_KMAX;
+  if (kp > nb) kp = nb;
+  int nfb = 0;
+  rc = yk_knn_bf16(type, nq, nb, d, k, kp, dq, qn, qh, qst, db, bn, bh, mx, da, dd, &nfb);
+  if (rc == YK_OK) *used = 1;
+  return rc;
+}
+
 /* nn.c:451-525 semantics on the GPU; assign/dis are k x nq. */
 int yh_knn(int type, int nq, int nb, int d, int k, const float *b, const float *q,
            const float *w, int *assign, float *dis) {
@@ -103,6 +148,25 @@ int yh_knn(int type, int nq, int nb, int d, int k, const float *b, const float *
     bn = (float *)yh_buf(YH_BN, sizeof(float) * yh_pad32(nb));
     qn = (double *)yh_buf(YH_QN, sizeof(double) * nq);
     if (!bn || !qn) return YK_ENOMEM;
+  }
+  const int want_bf16 =
+      !w && k <= YK_KMAX / 2 &&
+      (g_knn_precision == 2 ||
+       (g_knn_precision == 0 && nq >= YH_KNN_BF16_MIN_NQ && (long)nq * nb >= YH_KNN_BF16_MIN_PAIRS));
+  if (want_bf16) {
+    int *da = (int *)yh_buf(YH_OUTI, sizeof(int) * (size_t)nq * k);
+    float *dd = (float *)yh_buf(YH_OUTD, sizeof(float) * (size_t)nq * k);
+    if (!da || !dd) return YK_ENOMEM;
+    int used = 0;
+    int rc = yh_knn_bf16(type, nq, nb, d, k, db, dq, bn, qn, da, dd, &used);
+    if (rc != YK_OK) return rc;
+    if (used) {
+      rc = yk_memcpy_d2h(assign, da, sizeof(int) * (size_t)nq * k);
+      if (rc == YK_OK) rc = yk_memcpy_d2h(dis, dd, sizeof(float) * (size_t)nq * k);
+      return rc;
+    }
+  }
+  if (type == 2) {
     int rc = yk_norms(db, nb, d, bn, NULL);
     if (rc == YK_OK) rc = yk_norms(dq, nq, d, NULL, qn);
     if (rc != YK_OK) return rc;

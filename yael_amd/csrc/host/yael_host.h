@@ -9,6 +9,7 @@ enum {
   YH_B = 0, YH_Q, YH_BN, YH_QN, YH_W, YH_OUTI, YH_OUTD,
   YH_KM_V, YH_KM_QN, YH_KM_C, YH_KM_CN, YH_KM_A, YH_KM_D, YH_KM_CNT, YH_KM_WS,
   YH_KM_BEST_C, YH_KM_SEL, YH_KM_VH, YH_KM_QS, YH_KM_CH, YH_KM_CS,
+  YH_BH, YH_QH, YH_BST, YH_QST, YH_MX,
   YH_NSLOTS
 };
 

@@ -77,7 +77,8 @@ int yk_knn_f32(int type, long nq, long nb, int d, int k, const float *q,
  * (row stride yk_bf16_ld(d, type) uint16; for L2 the squared norm is appended
  * in three BF16 pieces, queries are stored as -2q) and per-vector statistics
  * stats[4*i..] = {|x|, |bf(x)|, |x - bf(x)|, |x|^2}; stats_max (4 floats,
- * optional) receives their maxima over the set; fnorm (base sets only,
+ * optional) receives their maxima over the set (NaN if any value is NaN);
+ * fnorm (base sets only,
  * optional) the float norms exactly as yk_norms writes them.
  * yk_knn_bf16: BF16 MFMA first pass keeps the kp best candidates per query,
  * re-ranks them with the FP32 form's arithmetic (bit-identical values) and

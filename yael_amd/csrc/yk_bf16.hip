@@ -78,6 +78,12 @@ extern "C" long yk_bf16_bytes(long n, int d, int type, int is_query) {
 // statistics stats[i] = {|x|, |bf(x)|, |x - bf(x)|, |x|^2} (float); the
 // per-set maxima go to smax (uint bit patterns of non-negative floats, whose
 // integer order is their float order) by atomicMax.
+// max that keeps a NaN (fmaxf drops it): the set maxima of the statistics
+// then show a NaN anywhere in the set
+__device__ __forceinline__ float yk_fmaxn(float a, float b) {
+  return a != a ? a : (b != b ? b : fmaxf(a, b));
+}
+
 __global__ __launch_bounds__(256) void yk_to_bf16_kernel(const float *__restrict__ x, long n,
                                                          int d, int ld, int ldh, int type,
                                                          int is_query,
@@ -126,14 +132,14 @@ __global__ __launch_bounds__(256) void yk_to_bf16_kernel(const float *__restrict
   for (int j = d + lane; j < ldh; j += 64) o[j] = (j - d) < extra ? tail[j - d] : 0;
   float4 s = make_float4((float)sqrt(s2), (float)sqrt(sb), (float)sqrt(se), (float)s2);
   if (lane == 0) stats[i] = s;
-  mx = make_float4(fmaxf(mx.x, s.x), fmaxf(mx.y, s.y), fmaxf(mx.z, s.z), fmaxf(mx.w, s.w));
+  mx = make_float4(yk_fmaxn(mx.x, s.x), yk_fmaxn(mx.y, s.y), yk_fmaxn(mx.z, s.z), yk_fmaxn(mx.w, s.w));
   }  // rows
   if (lane == 0) wmax[wave] = mx;
   __syncthreads();
   if (smax && threadIdx.x == 0) {
     for (int w = 1; w < 4; w++)
-      mx = make_float4(fmaxf(mx.x, wmax[w].x), fmaxf(mx.y, wmax[w].y), fmaxf(mx.z, wmax[w].z),
-                       fmaxf(mx.w, wmax[w].w));
+      mx = make_float4(yk_fmaxn(mx.x, wmax[w].x), yk_fmaxn(mx.y, wmax[w].y), yk_fmaxn(mx.z, wmax[w].z),
+                       yk_fmaxn(mx.w, wmax[w].w));
     atomicMax(smax + 0, __float_as_uint(mx.x));
     atomicMax(smax + 1, __float_as_uint(mx.y));
     atomicMax(smax + 2, __float_as_uint(mx.z));
@@ -257,21 +263,21 @@ __global__ __launch_bounds__(256) void yk_to_bf16_frag_kernel(const float *__res
         stats[row] = st;
         if (fnorm) fnorm[row] = sf;
       }
-      mx = make_float4(fmaxf(mx.x, st.x), fmaxf(mx.y, st.y), fmaxf(mx.z, st.z), fmaxf(mx.w, st.w));
+      mx = make_float4(yk_fmaxn(mx.x, st.x), yk_fmaxn(mx.y, st.y), yk_fmaxn(mx.z, st.z), yk_fmaxn(mx.w, st.w));
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
-    mx.x = fmaxf(mx.x, __shfl_xor(mx.x, o));
-    mx.y = fmaxf(mx.y, __shfl_xor(mx.y, o));
-    mx.z = fmaxf(mx.z, __shfl_xor(mx.z, o));
-    mx.w = fmaxf(mx.w, __shfl_xor(mx.w, o));
+    mx.x = yk_fmaxn(mx.x, __shfl_xor(mx.x, o));
+    mx.y = yk_fmaxn(mx.y, __shfl_xor(mx.y, o));
+    mx.z = yk_fmaxn(mx.z, __shfl_xor(mx.z, o));
+    mx.w = yk_fmaxn(mx.w, __shfl_xor(mx.w, o));
   }
   if (lane == 0) wmax[wave] = mx;
   __syncthreads();
   if (smax && threadIdx.x == 0) {
     for (int w = 1; w < 4; w++)
-      mx = make_float4(fmaxf(mx.x, wmax[w].x), fmaxf(mx.y, wmax[w].y), fmaxf(mx.z, wmax[w].z),
-                       fmaxf(mx.w, wmax[w].w));
+      mx = make_float4(yk_fmaxn(mx.x, wmax[w].x), yk_fmaxn(mx.y, wmax[w].y), yk_fmaxn(mx.z, wmax[w].z),
+                       yk_fmaxn(mx.w, wmax[w].w));
     atomicMax(smax + 0, __float_as_uint(mx.x));
     atomicMax(smax + 1, __float_as_uint(mx.y));
     atomicMax(smax + 2, __float_as_uint(mx.z));
