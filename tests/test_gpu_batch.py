@@ -149,3 +149,35 @@ def test_pq_subspaces_match_single_runs(ya):
     sub = np.concatenate([v[:, j * dsub:(j + 1) * dsub] for j in range(nsub)])
     sizes = np.full(nsub, n, np.int32)
     compare((cent, ret, dis, asg, nas), single(ya, sub, sizes, k, niter, seeds), sizes)
+
+
+@pytest.mark.parametrize("dsub", [6, 16, 32])
+def test_pq_subspaces_large_instances(ya, dsub):
+    """Instances of 40000 points: the update's partial sums take one wave per
+    float4 column of the sub-vectors (dsub 16, 32) or the one-wave path
+    (dsub 6, not a multiple of 4)."""
+    import ctypes
+    from yael_amd import device
+    rng = np.random.default_rng(40 + dsub)
+    n, nsub, k, niter = 40000, 2, 64, 6
+    v = mixture(rng, n, nsub * dsub, 120, 20)
+    V = device.DeviceSet(v)
+    cent = np.empty((nsub, k, dsub), np.float32)
+    dis = np.empty(nsub * n, np.float32)
+    asg = np.empty(nsub * n, np.int32)
+    nas = np.empty((nsub, k), np.int32)
+    ret = np.empty(nsub, np.float32)
+    it = np.empty(nsub, np.int32)
+    chunk = ctypes.c_int(0)
+    seeds = np.arange(7, 7 + nsub, dtype=np.int64)
+    print("n", n, "dsub", dsub, flush=True)
+    rc = ya.lib().yael_kmeans_subspaces_device(nsub * dsub, n, nsub, k, niter, V.vec.ptr, 0x10000,
+                                               seeds.ctypes.data, cent.ctypes.data,
+                                               dis.ctypes.data, asg.ctypes.data, nas.ctypes.data,
+                                               ret.ctypes.data, it.ctypes.data,
+                                               ctypes.byref(chunk))
+    ya.check(rc, "subspaces")
+    V.free()
+    sub = np.concatenate([v[:, j * dsub:(j + 1) * dsub] for j in range(nsub)])
+    sizes = np.full(nsub, n, np.int32)
+    compare((cent, ret, dis, asg, nas), single(ya, sub, sizes, k, niter, seeds), sizes)

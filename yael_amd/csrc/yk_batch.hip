@@ -9,11 +9,11 @@
 //   assign                            yk_batch_assign (yk_select.hip): the
 //                                     FP32 scan kernel with a work-item ->
 //                                     (instance, 128-point tile) table
-//   update, partial sums              yk_bupd_partial_kernel: one wave per
-//                                     (instance, chunk of P points) adds the
-//                                     chunk's points in point order into an
-//                                     LDS FP64 table [k][d] (ds_add_f64),
-//                                     and sums the chunk's objective
+//   update, partial sums              yk_bupd_partial_kernel: one workgroup
+//                                     per (instance, chunk of P points) adds
+//                                     the chunk's points in point order into
+//                                     an LDS FP64 table [k][d], and sums the
+//                                     chunk's objective
 //   update, reduce + scale            yk_bupd_reduce_kernel: chunks summed in
 //                                     chunk order, (float)(sum * (1/count))
 //   empty-cluster split               yk_bsplit_kernel: kmeans.c:166-209 on
@@ -59,25 +59,37 @@ struct BatchDev {
   int *it;  // iteration counter
 };
 
+// Row stride (doubles) of the LDS table: d < 64 with d a multiple of 4
+// pads the rows to an odd length so that the lanes of one instruction,
+// each on its own cluster's row, spread over the LDS banks.
+__host__ __device__ static inline int yk_bupd_stride(int d) {
+  return (d < 64 && (d & 3) == 0) ? d + 1 : d;
+}
+
 // One workgroup per update chunk adds the chunk's points into an LDS FP64
 // table [k][d].  d >= 64: one thread per dimension (up to 4 waves), each
 // (cluster, dimension) sum owned by one thread and accumulated in point
-// order.  d < 64: one wave whose lanes split into 64/d groups of d, 64/d
-// points per pass, with ds_add_f64: LDS atomics of one wave apply in
-// instruction order and, inside one instruction, in lane order, so the
-// sums are again in point order.  Deterministic, whatever the grid.  The
-// chunk's objective (its points' distances, FP64) goes to pobj[u].
+// order.  d < 64, d a multiple of 4: wave w owns the float4 columns
+// w, w + 4, ... of the rows (no two waves share a sum), a lane takes one
+// point per step, four steps in flight; d < 64 otherwise: one wave whose
+// lanes split into 64/d groups of d.  Both use ds_add_f64: the LDS atomics
+// of one wave apply in instruction order and, inside one instruction, in a
+// fixed order, so every sum is accumulated in point order.  Deterministic,
+// whatever the grid.  The chunk's objective (its points' distances, FP64,
+// thread-strided then a fixed tree) goes to pobj[u].
 __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
-  extern __shared__ double lsum[];  // k x d
-  int *lcnt = (int *)(lsum + (size_t)b.k * b.d);
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  extern __shared__ double lsum[];  // k x sd
+  __shared__ double wobj[4];
+  const int d = b.d, ld = b.ld, sd = yk_bupd_stride(d);
+  int *lcnt = (int *)(lsum + (size_t)b.k * sd);
+  const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wave = tid >> 6;
   const long u = blockIdx.x;
   const int inst = b.uitem_inst[u];
   if (b.done[inst]) return;
   const long off = b.inst_off[inst], n = b.inst_off[inst + 1] - off;
   const long c0 = (u - b.uitem_first[inst]) * (long)b.chunk;
   const long p0 = off + c0, p1 = off + (c0 + b.chunk < n ? c0 + b.chunk : n);
-  for (int i = tid; i < b.k * b.d; i += nthr) lsum[i] = 0.0;
+  for (int i = tid; i < b.k * sd; i += nthr) lsum[i] = 0.0;
   for (int i = tid; i < b.k; i += nthr) lcnt[i] = 0;
   // a point without a nearest centroid (NaN input) has assign -1: the
   // instance is handed to the single-instance path, which reports it
@@ -87,13 +99,12 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
     if (tid == 0) b.overflow[inst] = 1;
     return;
   }
-  if (tid < 64) {  // the chunk objective, one wave, fixed order
+  {  // the chunk objective
     double obj = 0.0;
-    for (long p = p0 + tid; p < p1; p += 64) obj += (double)b.dis[p];
+    for (long p = p0 + tid; p < p1; p += nthr) obj += (double)b.dis[p];
     for (int o = 32; o > 0; o >>= 1) obj += __shfl_xor(obj, o);
-    if (tid == 0) b.pobj[u] = obj;
+    if (lane == 0) wobj[wave] = obj;
   }
-  const int d = b.d, ld = b.ld;
   if (d >= 64) {
     // a thread owns its dimensions: plain read-add-write, four points in
     // flight
@@ -121,9 +132,36 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
         if (t == 0) lcnt[a]++;
       }
     }
+  } else if ((d & 3) == 0) {
+    const int nwave = nthr >> 6;
+    for (int col = wave; col < (d >> 2); col += nwave) {
+      const int j0 = col * 4;
+      for (long p = p0 + lane; p < p1; p += 256) {
+        int a[4];
+        float4 x[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          const long q = p + 64 * s;
+          a[s] = q < p1 ? b.assign[q] : -1;
+          x[s] = q < p1 ? *(const float4 *)(b.v + q * ld + j0) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          if (a[s] < 0) continue;
+          double *row = lsum + a[s] * sd + j0;
+          __hip_atomic_fetch_add(row, (double)x[s].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(row + 1, (double)x[s].y, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(row + 2, (double)x[s].z, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(row + 3, (double)x[s].w, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (col == 0)
+            __hip_atomic_fetch_add(&lcnt[a[s]], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+    }
   } else {
-    // one wave: lanes in 64/d groups of d, LDS atomics (same address from
-    // two groups of one instruction: applied in lane order)
     const int ppp = 64 / d, g = tid / d, t = tid - g * d;
     if (g < ppp) {
       for (long p = p0 + g; p < p1; p += ppp) {
@@ -136,8 +174,13 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
     }
   }
   __syncthreads();
+  if (tid == 0) {
+    double obj = 0.0;
+    for (int w = 0; w < (nthr >> 6); w++) obj += wobj[w];
+    b.pobj[u] = obj;
+  }
   double *ps = b.psum + u * (size_t)b.k * b.d;
-  for (int i = tid; i < b.k * b.d; i += nthr) ps[i] = lsum[i];
+  for (int i = tid; i < b.k * d; i += nthr) ps[i] = lsum[(i / d) * sd + i % d];
   for (int i = tid; i < b.k; i += nthr) b.pcnt[u * b.k + i] = lcnt[i];
 }
 
@@ -290,7 +333,7 @@ struct yk_batch {
 };
 
 extern "C" size_t yk_batch_lds_bytes(int k, int d) {
-  return sizeof(double) * (size_t)k * d + sizeof(int) * (size_t)k;
+  return sizeof(double) * (size_t)k * yk_bupd_stride(d) + sizeof(int) * (size_t)k;
 }
 
 // The device state of one batched run.  All pointers are device pointers
@@ -343,8 +386,10 @@ static int yk_batch_enqueue(yk_batch *bt) {
   YK_TRY(yk_norms(b.cent, (long)b.ninst * b.kpad, b.d, b.cn, nullptr));
   YK_TRY(yk_batch_assign(b.nitems, b.item_inst, b.item_tile, b.inst_off, b.d, b.k, b.kpad, b.v,
                          b.vn, b.cent, b.cn, b.done, b.assign, b.dis));
-  // d < 64: one wave; else one thread per dimension (at most 256)
-  const int pthreads = b.d < 64 ? 64 : (b.d > 256 ? 256 : ((b.d + 63) / 64) * 64);
+  // d < 64: a wave per float4 column (at most 4), or one wave when d is not
+  // a multiple of 4; else one thread per dimension (at most 256)
+  const int pthreads = b.d < 64 ? ((b.d & 3) == 0 ? 64 * (b.d / 4 < 4 ? b.d / 4 : 4) : 64)
+                                : (b.d > 256 ? 256 : ((b.d + 63) / 64) * 64);
   hipLaunchKernelGGL(yk_bupd_partial_kernel, dim3((unsigned)b.nuitems), dim3(pthreads),
                      yk_batch_lds_bytes(b.k, b.d), st, b);
   hipLaunchKernelGGL(yk_bupd_reduce_kernel, dim3((unsigned)(((long)b.ninst * b.k + 3) / 4)),

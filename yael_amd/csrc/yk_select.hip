@@ -109,6 +109,45 @@ __device__ __forceinline__ void yk_reg_insert(uint64_t (&t)[R], uint64_t x) {
   t[0] = x < t[0] ? x : t[0];
 }
 
+// One full block's epilogue for k = 1 and the sketch (each lane keeps the best
+// of its own rows), unweighted.  The lane's 16 values are tested first on
+// the float approximation qn + (bn - 2 acc), which differs from the exact
+// expression by far less than slack = 2^-19 (|q|^2 + max|b|^2)
+// (|2 q.b| <= |q|^2 + |b|^2); x -> fl(qn + x) is monotone, so the query
+// norm is added once, to the minimum of bn - 2 acc.  A block that may hold
+// a better value takes the exact FP64-normed expression for all 16 values,
+// branch-free, in ascending row order with a strict '<' (the smallest index
+// wins a tie).
+template <bool DOT>
+__device__ __forceinline__ void yk_best1_block(const f32x16 &acc, const float (&nn)[16],
+                                               double qnd, float nmax, long bb, int h,
+                                               float &bv, int &bi) {
+  const float qf = (float)qnd;
+  float mm = DOT ? acc[0] : __builtin_fmaf(-2.0f, acc[0], nn[0]);
+#pragma unroll
+  for (int i = 1; i < 16; i++) mm = fminf(mm, DOT ? acc[i] : __builtin_fmaf(-2.0f, acc[i], nn[i]));
+  const float m = DOT ? mm : qf + mm;
+  const float slack = DOT ? 0.f : 0x1p-19f * (qf + nmax);
+  if (!(m - slack <= bv) && m == m) return;
+  // empty asm statements on the operands keep the exact pass below the
+  // test (the compiler would otherwise hoist its parts shared with the
+  // general path above the branch)
+  double qd = qnd;
+  asm volatile("" : "+v"(qd));
+  // value i sits at row bb + 4 h + 8 (i >> 2) + (i & 3)
+  int bo = -1;
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    float ax = acc[i], nx = nn[i];
+    asm volatile("" : "+v"(ax), "+v"(nx));
+    const float v = DOT ? ax : (float)(qd + (double)nx) - 2.0f * ax;
+    const bool take = v < bv;
+    bv = take ? v : bv;
+    bo = take ? 8 * (i >> 2) + (i & 3) : bo;
+  }
+  if (bo >= 0) bi = (int)bb + 4 * h + bo;
+}
+
 // NCH > 0: queries register-resident, exactly NCH 16-wide chunks (ld = 16
 // NCH, so every chunk loop is unrolled without guards); NCH = 0: both
 // operands streamed from memory.
@@ -220,7 +259,16 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
   // item's rows within 2^31 bytes.
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
       (void *)(a.b + bbeg * ld), 0, (int)min((a.nb - bbeg) * ld * 4, 0x7fffffffL), YK_BUF_DWORD3);
+  // With one or two chunks a block is only 8-16 MFMAs: its base norms
+  // (rows 8g + 4h .. +3) are loaded one block ahead as well, so that the
+  // epilogue does not wait for them -- and, with them, for the next block's
+  // operands issued meanwhile.
+  // Weighted searches are served by the other variants (the launcher's
+  // choice), so these kernels compile without the weights' loads.
+  constexpr bool NPF = QRES && NCH <= 2;
+  const float *const wts = NPF ? nullptr : a.w;
   float4 anext[QRES ? 2 * NCH : 1];
+  float4 nnext[NPF ? 4 : 1];
   auto loadb = [&](long bb) {
     const int soff = (int)((bb - bbeg) * ld * 4);
     const int voff = ((int)min((long)r, a.nb - 1 - bb) * ld + h * 8) * 4;
@@ -230,6 +278,11 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
           float4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff + c * 64, 0));
       anext[2 * c + 1] = __builtin_bit_cast(
           float4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff + 16, soff + c * 64, 0));
+    }
+    if constexpr (NPF) {
+      if (!dot)
+#pragma unroll
+        for (int g = 0; g < 4; g++) nnext[g] = *(const float4 *)(a.bn + bb + 8 * g + 4 * h);
     }
   };
   if (QRES && bbeg < bend) loadb(bbeg);
@@ -242,10 +295,14 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
 #pragma unroll
       for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
 
+    float4 ncur[NPF ? 4 : 1];
     if (QRES) {
       float4 av[QRES ? 2 * NCH : 1];
 #pragma unroll
       for (int c = 0; c < 2 * NCH; c++) av[c] = anext[c];
+      if constexpr (NPF)
+#pragma unroll
+        for (int g = 0; g < 4; g++) ncur[g] = nnext[g];
       {
         // unconditional (past the range it reloads the current block): a
         // conditional prefetch joins paths with different outstanding-load
@@ -295,8 +352,9 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
 #pragma unroll
     for (int g = 0; g < 4; g++) {
       const long b4 = bb + 8 * g + 4 * h;
-      float4 n4 = dot ? make_float4(0.f, 0.f, 0.f, 0.f) : *(const float4 *)(a.bn + b4);
-      float4 w4 = a.w ? *(const float4 *)(a.w + b4) : make_float4(1.f, 1.f, 1.f, 1.f);
+      float4 n4 = dot ? make_float4(0.f, 0.f, 0.f, 0.f)
+                      : (NPF ? ncur[NPF ? g : 0] : *(const float4 *)(a.bn + b4));
+      float4 w4 = wts ? *(const float4 *)(wts + b4) : make_float4(1.f, 1.f, 1.f, 1.f);
       nn[4 * g] = n4.x; nn[4 * g + 1] = n4.y; nn[4 * g + 2] = n4.z; nn[4 * g + 3] = n4.w;
       ww[4 * g] = w4.x; ww[4 * g + 1] = w4.y; ww[4 * g + 2] = w4.z; ww[4 * g + 3] = w4.w;
     }
@@ -305,6 +363,17 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
     for (int i = 0; i < 16; i++) nmax = fmaxf(nmax, nn[i]);
 #pragma unroll
     for (int j = 0; j < NB; j++) {
+      if constexpr (MODE == 0 || MODE == 1) {
+        // full blocks of unweighted searches (the last, partial block of a
+        // range and weighted searches take the general path below)
+        if (!wts && bb + 32 <= bend) {
+          if (dot)
+            yk_best1_block<true>(acc[j], nn, qnd[j], nmax, bb, h, bv[j], bi[j]);
+          else
+            yk_best1_block<false>(acc[j], nn, qnd[j], nmax, bb, h, bv[j], bi[j]);
+          continue;
+        }
+      }
       // Prefilter on float approximations qn + (bn - 2 acc), which differ
       // from the exact expression by far less than 2^-19 (|q|^2 + max|b|^2)
       // (|2 q.b| <= |q|^2 + |b|^2): a min over the lane's 16 values, then a
@@ -317,7 +386,7 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
       for (int i = 0; i < 16; i++)
         ap[i] = dot ? acc[j][i] : qf + __builtin_fmaf(-2.0f, acc[j][i], nn[i]);
       const float lim = MODE == 3 ? Tv[j] : (MODE == 2 ? INFINITY : bv[j]);
-      if (!a.w && MODE != 2) {
+      if (!wts && MODE != 2) {
         float m = ap[0];
 #pragma unroll
         for (int i = 1; i < 16; i++) m = fminf(m, ap[i]);
@@ -325,12 +394,12 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
       }
 #pragma unroll
       for (int i = 0; i < 16; i++) {
-        if (!a.w && MODE != 2 && !(ap[i] - slack <= (MODE == 3 ? Tv[j] : bv[j])) && ap[i] == ap[i])
+        if (!wts && MODE != 2 && !(ap[i] - slack <= (MODE == 3 ? Tv[j] : bv[j])) && ap[i] == ap[i])
           continue;
         const long bidx = bb + 8 * (i >> 2) + 4 * h + (i & 3);
         const bool bok = bidx < bend;
         float v = dot ? acc[j][i] : (float)(qnd[j] + (double)nn[i]) - 2.0f * acc[j][i];
-        if (a.w) v *= ww[i];
+        if (wts) v *= ww[i];
         if (MODE == 0 || MODE == 1) {
           if (bok && v < bv[j]) {  // strict '<' in ascending index order
             bv[j] = v;
@@ -571,7 +640,8 @@ static int yk_scratch(int slot, size_t bytes, void **out) {
 // the kernel with exactly ld/16 register-resident chunks (ld <= 128), else
 // the one streaming both operands
 template <int MODE, int R>
-static void (*yk_f32_pick(int ld))(ScanArgs) {
+static void (*yk_f32_pick(int ld, bool weighted))(ScanArgs) {
+  if (weighted && ld <= 32) return yk_scan_f32<1, 0, MODE, R>;  // no weights in NCH <= 2
   switch (ld >> 4) {
     case 1: return yk_scan_f32<1, 1, MODE, R>;
     case 2: return yk_scan_f32<1, 2, MODE, R>;
@@ -594,7 +664,7 @@ static int yk_launch_scan(const YkPlan &p, ScanArgs a, bool qres) {
   a.total = p.tiles * p.nsplit;
   a.per_xcd = (a.total + 7) / 8;
   (void)qres;
-  auto kern = yk_f32_pick<MODE, R>(a.ld);
+  auto kern = yk_f32_pick<MODE, R>(a.ld, a.w != nullptr);
   const int per_xcd_res = yk_resident_wgs((const void *)kern, 64 * p.waves) / 8;
   dim3 grid(8 * (a.per_xcd < per_xcd_res ? a.per_xcd : per_xcd_res)), block(64 * p.waves);
   int tok = yk_prof_begin(MODE == 1 ? YK_PROF_F32_SKETCH : YK_PROF_FUSED);
@@ -766,7 +836,7 @@ extern "C" int yk_batch_assign(long nitems, const int *item_inst, const int *ite
   a.done = done;
   a.total = (int)nitems;
   a.per_xcd = (a.total + 7) / 8;
-  auto kern = yk_f32_pick<0, 1>(a.ld);
+  auto kern = yk_f32_pick<0, 1>(a.ld, false);
   const int per_res = yk_resident_wgs((const void *)kern, 256) / 8;
   dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), block(256);
   hipLaunchKernelGGL(kern, grid, block, 0, yk_stream(), a);
