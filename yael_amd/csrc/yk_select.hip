@@ -117,21 +117,31 @@ __device__ __forceinline__ void yk_reg_insert(uint64_t (&t)[R], uint64_t x) {
 // norm is added once, to the minimum of bn - 2 acc.  A block that may hold
 // a better value takes the exact FP64-normed expression for all 16 values,
 // branch-free, in ascending row order with a strict '<' (the smallest index
-// wins a tie).
+// wins a tie).  The test is skipped (test = false) for a range's first
+// blocks: a wave runs the exact pass when any of its 64 lanes may improve,
+// which at block j of a range happens with probability about
+// 1 - (1 - 1/(j+1))^64, so early blocks would pay for both.
 template <bool DOT>
 __device__ __forceinline__ void yk_best1_block(const f32x16 &acc, const float (&nn)[16],
-                                               double qnd, float nmax, long bb, int h,
-                                               float &bv, int &bi) {
-  const float qf = (float)qnd;
-  float mm = DOT ? acc[0] : __builtin_fmaf(-2.0f, acc[0], nn[0]);
+                                               double qnd, long bb, int h, bool test, float &bv,
+                                               int &bi) {
+  // Empty asm statements on the operands keep each pass inside its branch
+  // (the compiler would otherwise hoist parts it shares with the general
+  // path, executing them for every block).
+  if (test) {
+    const float qf = (float)qnd;
+    float mm = INFINITY, nmax = 0.f;
 #pragma unroll
-  for (int i = 1; i < 16; i++) mm = fminf(mm, DOT ? acc[i] : __builtin_fmaf(-2.0f, acc[i], nn[i]));
-  const float m = DOT ? mm : qf + mm;
-  const float slack = DOT ? 0.f : 0x1p-19f * (qf + nmax);
-  if (!(m - slack <= bv) && m == m) return;
-  // empty asm statements on the operands keep the exact pass below the
-  // test (the compiler would otherwise hoist its parts shared with the
-  // general path above the branch)
+    for (int i = 0; i < 16; i++) {
+      float ax = acc[i], nx = nn[i];
+      asm volatile("" : "+v"(ax), "+v"(nx));
+      mm = fminf(mm, DOT ? ax : __builtin_fmaf(-2.0f, ax, nx));
+      nmax = fmaxf(nmax, nx);
+    }
+    const float m = DOT ? mm : qf + mm;
+    const float slack = DOT ? 0.f : 0x1p-19f * (qf + nmax);
+    if (!(m - slack <= bv) && m == m) return;
+  }
   double qd = qnd;
   asm volatile("" : "+v"(qd));
   // value i sits at row bb + 4 h + 8 (i >> 2) + (i & 3)
@@ -358,19 +368,17 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
       nn[4 * g] = n4.x; nn[4 * g + 1] = n4.y; nn[4 * g + 2] = n4.z; nn[4 * g + 3] = n4.w;
       ww[4 * g] = w4.x; ww[4 * g + 1] = w4.y; ww[4 * g + 2] = w4.z; ww[4 * g + 3] = w4.w;
     }
-    float nmax = 0.f;
-#pragma unroll
-    for (int i = 0; i < 16; i++) nmax = fmaxf(nmax, nn[i]);
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       if constexpr (MODE == 0 || MODE == 1) {
         // full blocks of unweighted searches (the last, partial block of a
         // range and weighted searches take the general path below)
         if (!wts && bb + 32 <= bend) {
+          const bool test = bb - bbeg >= 48 * 32;
           if (dot)
-            yk_best1_block<true>(acc[j], nn, qnd[j], nmax, bb, h, bv[j], bi[j]);
+            yk_best1_block<true>(acc[j], nn, qnd[j], bb, h, test, bv[j], bi[j]);
           else
-            yk_best1_block<false>(acc[j], nn, qnd[j], nmax, bb, h, bv[j], bi[j]);
+            yk_best1_block<false>(acc[j], nn, qnd[j], bb, h, test, bv[j], bi[j]);
           continue;
         }
       }
@@ -379,6 +387,9 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
       // (|2 q.b| <= |q|^2 + |b|^2): a min over the lane's 16 values, then a
       // per-value test; only values that may matter take the exact
       // FP64-normed expression.  Weighted searches evaluate every value.
+      float nmax = 0.f;
+#pragma unroll
+      for (int i = 0; i < 16; i++) nmax = fmaxf(nmax, nn[i]);
       const float qf = (float)qnd[j];
       const float slack = dot ? 0.f : 0x1p-19f * (qf + nmax);
       float ap[16];
