@@ -183,15 +183,18 @@ __global__ __launch_bounds__(256) void yk_to_bf16_frag_kernel(const float *__res
     // the lane's 8 values of chunk c (columns past d read as zero; the FP32
     // rows are padded to ld, a multiple of 16, so full 8-column runs below
     // d are two aligned float4 loads)
+    // (a uniform test per chunk; rows past n load row 0 and are zeroed by
+    // the caller, so that the loads of full chunks carry no per-lane
+    // condition)
     auto vals = [&](int c, float (&v)[8]) {
       const int c0 = 16 * c + 8 * h;
-      if (valid && c0 + 8 <= d) {
+      if (16 * c + 16 <= d) {
         const float4 p = *(const float4 *)(xr + c0), q = *(const float4 *)(xr + c0 + 4);
         v[0] = p.x; v[1] = p.y; v[2] = p.z; v[3] = p.w;
         v[4] = q.x; v[5] = q.y; v[6] = q.z; v[7] = q.w;
       } else {
 #pragma unroll
-        for (int t = 0; t < 8; t++) v[t] = (valid && c0 + t < d) ? xr[c0 + t] : 0.f;
+        for (int t = 0; t < 8; t++) v[t] = c0 + t < d ? xr[c0 + t] : 0.f;
       }
     };
     auto pack = [](const uint16_t (&e)[8]) {
@@ -203,9 +206,20 @@ __global__ __launch_bounds__(256) void yk_to_bf16_frag_kernel(const float *__res
     double s2 = 0, sb = 0, se = 0;
     float sf = 0.f;
     uint16_t keep[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = 0; c < nch && 16 * c < d; c++) {
+    // chunks holding data, four at a time: their loads are issued together
+    const int cdl = min(nch, (d + 15) >> 4);
+    for (int cg = 0; cg < cdl; cg += 4) {
+    float vg[4][8];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (cg + u < cdl) vals(cg + u, vg[u]);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int c = cg + u;
+      if (c >= cdl) break;
       float v[8];
-      vals(c, v);
+#pragma unroll
+      for (int t = 0; t < 8; t++) v[t] = valid ? vg[u][t] : 0.f;
       uint16_t e[8];
 #pragma unroll
       for (int t = 0; t < 8; t++) {
@@ -232,6 +246,7 @@ __global__ __launch_bounds__(256) void yk_to_bf16_frag_kernel(const float *__res
 #pragma unroll
         for (int t = 0; t < 8; t++) keep[t] = e[t];
       }
+    }
     }
     s2 += __shfl_xor(s2, 32);
     sb += __shfl_xor(sb, 32);
@@ -301,10 +316,11 @@ extern "C" int yk_to_bf16(const float *x, long n, int d, int type, int is_query,
     hipLaunchKernelGGL(yk_to_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, yk_stream(), x, n,
                        d, ld, ldh, type, is_query, out, (float4 *)stats, (unsigned *)stats_max);
   } else {
-    // a grid-stride grid of at most 512 workgroups: enough to stream the set,
-    // few enough that the per-workgroup maxima do not queue on 4 atomics
+    // a grid-stride grid of at most 2048 workgroups (8 per CU): enough waves
+    // in flight to stream the set, few enough that the per-workgroup maxima
+    // do not queue on 4 atomics
     long blocks = ((n + 31) / 32 + 3) / 4;
-    if (blocks > 512) blocks = 512;
+    if (blocks > 2048) blocks = 2048;
     hipLaunchKernelGGL(yk_to_bf16_frag_kernel, dim3((unsigned)blocks), dim3(256), 0, yk_stream(),
                        x, n, d, ld, ldh, type, out, (float4 *)stats, (unsigned *)stats_max, fnorm);
   }
