@@ -74,6 +74,22 @@ def test_knn_integer_mixture_bitexact(ya, oracle, k, typ):
     parity.check_knn(typ, q, b, k, gi, gd, oi, od, exact=True, label="mix%d" % k)
 
 
+@pytest.mark.parametrize("d", [8, 20, 32])
+@pytest.mark.parametrize("k", [1, 40])
+@pytest.mark.parametrize("typ", [2, 16])
+def test_knn_short_vectors_bitexact(ya, oracle, d, k, typ):
+    """d <= 32 (one or two 16-column chunks: base norms loaded a block ahead;
+    k = 1 and the k > 16 sketch take the branch-free exact pass on full
+    blocks); nb = 20011 leaves a partial last block."""
+    rng = np.random.default_rng(300 + 3 * d + k + typ)
+    b = mixture(rng, 20011, d, 64, 20)
+    q = mixture(rng, 555, d, 64, 20)
+    print("knn short vectors", typ, d, k, flush=True)
+    gi, gd = ya.knn(q, b, k, distance_type=typ)
+    oi, od = oracle.knn_full(typ, b, q, k)
+    parity.check_knn(typ, q, b, k, gi, gd, oi, od, exact=True, label="short%d_%d_%d" % (typ, d, k))
+
+
 @pytest.mark.parametrize("d", [16, 20, 128, 200, 960])
 @pytest.mark.parametrize("k", [1, 8])
 def test_knn_float_dims(ya, oracle, d, k):

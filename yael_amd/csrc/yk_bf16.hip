@@ -678,18 +678,22 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
 
 
 // Long vectors (d'/16 > 16 chunks): neither operand fits in registers.  Each
-// wave owns NB = 2 query tiles (64 queries) and takes the base in groups of
-// G = 2 blocks: the accumulators of all G x NB tiles stay in registers while
-// the chunk loop walks K, so each fragment loaded feeds two MFMAs (the
-// register-resident kernel's streaming at long d loaded three fragments per
-// two MFMAs).  The queries come from a fragment-major copy (qf), so every
-// load is one contiguous 1 KB fragment.  Chunk c + 1's fragments load while
-// chunk c's MFMAs run (two register sets used in turn, the loop unrolled by
-// two).  Selection per block as in yk_scan_bf16.  (Groups of 3 or 4 blocks
-// measured no faster and need one wave per SIMD.)
-template <int MODE, int R>
-__global__ __launch_bounds__(256) void yk_scan_bf16_long(Bf16Args a) {
-  constexpr int NB = 2, G = 2;
+// wave owns NB query tiles (NB = 3: 96 queries) and takes the base in groups
+// of G = 2 blocks: the accumulators of all G x NB tiles stay in registers
+// while the chunk loop walks K, so the NB + G fragments loaded per chunk feed
+// G x NB MFMAs (0.83 KB per MFMA at NB = 3; the register-resident kernel's
+// streaming at long d loaded three fragments per two MFMAs).  The queries
+// come from a fragment-major copy (qf), so every load is one contiguous 1 KB
+// fragment.  Chunk c + 1's fragments load while chunk c's MFMAs run (two
+// register sets used in turn, the loop unrolled by two).  Selection per
+// block as in yk_scan_bf16.  The kernel is held to two waves per SIMD, which
+// at NB = 3 spills a few selection registers to scratch; measured at d=960:
+// 156K queries/s against 140K with NB = 2 (NB = 4 spills 450 bytes per lane;
+// groups of 3 or 4 blocks measured no faster and need one wave per SIMD).
+template <int MODE, int R, int NB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+yk_scan_bf16_long(Bf16Args a) {
+  constexpr int G = 2;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int r = lane & 31, h = lane >> 5;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, wpx = gridDim.x >> 3;
@@ -706,10 +710,10 @@ __global__ __launch_bounds__(256) void yk_scan_bf16_long(Bf16Args a) {
     sel.init(a, qbase, r, lane, cstage + wave * (MODE == 3 ? NB * 64 * YK_CSTAGE : 0),
              vstage + (MODE >= 2 ? (wave * 64 + lane) * 16 : 0));
     if (MODE == 3 && !__any(sel.active())) continue;
-    // query tiles qbase/32 and qbase/32 + 1 of the fragment-major copy (its
-    // rows are padded to the launch's query span, the pad rows zero)
+    // query tiles qbase/32 .. qbase/32 + NB - 1 of the fragment-major copy
+    // (its rows are padded to the launch's query span, the pad rows zero)
     const uint16_t *qf0 = a.qf + ((qbase >> 5) * nch * 64 + lane) * 8;
-    const uint16_t *qf1 = qf0 + (long)nch * 512;
+    const long qtstride = (long)nch * 512;  // elements between query tiles
     const long nbk = (a.nb + 31) >> 5;
     const long rbytes = (nbk - (bbeg >> 5)) * (long)nch * 1024;
     const uint16_t *ibase = a.b + (bbeg >> 5) * (long)nch * 512;
@@ -727,31 +731,31 @@ __global__ __launch_bounds__(256) void yk_scan_bf16_long(Bf16Args a) {
         for (int j = 0; j < NB; j++)
 #pragma unroll
           for (int e = 0; e < 16; e++) acc[g][j][e] = 0.f;
-      bf16x8 qa0, qa1, ba[G], qb0, qb1, bv2[G];
+      bf16x8 qa[NB], ba[G], qb[NB], bv2[G];
       // past the range, the base loads return zeros (buffer bounds)
-      auto ld = [&](int c, bf16x8 &q0, bf16x8 &q1, bf16x8(&bv)[G]) {
-        q0 = *(const bf16x8 *)(qf0 + c * 512);
-        q1 = *(const bf16x8 *)(qf1 + c * 512);
+      auto ld = [&](int c, bf16x8(&q)[NB], bf16x8(&bv)[G]) {
+#pragma unroll
+        for (int j = 0; j < NB; j++) q[j] = *(const bf16x8 *)(qf0 + j * qtstride + c * 512);
 #pragma unroll
         for (int g = 0; g < G; g++)
           bv[g] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
                                                  rs, voff, sblk + g * gstride + c * 1024, 0));
       };
-      auto mm = [&](const bf16x8 &q0, const bf16x8 &q1, const bf16x8(&bv)[G]) {
+      auto mm = [&](const bf16x8(&q)[NB], const bf16x8(&bv)[G]) {
 #pragma unroll
-        for (int g = 0; g < G; g++) {
-          acc[g][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv[g], q0, acc[g][0], 0, 0, 0);
-          acc[g][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv[g], q1, acc[g][1], 0, 0, 0);
-        }
+        for (int g = 0; g < G; g++)
+#pragma unroll
+          for (int j = 0; j < NB; j++)
+            acc[g][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv[g], q[j], acc[g][j], 0, 0, 0);
       };
-      ld(0, qa0, qa1, ba);
+      ld(0, qa, ba);
       for (int c = 0; c < nch; c += 2) {
         const int c1 = c + 1 < nch ? c + 1 : c, c2 = c + 2 < nch ? c + 2 : c1;
-        ld(c1, qb0, qb1, bv2);
-        mm(qa0, qa1, ba);
+        ld(c1, qb, bv2);
+        mm(qa, ba);
         if (c + 1 >= nch) break;
-        ld(c2, qa0, qa1, ba);
-        mm(qb0, qb1, bv2);
+        ld(c2, qa, ba);
+        mm(qb, bv2);
       }
 #pragma unroll
       for (int g = 0; g < G; g++) {
@@ -1359,9 +1363,12 @@ static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, 
 
 // Query tiles (32 queries each) per wave: three where the kernel still fits
 // two waves per SIMD (256 VGPRs: query fragments 12 * nch, accumulators 48,
-// base buffer 4 * nch, selection state), else two.  Each base fragment
-// loaded then feeds three MFMAs instead of two.
-static int yk_bnb(int mode, int nch) { return mode != 2 && nch >= 1 && nch <= 9 ? 3 : 2; }
+// base buffer 4 * nch, selection state), and in the long-vector kernel; else
+// two.  Each base fragment loaded then feeds three MFMAs instead of two.
+static int yk_bnb(int mode, int nch) {
+  if (mode != 2 && nch > 16) return 3;  // yk_scan_bf16_long<MODE, R, 3>
+  return mode != 2 && nch >= 1 && nch <= 9 ? 3 : 2;
+}
 
 // streaming kernel with exactly nch register-resident chunks (nch <= 16),
 // else the one streaming both operands
@@ -1396,7 +1403,7 @@ static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
   const int nbq = yk_bnb(MODE, a.ldh >> 4);
   void (*kern)(Bf16Args) = yk_bscan_pick<MODE, R>(a.ldh >> 4);
   if constexpr (MODE != 2) {
-    if ((a.ldh >> 4) > 16) kern = yk_scan_bf16_long<MODE, R>;
+    if ((a.ldh >> 4) > 16) kern = yk_scan_bf16_long<MODE, R, 3>;
   }
   a.brange = p.brange;
   a.bstep = p.bstep;
@@ -1535,8 +1542,9 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
     a.ldh = ldh;
     if ((ldh >> 4) > 16 && kp > YK_REG_R) {
       // long vectors: the batch's queries fragment-major for the long scan,
-      // rows padded to its 256-query workgroup span
-      const long rows = (n + 255) / 256 * 256, nw = (rows >> 5) * (ldh >> 4);
+      // rows padded to its workgroup span (4 waves x 32 x query tiles)
+      const long span = 128L * yk_bnb(3, ldh >> 4);
+      const long rows = (n + span - 1) / span * span, nw = (rows >> 5) * (ldh >> 4);
       void *vqf;
       YK_TRY(yk_bscratch(9, sizeof(uint16_t) * (size_t)rows * ldh, &vqf));
       hipLaunchKernelGGL(yk_frag_from_rows_kernel, dim3((unsigned)((nw + 3) / 4)), dim3(256), 0,
