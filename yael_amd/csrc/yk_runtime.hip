@@ -258,8 +258,9 @@ int yk_prof_setup(void) {
 static void yk_prof_flush(void) {
   for (int i = 0; i < g_pev_n; i++) {
     float ms = 0;
-    hipEventSynchronize(g_pev[i][1]);
-    if (hipEventElapsedTime(&ms, g_pev[i][0], g_pev[i][1]) == hipSuccess) {
+    // a pair whose events failed is left out of the sums
+    if (hipEventSynchronize(g_pev[i][1]) == hipSuccess &&
+        hipEventElapsedTime(&ms, g_pev[i][0], g_pev[i][1]) == hipSuccess) {
       g_prof_ms[g_pev_slot[i]] += ms;
       g_prof_cnt[g_pev_slot[i]]++;
     }
@@ -270,15 +271,17 @@ static void yk_prof_flush(void) {
 int yk_prof_begin(int slot) {
   if (!g_prof_on) return -1;
   if (g_pev_n == YK_PROF_EV) yk_prof_flush();
-  int i = g_pev_n++;
+  const int i = g_pev_n;
+  if (hipEventRecord(g_pev[i][0], g_stream) != hipSuccess) return -1;  // not timed
   g_pev_slot[i] = slot;
-  hipEventRecord(g_pev[i][0], g_stream);
+  g_pev_n++;
   return i;
 }
 
 void yk_prof_end(int tok) {
   if (tok < 0) return;
-  hipEventRecord(g_pev[tok][1], g_stream);
+  // a failed record leaves the pair unfinished: yk_prof_flush skips it
+  (void)hipEventRecord(g_pev[tok][1], g_stream);
 }
 
 extern "C" int yk_prof_enable(int on) {
