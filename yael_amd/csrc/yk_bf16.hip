@@ -1338,7 +1338,9 @@ struct BPlan {
   long brange, bstep;
 };
 
-static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, long max_split) {
+// bbytes: bytes of one 32-row block of the base copy (32 rows x ldh BF16)
+static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, long max_split,
+                      long bbytes) {
   BPlan p;
   const long qt = 32L * nblk * 4;
   p.tiles = (int)((nq + qt - 1) / qt);
@@ -1353,8 +1355,9 @@ static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, 
   if (want < 1) want = 1;
   long per = (nblocks + want - 1) / want;
   // an item's fragments stay within 2^30 bytes (buffer-load offsets are
-  // 32-bit; 16 KB per 32-row block at d' = 256)
-  const long max_per = (1L << 30) / (16384L * sample);
+  // 32-bit signed; a 32-row block is 64 * ldh bytes, 61 KB at d' = 976)
+  long max_per = (1L << 30) / (bbytes * sample);
+  if (max_per < 1) max_per = 1;
   if (per > max_per) per = max_per;
   p.brange = per * p.bstep;
   p.nsplit = (int)((nb + p.brange - 1) / p.brange);
@@ -1442,10 +1445,11 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   const long nq = a.nq;
   void *vpart;
   if (kp <= YK_REG_R) {
-    BPlan p = yk_bplan(nq, a.nb, 2, 1, 1, 256);
+    const long bbytes = 64L * a.ldh;
+    BPlan p = yk_bplan(nq, a.nb, 2, 1, 1, 256, bbytes);
     // enough query tiles to fill the chip: no base split, and the lane pairs
     // write their merged lists straight to cand_out
-    if (p.tiles >= 2048) p = yk_bplan(nq, a.nb, 2, 1, 1, 1);
+    if (p.tiles >= 2048) p = yk_bplan(nq, a.nb, 2, 1, 1, 1, bbytes);
     const int R = kp <= 8 ? 8 : YK_REG_R;
     if (p.nsplit == 1) {
       a.cand = cand_out;
@@ -1478,7 +1482,8 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   if (stride < 1) stride = 1;
   const long nblocks = (a.nb + 31) / 32;
   while (stride > 1 && nblocks / stride < 2L * kp) stride >>= 1;
-  BPlan pa = yk_bplan(nq, a.nb, yk_bnb(1, a.ldh >> 4), stride, kp, kp > 256 ? kp : 256);
+  BPlan pa = yk_bplan(nq, a.nb, yk_bnb(1, a.ldh >> 4), stride, kp, kp > 256 ? kp : 256,
+                      64L * a.ldh);
   YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * pa.nsplit * 4, &vpart));
   a.part = (uint64_t *)vpart;
   YK_TRY((yk_launch_bscan<1, 1>(pa, a)));
@@ -1486,7 +1491,7 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   hipLaunchKernelGGL(yk_bf_zero_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
                      yk_stream(), cnt, nq);
   hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), flag, 1L);
-  BPlan pb = yk_bplan(nq, a.nb, yk_bnb(3, a.ldh >> 4), 1, 1, 4096);
+  BPlan pb = yk_bplan(nq, a.nb, yk_bnb(3, a.ldh >> 4), 1, 1, 4096, 64L * a.ldh);
   a.thr = thr;
   a.cnt = cnt;
   a.cand = (uint64_t *)vcand;
