@@ -569,7 +569,7 @@ struct BfSel {
 // base fragments loaded straight to registers one block ahead; NCH = 0 both
 // operands streamed from memory (long vectors).
 template <int NB, int NCH, int MODE, int R>
-__global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
+__device__ __forceinline__ void yk_scan_bf16_body(const Bf16Args &a) {
   constexpr bool QRES = NCH > 0;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int r = lane & 31, h = lane >> 5;
@@ -674,6 +674,20 @@ __global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
     }
     sel.finish(a, qbase, split, r, h);
   }  // work items
+}
+
+template <int NB, int NCH, int MODE, int R>
+__global__ __launch_bounds__(256) void yk_scan_bf16(Bf16Args a) {
+  yk_scan_bf16_body<NB, NCH, MODE, R>(a);
+}
+
+// The register top-R scan with three query tiles, held to two waves per SIMD
+// (the insertion network's state for three tiles spills a few registers to
+// scratch; the fast test keeps that path rare once the lists have filled).
+template <int NB, int NCH, int MODE, int R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+yk_scan_bf16_w2(Bf16Args a) {
+  yk_scan_bf16_body<NB, NCH, MODE, R>(a);
 }
 
 
@@ -1366,10 +1380,13 @@ static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, 
 
 // Query tiles (32 queries each) per wave: three where the kernel still fits
 // two waves per SIMD (256 VGPRs: query fragments 12 * nch, accumulators 48,
-// base buffer 4 * nch, selection state), and in the long-vector kernel; else
-// two.  Each base fragment loaded then feeds three MFMAs instead of two.
-static int yk_bnb(int mode, int nch) {
+// base buffer 4 * nch, selection state), in the long-vector kernel and in the
+// register top-8 scan (the k-means assign; yk_scan_bf16_w2, measured 5.5%
+// faster than two tiles at k = 65536); else two.  Each base fragment loaded
+// then feeds three MFMAs instead of two.
+static int yk_bnb(int mode, int nch, int r = YK_REG_R) {
   if (mode != 2 && nch > 16) return 3;  // yk_scan_bf16_long<MODE, R, 3>
+  if (mode == 2 && r <= 8 && nch >= 1 && nch <= 9) return 3;  // yk_scan_bf16_w2
   return mode != 2 && nch >= 1 && nch <= 9 ? 3 : 2;
 }
 
@@ -1403,8 +1420,23 @@ template <int MODE, int R>
 static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
   // 64 or 96 queries per wave at two waves per SIMD (measured faster than
   // 128 queries per wave at one wave per SIMD)
-  const int nbq = yk_bnb(MODE, a.ldh >> 4);
+  const int nbq = yk_bnb(MODE, a.ldh >> 4, R);
   void (*kern)(Bf16Args) = yk_bscan_pick<MODE, R>(a.ldh >> 4);
+  if constexpr (MODE == 2 && R <= 8) {
+    if (nbq == 3) {
+      switch (a.ldh >> 4) {
+        case 1: kern = yk_scan_bf16_w2<3, 1, MODE, R>; break;
+        case 2: kern = yk_scan_bf16_w2<3, 2, MODE, R>; break;
+        case 3: kern = yk_scan_bf16_w2<3, 3, MODE, R>; break;
+        case 4: kern = yk_scan_bf16_w2<3, 4, MODE, R>; break;
+        case 5: kern = yk_scan_bf16_w2<3, 5, MODE, R>; break;
+        case 6: kern = yk_scan_bf16_w2<3, 6, MODE, R>; break;
+        case 7: kern = yk_scan_bf16_w2<3, 7, MODE, R>; break;
+        case 8: kern = yk_scan_bf16_w2<3, 8, MODE, R>; break;
+        default: kern = yk_scan_bf16_w2<3, 9, MODE, R>; break;
+      }
+    }
+  }
   if constexpr (MODE != 2) {
     if ((a.ldh >> 4) > 16) kern = yk_scan_bf16_long<MODE, R, 3>;
   }
@@ -1446,11 +1478,12 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   void *vpart;
   if (kp <= YK_REG_R) {
     const long bbytes = 64L * a.ldh;
-    BPlan p = yk_bplan(nq, a.nb, 2, 1, 1, 256, bbytes);
+    const int R = kp <= 8 ? 8 : YK_REG_R;
+    const int nbq = yk_bnb(2, a.ldh >> 4, R);
+    BPlan p = yk_bplan(nq, a.nb, nbq, 1, 1, 256, bbytes);
     // enough query tiles to fill the chip: no base split, and the lane pairs
     // write their merged lists straight to cand_out
-    if (p.tiles >= 2048) p = yk_bplan(nq, a.nb, 2, 1, 1, 1, bbytes);
-    const int R = kp <= 8 ? 8 : YK_REG_R;
+    if (p.tiles * nbq >= 4096) p = yk_bplan(nq, a.nb, nbq, 1, 1, 1, bbytes);
     if (p.nsplit == 1) {
       a.cand = cand_out;
       a.kp_direct = kp;

@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
         // full blocks of unweighted searches (the last, partial block of a
         // range and weighted searches take the general path below)
         if (!wts && bb + 32 <= bend) {
-          const bool test = bb - bbeg >= 48 * 32;
+          const bool test = bb - bbeg >= 48L * a.bstep;
           if (dot)
             yk_best1_block<true>(acc[j], nn, qnd[j], bb, h, test, bv[j], bi[j]);
           else
