@@ -66,12 +66,15 @@ def compare(batched, singles, sizes):
         off += n
 
 
-def test_uneven_instances_match_single_runs(ya):
-    rng = np.random.default_rng(1)
+@pytest.mark.parametrize("d", [32, 100, 128])
+def test_uneven_instances_match_single_runs(ya, d):
+    """d = 32: the float4-column update path (4 waves); d = 100: one thread
+    per dimension; d = 128: groups of 32 dimensions per wave."""
+    rng = np.random.default_rng(1 + d)
     sizes = zipf_sizes(24, 60000, 256, 20000)
-    v = mixture(rng, int(sizes.sum()), 32, 200, 18)
+    v = mixture(rng, int(sizes.sum()), d, 200, 18)
     seeds = np.arange(1, 25)
-    print("sizes", sizes.tolist(), flush=True)
+    print("d", d, "sizes", sizes.tolist(), flush=True)
     b = ya.kmeans_batched(v, sizes, 16, niter=12, seeds=seeds)
     compare(b, single(ya, v, sizes, 16, 12, seeds), sizes)
 

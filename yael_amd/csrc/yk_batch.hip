@@ -59,23 +59,27 @@ struct BatchDev {
   int *it;  // iteration counter
 };
 
-// Row stride (doubles) of the LDS table: d < 64 with d a multiple of 4
-// pads the rows to an odd length so that the lanes of one instruction,
-// each on its own cluster's row, spread over the LDS banks.
+// Row stride (doubles) of the LDS table: the column paths (d < 64 with d a
+// multiple of 4, d >= 64 a multiple of 32) pad the rows to an odd length so
+// that the lanes of one instruction, each on its own cluster's row, spread
+// over the LDS banks.
 __host__ __device__ static inline int yk_bupd_stride(int d) {
-  return (d < 64 && (d & 3) == 0) ? d + 1 : d;
+  return ((d < 64 && (d & 3) == 0) || (d & 31) == 0) ? d + 1 : d;
 }
 
 // One workgroup per update chunk adds the chunk's points into an LDS FP64
-// table [k][d].  d >= 64: one thread per dimension (up to 4 waves), each
+// table [k][d].  d >= 64 a multiple of 32: wave w owns the groups of 8 float4
+// columns (32 dimensions) w, w + 4, ... (no two waves share a sum); inside
+// a group, lane 8 p + c takes column c of the step's point p, so one load
+// instruction reads 8 whole 128-byte row segments, eight steps (64 points)
+// in flight.  Other d >= 64: one thread per dimension (up to 4 waves), each
 // (cluster, dimension) sum owned by one thread and accumulated in point
 // order.  d < 64, d a multiple of 4: wave w owns the float4 columns
-// w, w + 4, ... of the rows (no two waves share a sum), a lane takes one
-// point per step, four steps in flight; d < 64 otherwise: one wave whose
-// lanes split into 64/d groups of d.  Both use ds_add_f64: the LDS atomics
-// of one wave apply in instruction order and, inside one instruction, in a
-// fixed order, so every sum is accumulated in point order.  Deterministic,
-// whatever the grid.  The chunk's objective (its points' distances, FP64,
+// w, w + 4, ... of the rows, a lane takes one point per step, four steps in
+// flight; d < 64 otherwise: one wave whose lanes split into 64/d groups of
+// d.  The column paths use ds_add_f64: the LDS atomics of one wave apply in
+// instruction order and, inside one instruction, in a fixed order, so every
+// sum is accumulated in point order.  Deterministic, whatever the grid.  The chunk's objective (its points' distances, FP64,
 // thread-strided then a fixed tree) goes to pobj[u].
 __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
   extern __shared__ double lsum[];  // k x sd
@@ -105,7 +109,36 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
     for (int o = 32; o > 0; o >>= 1) obj += __shfl_xor(obj, o);
     if (lane == 0) wobj[wave] = obj;
   }
-  if (d >= 64) {
+  if (d >= 64 && (d & 31) == 0) {
+    const int nwave = nthr >> 6, pl = lane >> 3, cl = lane & 7;
+    for (int g = wave; g < (d >> 5); g += nwave) {
+      const int j0 = g * 32 + cl * 4;
+      for (long p = p0 + pl; p < p1; p += 64) {
+        int a[8];
+        float4 x[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+          const long q = p + 8 * s;
+          a[s] = q < p1 ? b.assign[q] : -1;
+          x[s] = q < p1 ? *(const float4 *)(b.v + q * ld + j0) : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+          if (a[s] < 0) continue;
+          double *row = lsum + a[s] * sd + j0;
+          __hip_atomic_fetch_add(row, (double)x[s].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(row + 1, (double)x[s].y, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(row + 2, (double)x[s].z, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          __hip_atomic_fetch_add(row + 3, (double)x[s].w, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (g == 0 && cl == 0)
+            __hip_atomic_fetch_add(&lcnt[a[s]], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+      }
+    }
+  } else if (d >= 64) {
     // a thread owns its dimensions: plain read-add-write, four points in
     // flight
     for (int t = tid; t < d; t += nthr) {
@@ -386,10 +419,12 @@ static int yk_batch_enqueue(yk_batch *bt) {
   YK_TRY(yk_norms(b.cent, (long)b.ninst * b.kpad, b.d, b.cn, nullptr));
   YK_TRY(yk_batch_assign(b.nitems, b.item_inst, b.item_tile, b.inst_off, b.d, b.k, b.kpad, b.v,
                          b.vn, b.cent, b.cn, b.done, b.assign, b.dis));
+  // d >= 64 a multiple of 32: a wave per 32-dimension group (at most 4);
   // d < 64: a wave per float4 column (at most 4), or one wave when d is not
   // a multiple of 4; else one thread per dimension (at most 256)
-  const int pthreads = b.d < 64 ? ((b.d & 3) == 0 ? 64 * (b.d / 4 < 4 ? b.d / 4 : 4) : 64)
-                                : (b.d > 256 ? 256 : ((b.d + 63) / 64) * 64);
+  const int pthreads = (b.d >= 64 && (b.d & 31) == 0) ? 64 * (b.d / 32 < 4 ? b.d / 32 : 4)
+                       : b.d < 64      ? ((b.d & 3) == 0 ? 64 * (b.d / 4 < 4 ? b.d / 4 : 4) : 64)
+                                       : (b.d > 256 ? 256 : ((b.d + 63) / 64) * 64);
   hipLaunchKernelGGL(yk_bupd_partial_kernel, dim3((unsigned)b.nuitems), dim3(pthreads),
                      yk_batch_lds_bytes(b.k, b.d), st, b);
   hipLaunchKernelGGL(yk_bupd_reduce_kernel, dim3((unsigned)(((long)b.ninst * b.k + 3) / 4)),
