@@ -21,6 +21,7 @@
  * cache file keyed by the problem shape.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -145,6 +146,78 @@ static int yh_batch_init_state(yh_bstate *s) {
   return rc;
 }
 
+/* host precomputation of instances [i0, ninst) in steps of step: the
+   initial selection (kmeans.c:15-20, 402) and, per iteration, the split
+   stream's first YH_ECAP (draw, gaussian vector) pairs (kmeans.c:183-195,
+   296, 408-409) */
+typedef struct {
+  int ninst, niter, k, kpad, d;
+  const int *n_inst;
+  const long *off, *seed_used;
+  int *sel;
+  double *rd;
+  float *eps;
+  int i0, step, failed;
+} yh_prep;
+
+static void *yh_prep_worker(void *arg) {
+  yh_prep *p = (yh_prep *)arg;
+  const int niter = p->niter, d = p->d;
+  int *tmp = (int *)malloc(sizeof(int) * p->k);
+  if (!tmp) {
+    p->failed = 1;
+    return NULL;
+  }
+  for (int i = p->i0; i < p->ninst; i += p->step) {
+    unsigned seed = (unsigned)p->seed_used[i];
+    yh_random_init(p->n_inst[i], p->k, tmp, (unsigned)rand_r(&seed));
+    for (int c = 0; c < p->kpad; c++)
+      p->sel[(long)i * p->kpad + c] = (int)(p->off[i] + tmp[c < p->k ? c : 0]);
+    unsigned cs = (unsigned)rand_r(&seed);
+    for (int it = 0; it < niter; it++) {
+      unsigned S = (unsigned)rand_r(&cs); /* kmeans.c:296 */
+      for (int e = 0; e < YH_ECAP; e++) {
+        long t = ((long)i * niter + it) * YH_ECAP + e;
+        p->rd[t] = yh_drand_r(&S);
+        unsigned gs = (unsigned)rand_r(&S);
+        for (int l = 0; l < d; l++) p->eps[t * d + l] = (float)yh_gaussrand_r(&gs);
+      }
+    }
+  }
+  free(tmp);
+  return NULL;
+}
+
+/* runs yh_prep_worker over the instances; the results do not depend on the
+   number of threads (a thread that cannot start leaves its share to the
+   calling thread) */
+static void yh_prep_run(yh_prep *p) {
+  long work = 0;
+  for (int i = 0; i < p->ninst; i++)
+    work += p->n_inst[i] + (long)p->niter * YH_ECAP * p->d;
+  int nt = work < 200000 ? 1 : 16;
+  if (nt > p->ninst) nt = p->ninst;
+  yh_prep job[16];
+  pthread_t th[16];
+  int started[16] = {0};
+  for (int t = 0; t < nt; t++) {
+    job[t] = *p;
+    job[t].i0 = t;
+    job[t].step = nt;
+    job[t].failed = 0;
+    if (t > 0) started[t] = pthread_create(&th[t], NULL, yh_prep_worker, &job[t]) == 0;
+  }
+  yh_prep_worker(&job[0]);
+  for (int t = 1; t < nt; t++) {
+    if (started[t])
+      pthread_join(th[t], NULL);
+    else
+      yh_prep_worker(&job[t]);
+  }
+  p->failed = 0;
+  for (int t = 0; t < nt; t++) p->failed |= job[t].failed;
+}
+
 int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, const float *v,
                       int v_on_device, int flags, const long *seeds, float *cent_out,
                       float *dis_out, int *assign_out, int *nassign_out, float *ret_out,
@@ -179,40 +252,31 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   s.ntot = s.off[ninst];
   const int ld = s.ld, kpad = s.kpad;
 
-  /* seed chains, initial selections, split streams */
+  /* seed chains (lrand48 in instance order, as successive kmeans() calls
+     draw them), then per instance, on up to 16 host threads, the initial
+     selection and the split streams (each a function of its seed only) */
   int *sel = (int *)malloc(sizeof(int) * (size_t)ninst * kpad);
-  int *tmp = (int *)malloc(sizeof(int) * k);
   double *rd = (double *)malloc(sizeof(double) * (size_t)ninst * niter * YH_ECAP);
   float *eps = (float *)malloc(sizeof(float) * (size_t)ninst * niter * YH_ECAP * d);
-  unsigned *core = (unsigned *)malloc(sizeof(unsigned) * ninst);
   long *seed_used = (long *)malloc(sizeof(long) * ninst);
-  for (int i = 0; i < ninst; i++) {
-    long si = seeds ? seeds[i] : 0;
-    if (si == 0) si = lrand48(); /* kmeans.c:379-380 */
-    seed_used[i] = si;
-    unsigned seed = (unsigned)si;
-    yh_random_init(n_inst[i], k, tmp, (unsigned)rand_r(&seed));
-    for (int c = 0; c < kpad; c++) sel[(long)i * kpad + c] = (int)(s.off[i] + tmp[c < k ? c : 0]);
-    core[i] = (unsigned)rand_r(&seed);
-    unsigned cs = core[i];
-    for (int it = 0; it < niter; it++) {
-      unsigned S = (unsigned)rand_r(&cs); /* kmeans.c:296 */
-      for (int e = 0; e < YH_ECAP; e++) {
-        long t = ((long)i * niter + it) * YH_ECAP + e;
-        rd[t] = yh_drand_r(&S);
-        unsigned gs = (unsigned)rand_r(&S);
-        for (int l = 0; l < d; l++) eps[t * d + l] = (float)yh_gaussrand_r(&gs);
-      }
+  int host_ok = sel && rd && eps && seed_used;
+  if (host_ok) {
+    for (int i = 0; i < ninst; i++) {
+      long si = seeds ? seeds[i] : 0;
+      if (si == 0) si = lrand48(); /* kmeans.c:379-380 */
+      seed_used[i] = si;
     }
+    yh_prep prep = {ninst, niter, k, kpad, d, n_inst, s.off, seed_used, sel, rd, eps, 0, 1, 0};
+    yh_prep_run(&prep);
+    host_ok = !prep.failed;
   }
-  free(tmp);
-  free(core);
 
   /* assign work items: (instance, tile of 128 points) */
   long nitems = 0;
   for (int i = 0; i < ninst; i++) nitems += (n_inst[i] + YH_TILE - 1) / YH_TILE;
   int *ii = (int *)malloc(sizeof(int) * nitems), *it_ = (int *)malloc(sizeof(int) * nitems);
-  for (int i = 0, t = 0; i < ninst; i++)
+  host_ok = host_ok && ii && it_;
+  for (int i = 0, t = 0; host_ok && i < ninst; i++)
     for (int j = 0; j < (n_inst[i] + YH_TILE - 1) / YH_TILE; j++, t++) {
       ii[t] = i;
       it_[t] = j;
@@ -238,10 +302,10 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   s.pbuf = (float *)yh_dmalloc(&al, sizeof(float) * (size_t)ninst * k);
   s.item_inst = (int *)yh_dmalloc(&al, sizeof(int) * nitems);
   s.item_tile = (int *)yh_dmalloc(&al, sizeof(int) * nitems);
-  rc = YK_OK;
-  if (!s.d_off || !vdev || !s.vn || !s.cent || !s.cn || !s.dis || !s.assign || !s.sel ||
+  rc = host_ok ? YK_OK : yk_set_error_c(YK_ENOMEM, "batched k-means: host malloc");
+  if (rc == YK_OK && (!s.d_off || !vdev || !s.vn || !s.cent || !s.cn || !s.dis || !s.assign || !s.sel ||
       !s.nassign || !s.flags || !s.qerr2 || !s.rd || !s.eps || !s.pbuf || !s.item_inst ||
-      !s.item_tile)
+      !s.item_tile))
     rc = YK_ENOMEM;
   if (rc == YK_OK) rc = yk_memcpy_h2d(s.d_off, s.off, sizeof(long) * (ninst + 1));
   if (rc == YK_OK && !v_on_device) rc = yk_upload_vectors(vdev, v, s.ntot, d);
