@@ -40,25 +40,39 @@ template <bool WF, bool WD>
 __global__ __launch_bounds__(256) void yk_norms_kernel(const float *__restrict__ x, long n,
                                                        int ld, int d, float *__restrict__ of,
                                                        double *__restrict__ od) {
-  __shared__ float tile[256][17];
+  // 256 rows per workgroup, taken 32 columns (one 128-byte line per row) at
+  // a time: coalesced float4 loads into an LDS tile, then one thread per row
+  // sums its 32 values in column order.  The next chunk's loads are issued
+  // before the current chunk is summed.
+  constexpr int CW = 32, NL = CW / 4;  // columns per chunk, float4 per row segment
+  __shared__ float tile[256][CW + 1];
   const long row0 = (long)blockIdx.x * 256;
   const int t = threadIdx.x;
   float sf = 0.f;
   double sd = 0.0;
-  for (int c0 = 0; c0 < d; c0 += 16) {
+  float4 buf[NL];
+  auto load = [&](int c0) {
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      int f = t + 256 * i, r = f >> 2, cg = f & 3;
-      long row = row0 + r;
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (row < n) v = *(const float4 *)(x + row * ld + c0 + cg * 4);
-      tile[r][cg * 4 + 0] = v.x;
-      tile[r][cg * 4 + 1] = v.y;
-      tile[r][cg * 4 + 2] = v.z;
-      tile[r][cg * 4 + 3] = v.w;
+    for (int i = 0; i < NL; i++) {
+      const int f = t + 256 * i, r = f / NL, cg = f % NL;
+      const long row = row0 + r;
+      buf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (row < n && c0 + cg * 4 < ld) buf[i] = *(const float4 *)(x + row * ld + c0 + cg * 4);
+    }
+  };
+  load(0);
+  for (int c0 = 0; c0 < d; c0 += CW) {
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+      const int f = t + 256 * i, r = f / NL, cg = f % NL;
+      tile[r][cg * 4 + 0] = buf[i].x;
+      tile[r][cg * 4 + 1] = buf[i].y;
+      tile[r][cg * 4 + 2] = buf[i].z;
+      tile[r][cg * 4 + 3] = buf[i].w;
     }
     __syncthreads();
-    int cmax = d - c0 < 16 ? d - c0 : 16;
+    if (c0 + CW < d) load(c0 + CW);
+    const int cmax = d - c0 < CW ? d - c0 : CW;
     for (int j = 0; j < cmax; j++) {
       float v = tile[t][j];
       float p = __fmul_rn(v, v);  // float product, never fused
