@@ -133,25 +133,31 @@ __global__ __launch_bounds__(256) void yk_segsum_kernel(const float *__restrict_
   const double inv = 1.0 / (double)(end - beg);
   for (int j0 = 0; j0 < d; j0 += 128) {
     const int ja = j0 + 2 * lane;
+    const int jl = ja < d ? ja : 0;  // lanes past d load column 0 and discard it
     double s0 = 0.0, s1 = 0.0;
-    int p = beg;
-    for (; p + 4 <= end; p += 4) {
-      long i0 = perm[p], i1 = perm[p + 1], i2 = perm[p + 2], i3 = perm[p + 3];
-      if (ja < d) {
-        float2 x0 = *(const float2 *)(v + i0 * ld + ja);
-        float2 x1 = *(const float2 *)(v + i1 * ld + ja);
-        float2 x2 = *(const float2 *)(v + i2 * ld + ja);
-        float2 x3 = *(const float2 *)(v + i3 * ld + ja);
-        s0 += (double)x0.x; s1 += (double)x0.y;
-        s0 += (double)x1.x; s1 += (double)x1.y;
-        s0 += (double)x2.x; s1 += (double)x2.y;
-        s0 += (double)x3.x; s1 += (double)x3.y;
+    // the cluster's point indices 64 at a time (one coalesced load, each
+    // then read from its lane), rows 8 at a time: no row load waits for an
+    // index load; the sums still take the rows in point order
+    for (int pb = beg; pb < end; pb += 64) {
+      const int nb = end - pb < 64 ? end - pb : 64;
+      const int pidx = lane < nb ? perm[pb + lane] : 0;
+      int t = 0;
+      for (; t + 8 <= nb; t += 8) {
+        float2 x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const long i = __shfl(pidx, t + u);
+          x[u] = *(const float2 *)(v + i * ld + jl);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          s0 += (double)x[u].x;
+          s1 += (double)x[u].y;
+        }
       }
-    }
-    for (; p < end; p++) {
-      long i0 = perm[p];
-      if (ja < d) {
-        float2 x0 = *(const float2 *)(v + i0 * ld + ja);
+      for (; t < nb; t++) {
+        const long i = __shfl(pidx, t);
+        const float2 x0 = *(const float2 *)(v + i * ld + jl);
         s0 += (double)x0.x;
         s1 += (double)x0.y;
       }
