@@ -258,3 +258,23 @@ def test_knn_full_precision_form_nan_data(ya):
     assert np.array_equal(i16, i32)
     assert np.array_equal(d16.view(np.int32), d32.view(np.int32))
     assert not np.any(i16 == 17)
+
+
+def test_knn_large_base_offsets(ya):
+    """A 3 GB base (6e6 x 128 floats, a BF16 copy of 1.7 GB): rows far past
+    2^31 bytes are found.  Each query is a copy of a base row (the last one
+    included) on integer data with distinct rows, so its nearest neighbour is
+    that row at distance exactly 0, in both forms; the two forms agree bit
+    for bit on the whole lists."""
+    rng = np.random.default_rng(606)
+    nb, d, k = 6_000_000, 128, 10
+    b = rng.integers(0, 256, (nb, d), dtype=np.uint8).astype(np.float32)
+    rows = np.concatenate([np.linspace(0, nb - 1, 254).astype(np.int64), [nb - 2, nb - 1]])
+    q = b[rows].copy()
+    print("knn large base nb=%d, %d queries" % (nb, len(rows)), flush=True)
+    i16, d16, i32, d32 = _both_forms(ya, q, b, k)
+    del b
+    assert np.array_equal(i32[:, 0], rows.astype(np.int32))
+    assert np.all(d32[:, 0] == 0.0)
+    assert np.array_equal(i16, i32)
+    assert np.array_equal(d16.view(np.int32), d32.view(np.int32))
