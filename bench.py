@@ -557,8 +557,9 @@ def main():
     ap.add_argument("--kprime", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (got --gpus %d); the multi-GPU "
-              "sharding is described in DESIGN.md" % args.gpus, file=sys.stderr)
+        print("bench.py: this library runs on one GPU only (got --gpus %d): the base "
+              "(kNN) and point (k-means) sharding over several GPUs is not built"
+              % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import yael_amd as ya
