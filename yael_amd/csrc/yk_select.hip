@@ -270,15 +270,28 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
   const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
       (void *)(a.b + bbeg * ld), 0, (int)min((a.nb - bbeg) * ld * 4, 0x7fffffffL), YK_BUF_DWORD3);
   // With one or two chunks a block is only 8-16 MFMAs: its base norms
-  // (rows 8g + 4h .. +3) are loaded one block ahead as well, so that the
-  // epilogue does not wait for them -- and, with them, for the next block's
-  // operands issued meanwhile.
+  // (rows 8g + 4h .. +3) are loaded at the end of the previous block, into
+  // the registers the epilogue reads, so that the epilogue does not wait for
+  // them -- nor, with them, for the next block's operands issued meanwhile
+  // (those are younger).  Dot-product searches read no norms.
   // Weighted searches are served by the other variants (the launcher's
   // choice), so these kernels compile without the weights' loads.
   constexpr bool NPF = QRES && NCH <= 2;
   const float *const wts = NPF ? nullptr : a.w;
   float4 anext[QRES ? 2 * NCH : 1];
-  float4 nnext[NPF ? 4 : 1];
+  // Buffer loads based at the item's first row (bn holds round_up(nb, 32)
+  // entries); for a dot-product search the descriptor has no records, so
+  // the loads are unconditional and return zeros without touching memory.
+  float4 nn4[NPF ? 4 : 1];
+  const __amdgpu_buffer_rsrc_t rsn = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)(dot ? a.b : a.bn + bbeg), 0,
+      dot ? 0 : (int)min((((a.nb + 31) & ~31L) - bbeg) * 4, 0x7fffffffL), YK_BUF_DWORD3);
+  auto loadn = [&](long bb) {
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+      nn4[g] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(
+                                              rsn, 16 * h, (int)((bb - bbeg) * 4) + 32 * g, 0));
+  };
   auto loadb = [&](long bb) {
     const int soff = (int)((bb - bbeg) * ld * 4);
     const int voff = ((int)min((long)r, a.nb - 1 - bb) * ld + h * 8) * 4;
@@ -289,13 +302,9 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
       anext[2 * c + 1] = __builtin_bit_cast(
           float4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff + 16, soff + c * 64, 0));
     }
-    if constexpr (NPF) {
-      if (!dot)
-#pragma unroll
-        for (int g = 0; g < 4; g++) nnext[g] = *(const float4 *)(a.bn + bb + 8 * g + 4 * h);
-    }
   };
   if (QRES && bbeg < bend) loadb(bbeg);
+  if (NPF && bbeg < bend) loadn(bbeg);
   for (long bb = bbeg; bb < bend; bb += a.bstep) {
     const long brow = min(bb + r, a.nb - 1);
     const float *bp = a.b + brow * ld + h * 8;
@@ -305,14 +314,10 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
 #pragma unroll
       for (int e = 0; e < 16; e++) acc[j][e] = 0.f;
 
-    float4 ncur[NPF ? 4 : 1];
     if (QRES) {
       float4 av[QRES ? 2 * NCH : 1];
 #pragma unroll
       for (int c = 0; c < 2 * NCH; c++) av[c] = anext[c];
-      if constexpr (NPF)
-#pragma unroll
-        for (int g = 0; g < 4; g++) ncur[g] = nnext[g];
       {
         // unconditional (past the range it reloads the current block): a
         // conditional prefetch joins paths with different outstanding-load
@@ -362,8 +367,9 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
 #pragma unroll
     for (int g = 0; g < 4; g++) {
       const long b4 = bb + 8 * g + 4 * h;
-      float4 n4 = dot ? make_float4(0.f, 0.f, 0.f, 0.f)
-                      : (NPF ? ncur[NPF ? g : 0] : *(const float4 *)(a.bn + b4));
+      float4 n4 = NPF   ? nn4[NPF ? g : 0]
+                  : dot ? make_float4(0.f, 0.f, 0.f, 0.f)
+                        : *(const float4 *)(a.bn + b4);
       float4 w4 = wts ? *(const float4 *)(wts + b4) : make_float4(1.f, 1.f, 1.f, 1.f);
       nn[4 * g] = n4.x; nn[4 * g + 1] = n4.y; nn[4 * g + 2] = n4.z; nn[4 * g + 3] = n4.w;
       ww[4 * g] = w4.x; ww[4 * g + 1] = w4.y; ww[4 * g + 2] = w4.z; ww[4 * g + 3] = w4.w;
@@ -434,6 +440,8 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
         }
       }
     }
+    // the next block's norms (unconditional, as the operands' prefetch)
+    if constexpr (NPF) loadn(bb + a.bstep < bend ? bb + a.bstep : bb);
   }
 
   if (MODE == 0) {
