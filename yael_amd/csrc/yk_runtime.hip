@@ -170,11 +170,104 @@ extern "C" int yk_sync(void) {
 }
 
 // host n x d (dense) -> device n x ld (pad columns zeroed)
+// Large uploads from pageable host memory: YK_STAGE_THREADS host threads
+// each take a contiguous slice and push it through two pinned buffers of
+// YK_STAGE_BYTES on a stream of their own (default priority), the copy into
+// one buffer overlapping the DMA of the other.  The pinned buffers (128 MB
+// in all) are allocated at the first large upload and kept until the
+// process ends; a mutex keeps one staged upload at a time.
+#define YK_STAGE_THREADS 8
+#define YK_STAGE_BYTES ((size_t)8 << 20)
+#define YK_STAGE_MIN ((size_t)64 << 20)
+static pthread_mutex_t g_stage_mu = PTHREAD_MUTEX_INITIALIZER;
+static void *g_stage_buf[YK_STAGE_THREADS][2];
+static hipStream_t g_stage_st[YK_STAGE_THREADS];
+static hipEvent_t g_stage_ev[YK_STAGE_THREADS][2];
+static int g_stage_ready = 0;
+
+struct YkStageJob {
+  char *dst;
+  const char *src;
+  size_t bytes;
+  int t;
+  hipError_t err;
+};
+
+static void *yk_stage_worker(void *arg) {
+  YkStageJob *j = (YkStageJob *)arg;
+  bool used[2] = {false, false};
+  size_t off = 0;
+  for (int b = 0; off < j->bytes; b ^= 1) {
+    const size_t n = j->bytes - off < YK_STAGE_BYTES ? j->bytes - off : YK_STAGE_BYTES;
+    hipError_t e = used[b] ? hipEventSynchronize(g_stage_ev[j->t][b]) : hipSuccess;
+    if (e == hipSuccess) {
+      memcpy(g_stage_buf[j->t][b], j->src + off, n);
+      e = hipMemcpyAsync(j->dst + off, g_stage_buf[j->t][b], n, hipMemcpyHostToDevice,
+                         g_stage_st[j->t]);
+    }
+    if (e == hipSuccess) e = hipEventRecord(g_stage_ev[j->t][b], g_stage_st[j->t]);
+    if (e != hipSuccess) {
+      j->err = e;
+      break;
+    }
+    used[b] = true;
+    off += n;
+  }
+  const hipError_t e = hipStreamSynchronize(g_stage_st[j->t]);
+  if (j->err == hipSuccess) j->err = e;
+  return nullptr;
+}
+
+static int yk_stage_setup(void) {
+  if (g_stage_ready) return YK_OK;
+  for (int t = 0; t < YK_STAGE_THREADS; t++) {
+    for (int b = 0; b < 2; b++) {
+      YK_HIP(hipHostMalloc(&g_stage_buf[t][b], YK_STAGE_BYTES, hipHostMallocDefault));
+      YK_HIP(hipEventCreateWithFlags(&g_stage_ev[t][b], hipEventDisableTiming));
+    }
+    YK_HIP(hipStreamCreateWithFlags(&g_stage_st[t], hipStreamNonBlocking));
+  }
+  g_stage_ready = 1;
+  return YK_OK;
+}
+
+static int yk_h2d_staged(void *dst, const void *src, size_t bytes) {
+  YK_HIP(hipStreamSynchronize(g_stream));  // earlier work on dst has finished
+  pthread_mutex_lock(&g_stage_mu);
+  int rc = yk_stage_setup();
+  if (rc == YK_OK) {
+    const size_t per = (bytes / YK_STAGE_THREADS + 4095) & ~(size_t)4095;
+    YkStageJob job[YK_STAGE_THREADS];
+    pthread_t th[YK_STAGE_THREADS];
+    int started[YK_STAGE_THREADS] = {0}, nt = 0;
+    for (int t = 0; t < YK_STAGE_THREADS && (size_t)t * per < bytes; t++, nt++) {
+      const size_t o = (size_t)t * per;
+      job[t] = {(char *)dst + o, (const char *)src + o, bytes - o < per ? bytes - o : per, t,
+                hipSuccess};
+    }
+    for (int t = 1; t < nt; t++)
+      started[t] = pthread_create(&th[t], nullptr, yk_stage_worker, &job[t]) == 0;
+    yk_stage_worker(&job[0]);
+    for (int t = 1; t < nt; t++) {
+      if (started[t])
+        pthread_join(th[t], nullptr);
+      else
+        yk_stage_worker(&job[t]);  // a thread that could not start: its slice here
+    }
+    for (int t = 0; t < nt && rc == YK_OK; t++)
+      if (job[t].err != hipSuccess) rc = yk_hip_fail(job[t].err, "staged upload");
+  }
+  pthread_mutex_unlock(&g_stage_mu);
+  return rc;
+}
+
 extern "C" int yk_upload_vectors(float *dst, const float *src, long n, int d) {
   YK_TRY(yk_init());
   if (n <= 0) return YK_OK;
   int ld = yk_ld(d);
-  if (ld == d) {
+  if (ld == d && sizeof(float) * (size_t)n * d >= YK_STAGE_MIN) {
+    return yk_h2d_staged(dst, src, sizeof(float) * (size_t)n * d);
+  } else if (ld == d) {
     YK_HIP(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)n * d, hipMemcpyHostToDevice,
                           g_stream));
   } else {
