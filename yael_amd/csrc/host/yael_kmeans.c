@@ -219,7 +219,11 @@ static int yh_kmeans_core(int d, int n, int k, int niter, int flags, int verbose
     if ((rc = yk_kmeans_update(n, k, d, m->v, m->assign, m->c, m->counts,
                                (flags & KMEANS_NORMALIZE_CENTS) != 0, m->ws)) != YK_OK)
       return rc;
-    if ((rc = yk_memcpy_d2h(hcounts, m->counts, sizeof(int) * k)) != YK_OK) return rc;
+    /* the counts and this assign step's objective (kmeans.c:300-305: the
+       distances to the centroids before the update), one wait */
+    qerr_old = qerr;
+    if ((rc = yk_kmeans_readback(m->counts, k, hcounts, m->dis, n, &qerr, m->obj)) != YK_OK)
+      return rc;
     int nempty = 0;
     for (int c = 0; c < k; c++) nempty += hcounts[c] == 0;
     unsigned split_seed = (unsigned)rand_r(&seed);
@@ -235,8 +239,6 @@ static int yh_kmeans_core(int d, int n, int k, int niter, int flags, int verbose
       fprintf(stderr, "# kmeans: reassigned %d times, abandoning\n", tot);
       return YK_EABANDON;
     }
-    qerr_old = qerr;
-    if ((rc = yk_sum_f64(m->dis, n, &qerr, m->obj)) != YK_OK) return rc;
     if (qerr_old == qerr && nre == 0) break;
     if (verbose) {
       printf(" -> %.3f", qerr / n);

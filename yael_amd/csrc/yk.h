@@ -130,6 +130,10 @@ size_t yk_kmeans_workspace_size(long n, int k, int d);
 int yk_kmeans_update(long n, int k, int d, const float *v, const int *assign,
                      float *cent, int *counts, int normalize, void *ws);
 int yk_sum_f64(const float *x, long n, double *out_host, void *ws);
+/* counts_host[k] = counts, *obj_host = the FP64 objective of dis (as
+ * yk_sum_f64, ws its workspace), with one synchronisation */
+int yk_kmeans_readback(const int *counts, int k, int *counts_host, const float *dis, long n,
+                       double *obj_host, void *ws);
 /* k-means++ step: best[j] = min(best[j], squared distance of point j to
  * point s) for j < n (first: best starts at HUGE_VAL); vn: the points'
  * double norms (yk_norms out_d); host_copy (optional, from yk_host_alloc)

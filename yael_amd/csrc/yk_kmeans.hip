@@ -236,6 +236,23 @@ extern "C" int yk_sum_f64(const float *x, long n, double *out_host, void *ws) {
   return YK_OK;
 }
 
+// The counts of an update and the objective of its assign step read back
+// with one wait: the objective's kernels are queued behind the update, then
+// both copies, then one synchronisation (ws: the objective's workspace, as
+// for yk_sum_f64).
+extern "C" int yk_kmeans_readback(const int *counts, int k, int *counts_host, const float *dis,
+                                  long n, double *obj_host, void *ws) {
+  YK_TRY(yk_init());
+  double *part = (double *)ws;
+  YK_TRY(yk_sum_f64_async(dis, n, part));
+  YK_HIP(hipMemcpyAsync(counts_host, counts, sizeof(int) * (size_t)k, hipMemcpyDeviceToHost,
+                        yk_stream()));
+  YK_HIP(hipMemcpyAsync(obj_host, part + YK_OBJ_BLOCKS, sizeof(double), hipMemcpyDeviceToHost,
+                        yk_stream()));
+  YK_HIP(hipStreamSynchronize(yk_stream()));
+  return YK_OK;
+}
+
 extern "C" int yk_kmeans_update(long n, int k, int d, const float *v, const int *assign,
                                 float *cent, int *counts, int normalize, void *ws) {
   YK_TRY(yk_init());
