@@ -278,3 +278,26 @@ def test_knn_large_base_offsets(ya):
     assert np.all(d32[:, 0] == 0.0)
     assert np.array_equal(i16, i32)
     assert np.array_equal(d16.view(np.int32), d32.view(np.int32))
+
+
+def test_empty_inputs(ya):
+    """No queries, or an empty side of a cross-distance matrix: empty results
+    (the reference's loops run zero times), in both kNN forms."""
+    rng = np.random.default_rng(5)
+    b = rng.integers(0, 9, (100, 16)).astype(np.float32)
+    q0 = np.zeros((0, 16), np.float32)
+    lib = ya.lib()
+    for mode in (1, 2):
+        old = lib.yael_knn_precision(mode)
+        try:
+            idx, dis = ya.knn(q0, b, 5)
+        finally:
+            lib.yael_knn_precision(old)
+        assert idx.shape == (0, 5) and dis.shape == (0, 5)
+    d0 = ya.cross_distances(q0, b)
+    assert d0.shape == (100, 0)
+    d1 = ya.cross_distances(b, q0)
+    assert d1.shape == (0, 100)
+    # and a call after them still works
+    idx, dis = ya.knn(b[:3], b, 1)
+    assert np.array_equal(idx[:, 0], np.arange(3)) and np.all(dis[:, 0] == 0)
