@@ -301,3 +301,23 @@ def test_empty_inputs(ya):
     # and a call after them still works
     idx, dis = ya.knn(b[:3], b, 1)
     assert np.array_equal(idx[:, 0], np.arange(3)) and np.all(dis[:, 0] == 0)
+
+
+@pytest.mark.parametrize("k", [300, 500])
+def test_knn_full_precision_form_large_k(ya, k):
+    """Large k through the precision form (k' = 2k: candidate lists and the
+    sorts that trim them run past 1024 keys): the FP32 form's bits."""
+    rng = np.random.default_rng(700 + k)
+    cen = rng.uniform(0, 255, (128, 64))
+    b = np.clip(np.rint(cen[rng.integers(0, 128, 60000)] + 25 * rng.standard_normal((60000, 64))),
+                0, 255).astype(np.float32)
+    q = np.clip(np.rint(cen[rng.integers(0, 128, 300)] + 25 * rng.standard_normal((300, 64))),
+                0, 255).astype(np.float32)
+    print("knn_full both forms k=%d" % k, flush=True)
+    i16, d16, i32, d32 = _both_forms(ya, q, b, k)
+    assert np.array_equal(d16.view(np.int32), d32.view(np.int32))
+    # indices may differ only inside runs of equal distances (integer data)
+    bad = np.nonzero((i16 != i32).any(1))[0]
+    for i in bad:
+        for dv in np.unique(d32[i][i16[i] != i32[i]]):
+            assert sorted(i16[i][d16[i] == dv]) == sorted(i32[i][d32[i] == dv])
