@@ -550,7 +550,7 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="profile runs: first copy this many MiB three times with a plain "
                          "copy kernel (the reference point of the HBM traffic counters)")
-    ap.add_argument("--only", choices=["kmeans-large", "d960"], default=None,
+    ap.add_argument("--only", choices=["kmeans-large", "d960", "batched"], default=None,
                     help="profile runs: run one extra workload alone and print its record")
     ap.add_argument("--nb", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
@@ -578,8 +578,11 @@ def main():
         dst.free()
 
     if args.only:
-        rec = bench_kmeans_large(ya, dev) if args.only == "kmeans-large" else bench_precision_d960(ya, dev)
-        print(json.dumps(rec, default=float), flush=True)
+        only = {"kmeans-large": bench_kmeans_large, "d960": bench_precision_d960,
+                "batched": bench_batched}[args.only]
+        rec = only(ya, dev)
+        print(json.dumps(rec, default=lambda x: x.tolist() if hasattr(x, "tolist") else float(x)),
+              flush=True)
         return
 
     d, k = 128, 100

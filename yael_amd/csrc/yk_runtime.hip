@@ -351,11 +351,15 @@ int yk_prof_setup(void) {
 static void yk_prof_flush(void) {
   for (int i = 0; i < g_pev_n; i++) {
     float ms = 0;
-    // a pair whose events failed is left out of the sums
+    // a pair whose events failed is left out of the sums, and its error is
+    // consumed here (HIP keeps the last error for the next hipGetLastError,
+    // which would otherwise report it from an unrelated launch)
     if (hipEventSynchronize(g_pev[i][1]) == hipSuccess &&
         hipEventElapsedTime(&ms, g_pev[i][0], g_pev[i][1]) == hipSuccess) {
       g_prof_ms[g_pev_slot[i]] += ms;
       g_prof_cnt[g_pev_slot[i]]++;
+    } else {
+      (void)hipGetLastError();
     }
   }
   g_pev_n = 0;
@@ -363,6 +367,11 @@ static void yk_prof_flush(void) {
 
 int yk_prof_begin(int slot) {
   if (!g_prof_on) return -1;
+  // no timing inside a stream capture: the records would become nodes of
+  // the captured graph, never completed as ordinary events
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(g_stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return -1;
   if (g_pev_n == YK_PROF_EV) yk_prof_flush();
   const int i = g_pev_n;
   if (hipEventRecord(g_pev[i][0], g_stream) != hipSuccess) return -1;  // not timed
