@@ -184,3 +184,25 @@ def test_pq_subspaces_large_instances(ya, dsub):
     sub = np.concatenate([v[:, j * dsub:(j + 1) * dsub] for j in range(nsub)])
     sizes = np.full(nsub, n, np.int32)
     compare((cent, ret, dis, asg, nas), single(ya, sub, sizes, k, niter, seeds), sizes)
+
+
+def test_profiled_batched_run_leaves_no_error(ya):
+    """Library profiling switched on around a batched (graph-captured) run:
+    no profiling event may be recorded inside the capture, and the next,
+    unrelated calls must not see a stale HIP error."""
+    import ctypes
+    lib = ya.lib()
+    rng = np.random.default_rng(12)
+    sizes = np.array([3000, 2500, 4000], np.int32)
+    v = mixture(rng, int(sizes.sum()), 32, 40, 15)
+    lib.yk_prof_reset()
+    lib.yk_prof_enable(1)
+    try:
+        b = ya.kmeans_batched(v, sizes, 8, niter=4, seeds=np.array([3, 4, 5]))
+    finally:
+        lib.yk_prof_enable(0)
+    tot, cnt = ctypes.c_double(), ctypes.c_long()
+    lib.yk_prof_get(5, ctypes.byref(tot), ctypes.byref(cnt))  # the batch slot: reads the events
+    compare(b, single(ya, v, sizes, 8, 4, [3, 4, 5]), sizes)
+    idx, dis = ya.knn(v[:5], v, 1)
+    assert np.array_equal(idx[:, 0], np.arange(5))
