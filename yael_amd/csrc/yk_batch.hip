@@ -463,10 +463,13 @@ extern "C" int yk_batch_run(yk_batch *bt, int niter, int use_graph) {
 
 extern "C" int yk_batch_destroy(yk_batch *bt) {
   if (!bt) return YK_OK;
+  hipError_t e = hipSuccess, e2 = hipSuccess;
   if (bt->captured) {
-    (void)hipGraphExecDestroy(bt->exec);
-    (void)hipGraphDestroy(bt->graph);
+    e = hipGraphExecDestroy(bt->exec);
+    e2 = hipGraphDestroy(bt->graph);
   }
   delete bt;
+  if (e != hipSuccess) return yk_hip_fail(e, "hipGraphExecDestroy");
+  if (e2 != hipSuccess) return yk_hip_fail(e2, "hipGraphDestroy");
   return YK_OK;
 }

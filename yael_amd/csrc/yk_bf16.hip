@@ -1336,7 +1336,7 @@ static int g_bs_level = 0;
 static int yk_bscratch(int slot, size_t bytes, void **out) {
   slot += 10 * g_bs_level;
   if (bytes > g_bs_bytes[slot]) {
-    if (g_bs[slot]) (void)hipFree(g_bs[slot]);
+    if (g_bs[slot] && hipFree(g_bs[slot]) != hipSuccess) (void)hipGetLastError();
     g_bs[slot] = nullptr;
     g_bs_bytes[slot] = 0;
     hipError_t e = hipMalloc(&g_bs[slot], bytes);

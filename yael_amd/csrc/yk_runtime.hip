@@ -24,6 +24,9 @@ int yk_set_error(int code, const char *fmt, ...) {
 }
 
 int yk_hip_fail(hipError_t e, const char *what) {
+  // the error is reported here; HIP's copy of it is consumed so that a later
+  // hipGetLastError check does not report it again from an unrelated call
+  (void)hipGetLastError();
   return yk_set_error(e == hipErrorOutOfMemory ? YK_ENOMEM : YK_EHIP, "%s: %s", what,
                       hipGetErrorString(e));
 }
@@ -383,7 +386,7 @@ int yk_prof_begin(int slot) {
 void yk_prof_end(int tok) {
   if (tok < 0) return;
   // a failed record leaves the pair unfinished: yk_prof_flush skips it
-  (void)hipEventRecord(g_pev[tok][1], g_stream);
+  if (hipEventRecord(g_pev[tok][1], g_stream) != hipSuccess) (void)hipGetLastError();
 }
 
 extern "C" int yk_prof_enable(int on) {

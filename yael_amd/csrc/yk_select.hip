@@ -645,7 +645,7 @@ static size_t g_scratch_bytes[4];
 
 static int yk_scratch(int slot, size_t bytes, void **out) {
   if (bytes > g_scratch_bytes[slot]) {
-    if (g_scratch[slot]) (void)hipFree(g_scratch[slot]);
+    if (g_scratch[slot] && hipFree(g_scratch[slot]) != hipSuccess) (void)hipGetLastError();
     g_scratch[slot] = nullptr;
     g_scratch_bytes[slot] = 0;
     hipError_t e = hipMalloc(&g_scratch[slot], bytes);
