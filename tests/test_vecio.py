@@ -124,3 +124,39 @@ def test_cli_usage_and_missing_arguments():
         r = subprocess.run([os.path.join(BIN, prog), "-k", "3"], capture_output=True, text=True,
                            timeout=60)
         assert r.returncode != 0 and "required" in r.stderr
+
+
+def test_malformed_files_same_results_as_reference(lib, ref, tmp_path):
+    """A payload cut short, a trailing fragment shorter than a length word,
+    a record of another length and an empty file: sizes, bounded reads and
+    allocating reads return what the reference's own readers return."""
+    for L in (lib, ref):
+        L.fvecs_fsize.restype = ctypes.c_long
+    v = np.arange(40, dtype=np.float32).reshape(8, 5)
+    good = b"".join(np.int32(5).tobytes() + r.tobytes() for r in v)
+    cases = {
+        "good": good,
+        "cut_payload": good[:-3],
+        "cut_len": good + b"\x05\x00",
+        "baddim": good[:72] + np.int32(4).tobytes() + good[76:],
+        "empty": b"",
+    }
+    for name, data in cases.items():
+        p = tmp_path / (name + ".fvecs")
+        p.write_bytes(data)
+        got = []
+        for L in (lib, ref):
+            d, n = ctypes.c_int(), ctypes.c_int()
+            size = L.fvecs_fsize(str(p).encode(), ctypes.byref(d), ctypes.byref(n))
+            out = np.zeros((10, 5), np.float32)
+            r = L.fvecs_read(str(p).encode(), 5, 10, _fp(out))
+            res = [size, d.value, n.value, r, out[:max(r, 0)].tobytes()]
+            if name != "empty":  # the reference's allocating reader is not defined on it
+                fp, d2 = f32p(), ctypes.c_int()
+                r2 = L.fvecs_new_read(str(p).encode(), ctypes.byref(d2), ctypes.byref(fp))
+                res += [r2, d2.value]
+                if r2 > 0:
+                    res.append(np.ctypeslib.as_array(fp, shape=(r2, d2.value)).tobytes())
+                    ctypes.CDLL(None).free(fp)
+            got.append(res)
+        assert got[0] == got[1], name

@@ -1,120 +1,84 @@
 /*
- * kmeans -- command-line k-means on the GPU library, with the options,
- * defaults, stderr report and output files of yael's progs/kmeans.c:
- *   -i file / -itxt file   input vectors (fvecs / text; text needs -n, -d)
- *   -n #, -d #             number of vectors and dimension (fvecs: read
- *                          from the file unless both are given)
- *   -k #, -niter #, -nredo #, -seed #, -nt #
- *   -o file / -otxt file   output centroids (fvecs / text)
- * NaN input values are replaced by 2 (progs/kmeans.c:144-147).  The
- * clustering is kmeans() with flags = nt (progs/kmeans.c:152): the thread
- * bits are accepted and ignored, the run is not quiet.
+ * kmeans -- k-means clustering of a vector file from the command line, on
+ * the GPU library.  Same option names and defaults as the reference's
+ * progs/kmeans.c (its flag list is the interface scripts rely on): input
+ * -i (fvecs) or -itxt (text, needs -n and -d), -n, -d, -k (10), -niter (40),
+ * -nredo (1), -seed (0), -nt (accepted, unused), output -o (fvecs) or -otxt
+ * (text).  The clustering is one kmeans() call with the thread count as
+ * its flags, as the reference program makes it (progs/kmeans.c:152), so the
+ * centroids written are the same.  NaN components of the input count as 2.
  */
-#include <stdio.h>
-#include <stdlib.h>
-#include <string.h>
+#include "cli.h"
 
-#include "../../include/yael_mi355x.h"
-
-#define DEFAULT_NITER 40
-#define DEFAULT_NREDO 1
-
-static void usage(const char *cmd) {
-  printf("Usage: %s [options]\n", cmd);
-  printf("  Input\n"
-         "    -i filename     input set of vectors to cluster (fvec file format)\n"
-         "    -itxt filename  the same but for an input text file\n"
-         "    -n #            use n points from the file. Default: all the vectors\n"
-         "    -d #            dimension of the vector (used with -itxt only)\n"
-         "  Clustering parameters\n"
-         "    -k #            number of centroids to produce\n"
-         "    -niter #        maximum number of iterations. Default: %d\n"
-         "    -nredo #        number of runs to perform. Default: %d\n"
-         "    -seed #         initialize the random generate with a number\n"
-         "    -nt #           number of threads used. Default: nt=all\n"
-         "  Ouput\n"
-         "    -o clustfile    output cluster file (fvec format)\n"
-         "    -otxt clustfile output cluster file (text format)\n",
-         DEFAULT_NITER, DEFAULT_NREDO);
-  exit(0);
-}
-
-static int intarg(const char *s, const char *opt) {
-  int v;
-  if (sscanf(s, "%d", &v) != 1) {
-    fprintf(stderr, "kmeans: bad value '%s' for %s\n", s, opt);
-    exit(1);
-  }
-  return v;
-}
+static const char *what =
+    "Clusters the input vectors into k centroids with k-means and writes the\n"
+    "centroids, one vector each.";
 
 int main(int argc, char **argv) {
-  int k = 10, d = 0, n = 0, niter = DEFAULT_NITER, nredo = DEFAULT_NREDO, nt = 1, seed = 0;
-  int txt_in = 0, txt_out = 0;
-  const char *fi = NULL, *fo = NULL;
-  if (argc == 1) usage(argv[0]);
-  for (int i = 1; i < argc; i++) {
-    const char *a = argv[i];
-    const int more = i + 1 < argc;
-    if (!strcmp(a, "-h") || !strcmp(a, "--help")) usage(argv[0]);
-    if (!strcmp(a, "-k") && more) k = intarg(argv[++i], a);
-    else if (!strcmp(a, "-d") && more) d = intarg(argv[++i], a);
-    else if (!strcmp(a, "-n") && more) n = intarg(argv[++i], a);
-    else if (!strcmp(a, "-niter") && more) niter = intarg(argv[++i], a);
-    else if (!strcmp(a, "-nt") && more) nt = intarg(argv[++i], a);
-    else if (!strcmp(a, "-nredo") && more) nredo = intarg(argv[++i], a);
-    else if (!strcmp(a, "-seed") && more) seed = intarg(argv[++i], a);
-    else if (!strcmp(a, "-i") && more) fi = argv[++i], txt_in = 0;
-    else if (!strcmp(a, "-itxt") && more) fi = argv[++i], txt_in = 1;
-    else if (!strcmp(a, "-o") && more) fo = argv[++i], txt_out = 0;
-    else if (!strcmp(a, "-otxt") && more) fo = argv[++i], txt_out = 1;
-  }
-  if (!fi || !fo) {
-    fprintf(stderr, "kmeans: an input (-i/-itxt) and an output (-o/-otxt) are required\n");
-    return 1;
-  }
-  fprintf(stderr, "k = %d\nd = %d\nn = %d\nniter = %d\nnredo = %d\n", k, d, n, niter, nredo);
-  fprintf(stderr, "nt = %d\nseed = %d\n", nt, seed);
-  fprintf(stderr, "fi = %s  (fmt = %s)\n", fi, txt_in ? "txt" : "fvec");
-  fprintf(stderr, "fo = %s  (fmt = %s)\n", fo, txt_out ? "txt" : "fvec");
+  const char *prog = "kmeans";
+  int k = 10, d = 0, n = 0, niter = 40, nredo = 1, seed = 0, nt = 0, verbose = 1;
+  const char *in = NULL, *out = NULL;
+  cli_form in_form = CLI_FVECS, out_form = CLI_FVECS;
 
-  int got;
-  float *v;
-  if (!txt_in) {
-    if (d == 0 || n == 0) {
-      const long bytes = fvecs_fsize(fi, &d, &n);
-      fprintf(stderr, "File %s contains (%ld bytes) %d vectors of dimension %d\n", fi, bytes, n, d);
-      if (bytes <= 0) return 1;
-    }
-    v = (float *)malloc(sizeof(float) * (size_t)n * d);
-    got = v ? fvecs_read(fi, d, n, v) : -1;
-  } else {
-    v = (float *)malloc(sizeof(float) * (size_t)n * d + 1);
-    got = v ? fvecs_read_txt(fi, d, n, v) : -1;
+  const cli_opt opts[] = {
+      {"Input", CLI_SECTION, NULL, NULL, 0, NULL},
+      {"-i", CLI_PATH, &in, &in_form, CLI_FVECS, "vectors to cluster, fvecs"},
+      {"-itxt", CLI_PATH, &in, &in_form, CLI_TEXT, "vectors to cluster, text (needs -n, -d)"},
+      {"-n", CLI_INT, &n, NULL, 0, "use the first N vectors (default: all)"},
+      {"-d", CLI_INT, &d, NULL, 0, "dimension (needed for text input)"},
+      {"Clustering", CLI_SECTION, NULL, NULL, 0, NULL},
+      {"-k", CLI_INT, &k, NULL, 0, "number of centroids (default 10)"},
+      {"-niter", CLI_INT, &niter, NULL, 0, "most Lloyd iterations per run (default 40)"},
+      {"-nredo", CLI_INT, &nredo, NULL, 0, "runs; the best one is kept (default 1)"},
+      {"-seed", CLI_INT, &seed, NULL, 0, "random seed (default 0)"},
+      {"-nt", CLI_INT, &nt, NULL, 0, "host threads (accepted; the GPU does the work)"},
+      {"Output", CLI_SECTION, NULL, NULL, 0, NULL},
+      {"-o", CLI_PATH, &out, &out_form, CLI_FVECS, "centroids, fvecs"},
+      {"-otxt", CLI_PATH, &out, &out_form, CLI_TEXT, "centroids, text"},
+  };
+  const int nopts = (int)(sizeof opts / sizeof opts[0]);
+
+  if (argc < 2) {
+    cli_help(stdout, prog, what, opts, nopts);
+    return 0;
   }
-  if (got < n || n < 1 || d < 1) {
-    fprintf(stderr, "kmeans: read %d of %d vectors from %s\n", got, n, fi);
-    return 1;
+  const int pr = cli_parse(prog, what, opts, nopts, argc, argv);
+  if (pr) return pr > 0 ? 0 : 2;
+  if (!in || !out) {
+    fprintf(stderr, "%s: an input file (-i or -itxt) and an output file (-o or -otxt) are "
+                    "required\n", prog);
+    return 2;
   }
-  const long nnan = fvec_purge_nans(v, (long)n * d, 2);
-  if (nnan > 0) fprintf(stderr, "found %ld NaN values\n", nnan);
+  if (k < 1 || nredo < 1 || niter < 0 || nt < 0) {
+    fprintf(stderr, "%s: need k >= 1, nredo >= 1, niter >= 0, nt >= 0\n", prog);
+    return 2;
+  }
+
+  float *v = cli_load(prog, in, in_form, &n, &d, 2.0f, verbose);
+  if (!v) return 1;
+  if (k > n) {
+    fprintf(stderr, "%s: k=%d exceeds the %d input vectors\n", prog, k, n);
+    return 2;
+  }
+  fprintf(stderr, "%s: %d vectors, d=%d, k=%d, niter=%d, nredo=%d, seed=%d -> %s (%s)\n", prog,
+          n, d, k, niter, nredo, seed, out, cli_form_name(out_form));
 
   float *cent = (float *)malloc(sizeof(float) * (size_t)k * d);
-  int *nassign = (int *)malloc(sizeof(int) * (size_t)k);
-  float err = 0;
-  const int rc = yael_kmeans_checked(d, n, k, niter, v, nt, seed, nredo, cent, NULL, NULL,
-                                     nassign, &err);
-  if (rc != 0) {
-    fprintf(stderr, "kmeans: %s\n", yk_last_error());
+  int *sizes = (int *)malloc(sizeof(int) * (size_t)k);
+  if (!cent || !sizes) {
+    fprintf(stderr, "%s: no memory for %d centroids\n", prog, k);
     return 1;
   }
-  const int wrote = txt_out ? fvecs_write_txt(fo, d, k, cent) : fvecs_write(fo, d, k, cent);
-  if (wrote != k) {
-    fprintf(stderr, "kmeans: could not write %s\n", fo);
+  float mean_err = 0;
+  if (yael_kmeans_checked(d, n, k, niter, v, nt & 0xffff, seed, nredo, cent, NULL, NULL, sizes,
+                          &mean_err) != 0) {
+    fprintf(stderr, "%s: clustering failed: %s\n", prog, yk_last_error());
     return 1;
   }
+  fprintf(stderr, "%s: mean squared distance to the nearest centroid %g\n", prog, mean_err);
+  const int rc = cli_store(prog, out, out_form, d, k, cent, 0);
   free(cent);
-  free(nassign);
+  free(sizes);
   free(v);
-  return 0;
+  return rc ? 1 : 0;
 }
