@@ -1329,6 +1329,10 @@ __global__ void yk_gather_f64_kernel(const double *__restrict__ src, const int *
 // Scratch buffers kept across calls, slots 0..9 per nesting level (the
 // escalation of uncertified queries runs the precision form again, one level
 // down, while the outer level still holds its own buffers).
+// bytes rounded up to a whole number of 256-byte units (scratch pieces
+// carved from one buffer each start on such a boundary)
+static size_t yk_b256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 static void *g_bs[20];
 static size_t g_bs_bytes[20];
 static int g_bs_level = 0;
@@ -1551,6 +1555,7 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
                            const float *b, const float *bn, const uint16_t *bh,
                            const float *bstats_max, int *assign, float *dis, int *n_fallback) {
   YK_TRY(yk_init());
+  YkGuard guard;  // process-wide scratch (g_bs, the escalation level) is shared
   if (type == 12) type = YK_L2;
   if (type != YK_L2 && type != YK_DOT)
     return yk_set_error(YK_EINVAL, "distance type %d has no GPU path (2, 12, 16 only)", type);
@@ -1651,17 +1656,21 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
     // to the FP32 form there.  The result is the FP32 form's either way.
     const int kp2 = (int)min((long)min(8 * kp, YK_KMAX), nb);
     if (hn > 0 && g_bs_level == 0 && kp2 > kp) {
-      const size_t szq = sizeof(float) * (size_t)hn * ld, szn = sizeof(double) * hn,
-                   szh = sizeof(uint16_t) * (size_t)hn * ldh, szs = sizeof(float4) * hn;
+      // each piece starts on a 256-byte boundary: the gathers write and the
+      // nested search reads them with 16-byte vector accesses
+      const size_t szq = yk_b256(sizeof(float) * (size_t)hn * ld), szn = yk_b256(sizeof(double) * hn),
+                   szh = yk_b256(sizeof(uint16_t) * (size_t)hn * ldh),
+                   szs = yk_b256(sizeof(float4) * hn);
       void *vg, *voi;
-      YK_TRY(yk_bscratch(6, szq + szn + szh + szs + 256, &vg));
-      YK_TRY(yk_bscratch(7, (sizeof(int) + sizeof(float)) * (size_t)hn * k + 256, &voi));
+      YK_TRY(yk_bscratch(6, szq + szn + szh + szs, &vg));
+      YK_TRY(yk_bscratch(7, yk_b256(sizeof(int) * (size_t)hn * k) + sizeof(float) * (size_t)hn * k,
+                         &voi));
       float *gq = (float *)vg;
       double *gqn = (double *)((char *)vg + szq);
       uint16_t *gqh = (uint16_t *)((char *)gqn + szn);
       float4 *gqs = (float4 *)((char *)gqh + szh);
       int *foi = (int *)voi;
-      float *fod = (float *)(foi + (size_t)hn * k);
+      float *fod = (float *)((char *)voi + yk_b256(sizeof(int) * (size_t)hn * k));
       YK_TRY(yk_gather_rows(q + q0 * ld, unc, hn, d, gq));
       if (type != YK_DOT)
         hipLaunchKernelGGL(yk_gather_f64_kernel, dim3((hn + 255) / 256), dim3(256), 0, yk_stream(),
@@ -1688,13 +1697,15 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
     } else if (hn > 0) {
       // uncertified queries: the FP32 form on their rows
       void *vq, *vqn, *voi, *vod;
-      YK_TRY(yk_bscratch(6, sizeof(float) * (size_t)hn * ld + sizeof(double) * hn + 256, &vq));
-      YK_TRY(yk_bscratch(7, (sizeof(int) + sizeof(float)) * (size_t)hn * k + 256, &voi));
+      const size_t szq = yk_b256(sizeof(float) * (size_t)hn * ld);
+      YK_TRY(yk_bscratch(6, szq + sizeof(double) * hn, &vq));
+      YK_TRY(yk_bscratch(7, yk_b256(sizeof(int) * (size_t)hn * k) + sizeof(float) * (size_t)hn * k,
+                         &voi));
       float *fq = (float *)vq;
-      double *fqn = (double *)(fq + (size_t)hn * ld);
+      double *fqn = (double *)((char *)vq + szq);
       vqn = fqn;
       int *foi = (int *)voi;
-      float *fod = (float *)(foi + (size_t)hn * k);
+      float *fod = (float *)((char *)voi + yk_b256(sizeof(int) * (size_t)hn * k));
       vod = fod;
       YK_TRY(yk_gather_rows(q + q0 * ld, unc, hn, d, fq));
       if (type != YK_DOT)

@@ -17,6 +17,18 @@ int yk_resident_wgs(const void *fn, int block);
 int yk_prof_begin(int slot);
 void yk_prof_end(int tok);
 
+// The library lock (recursive): the device-level searches take it for the
+// whole call, because they share process-wide scratch buffers; the host
+// layer's calls nest inside it.
+void yk_lock(void);
+void yk_unlock(void);
+struct YkGuard {
+  YkGuard() { yk_lock(); }
+  ~YkGuard() { yk_unlock(); }
+  YkGuard(const YkGuard &) = delete;
+  YkGuard &operator=(const YkGuard &) = delete;
+};
+
 // profiling slots
 enum { YK_PROF_FUSED = 0, YK_PROF_NORM = 1, YK_PROF_UPDATE = 2, YK_PROF_MERGE = 3,
        YK_PROF_RERANK = 4, YK_PROF_BATCH = 5, YK_PROF_BF16 = 6, YK_PROF_BF16_SKETCH = 7,

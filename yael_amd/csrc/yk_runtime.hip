@@ -13,6 +13,10 @@ static __thread char g_err[512];
 static hipStream_t g_stream = nullptr;
 static int g_ready = 0;
 static pthread_mutex_t g_init_mu = PTHREAD_MUTEX_INITIALIZER;
+static pthread_mutex_t g_lib_mu = PTHREAD_RECURSIVE_MUTEX_INITIALIZER_NP;
+
+void yk_lock(void) { pthread_mutex_lock(&g_lib_mu); }
+void yk_unlock(void) { pthread_mutex_unlock(&g_lib_mu); }
 static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
 
 int yk_set_error(int code, const char *fmt, ...) {
@@ -373,11 +377,17 @@ int yk_prof_begin(int slot) {
   // no timing inside a stream capture: the records would become nodes of
   // the captured graph, never completed as ordinary events
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(g_stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+  if (hipStreamIsCapturing(g_stream, &cs) != hipSuccess) {
+    (void)hipGetLastError();  // consumed: the launch goes ahead untimed
     return -1;
+  }
+  if (cs != hipStreamCaptureStatusNone) return -1;
   if (g_pev_n == YK_PROF_EV) yk_prof_flush();
   const int i = g_pev_n;
-  if (hipEventRecord(g_pev[i][0], g_stream) != hipSuccess) return -1;  // not timed
+  if (hipEventRecord(g_pev[i][0], g_stream) != hipSuccess) {
+    (void)hipGetLastError();  // consumed: the launch goes ahead untimed
+    return -1;
+  }
   g_pev_slot[i] = slot;
   g_pev_n++;
   return i;
@@ -430,12 +440,19 @@ int yk_resident_wgs(const void *fn, int block) {
   if (!res) {
     if (!g_cu_count) {
       hipDeviceProp_t p;
-      g_cu_count = hipGetDeviceProperties(&p, 0) == hipSuccess ? p.multiProcessorCount : 256;
+      if (hipGetDeviceProperties(&p, 0) == hipSuccess) {
+        g_cu_count = p.multiProcessorCount;
+      } else {
+        (void)hipGetLastError();  // consumed: the grid is sized for 256 CUs instead
+        g_cu_count = 256;
+      }
     }
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0) != hipSuccess ||
-        per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, 0) != hipSuccess) {
+      (void)hipGetLastError();  // consumed: one workgroup per CU is assumed instead
       per_cu = 1;
+    }
+    if (per_cu < 1) per_cu = 1;
     res = per_cu * g_cu_count;
     res = res < 8 ? 8 : res & ~7;
     if (g_occ_n < YK_OCC_CACHE) {
