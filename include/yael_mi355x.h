@@ -220,4 +220,37 @@ int yael_kmeans_subspaces_device(int d, int n, int nsub, int k, int niter, const
 /* ---------------- part 3: device-level layer ---------------- */
 #include "../yael_amd/csrc/yk.h"
 
+/* ---------------- part 4: search without allocation ---------------- */
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Size of the static device array the no-allocation search takes its
+ * scratch from (every GPU the library runs on holds one copy until the
+ * process ends). */
+#define YK_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* yk_knn_f32 without allocating: the same arguments (device pointers, the
+ * library's stream) and results bit for bit, with the search's scratch
+ * (partial keys, thresholds, counters, candidate lists) taken from the
+ * static 64 MiB array above instead of library-allocated buffers; no
+ * hipMalloc, hipFree or allocating library call is made (the first call on
+ * a device looks up that device's copy of the array).  Queries go in passes
+ * sized to fit the array; a shape whose scratch cannot fit even in a pass
+ * of 32 queries is rejected with YK_EINVAL.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The array keeps the last call's contents and is shared by
+ * every call on that GPU: calls are ordered on the library's stream and
+ * serialised by the library lock; kernels queued from another stream
+ * while a call's kernels run would race on it. */
+int yk_knn_f32_noalloc(int type, long nq, long nb, int d, int k, const float *q,
+                       const double *qn, const float *b, const float *bn, const float *w,
+                       int *assign, float *dis);
+
+#ifdef __cplusplus
+}
+#endif
+
 #endif

@@ -79,6 +79,7 @@ PROTOS = {
     "yk_sync": (I, []),
     "yk_norms": (I, [vp, L, I, vp, vp]),
     "yk_knn_f32": (I, [I, L, L, I, I, vp, vp, vp, vp, vp, vp, vp]),
+    "yk_knn_f32_noalloc": (I, [I, L, L, I, I, vp, vp, vp, vp, vp, vp, vp]),
     "yk_cross_distances": (I, [I, L, L, I, vp, vp, vp, vp, vp, L]),
     "yk_bf16_ld": (I, [I, I]),
     "yk_to_bf16": (I, [vp, L, I, I, I, vp, vp, vp, vp]),

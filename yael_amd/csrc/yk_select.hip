@@ -45,8 +45,10 @@
 // again.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 
+#include "../../include/yael_mi355x.h"
 #include "yk.h"
 #include "yk_dev.h"
 
@@ -639,11 +641,32 @@ extern "C" int yk_knn_plan(long nq, long nb, int d, int k, int *nsplit, int *wav
   return YK_OK;
 }
 
-// grow-only device scratch of the selection pipeline
+// Device scratch of the selection pipeline.  Two sources: the grow-only
+// buffers the library allocates and keeps (arena == nullptr: slot i is one
+// buffer, reallocated when a call needs more), or a caller-given region
+// carved front to back in 256-byte-aligned pieces (the no-allocation entry
+// point's static array).  A request that does not fit the region is an
+// error, never an allocation.
 static void *g_scratch[4];
 static size_t g_scratch_bytes[4];
 
-static int yk_scratch(int slot, size_t bytes, void **out) {
+struct YkArena {
+  char *base;
+  size_t cap, used;
+};
+
+static size_t yk_align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static int yk_scratch(YkArena *ar, int slot, size_t bytes, void **out) {
+  if (ar) {
+    const size_t at = yk_align256(ar->used);
+    if (at > ar->cap || bytes > ar->cap - at)
+      return yk_set_error(YK_EINVAL, "scratch request of %zu bytes exceeds the %zu-byte region",
+                          bytes, ar->cap);
+    *out = ar->base + at;
+    ar->used = at + bytes;
+    return YK_OK;
+  }
   if (bytes > g_scratch_bytes[slot]) {
     if (g_scratch[slot] && hipFree(g_scratch[slot]) != hipSuccess) (void)hipGetLastError();
     g_scratch[slot] = nullptr;
@@ -705,30 +728,40 @@ static int yk_sortsel(const uint64_t *src, long nq, int n, int k, int *oi, float
   return YK_OK;
 }
 
+// Sketch stride of the k > 16 path: every lane's minimum over its rows of
+// a 1-in-stride sample of base blocks, with >= k splits -> >= 2k distinct
+// real keys per query.  The k-th smallest of 2k subset minima sits near
+// global rank 1.4 * stride * k, the expected candidate count; ~800 keeps
+// the filtered pass's append path rare and the list far from overflow.
+static long yk_sketch_stride(long nb, int k) {
+  long stride = 600L / k;
+  if (stride > 8) stride = 8;
+  if (stride < 1) stride = 1;
+  const long nblocks = (nb + 31) / 32;
+  while (stride > 1 && nblocks / stride < 2L * k) stride >>= 1;
+  return stride;
+}
+
+// queries per pass of the k > 16 path (bounds the candidate memory)
+#define YK_LARGE_K_QB 16384L
+
 // k > 16: sketch threshold, filtered scan, per-query sort (queries in batches
 // that bound the candidate memory).
-static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *dis) {
-  const long QB = 16384;
+static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *dis,
+                          YkArena *ar) {
+  const long QB = YK_LARGE_K_QB;
   const long nq_all = base.nq;
   void *vthr, *vcnt, *vcand, *vpart, *vflag;
   const long qb = nq_all < QB ? nq_all : QB;
-  YK_TRY(yk_scratch(0, sizeof(uint64_t) * qb, &vthr));
-  YK_TRY(yk_scratch(1, sizeof(int) * (qb + 64), &vcnt));
-  YK_TRY(yk_scratch(2, sizeof(uint64_t) * qb * YK_CAND_CAP, &vcand));
+  YK_TRY(yk_scratch(ar, 0, sizeof(uint64_t) * qb, &vthr));
+  YK_TRY(yk_scratch(ar, 1, sizeof(int) * (qb + 64), &vcnt));
+  YK_TRY(yk_scratch(ar, 2, sizeof(uint64_t) * qb * YK_CAND_CAP, &vcand));
+  const size_t fixed_used = ar ? ar->used : 0;
   uint64_t *thr = (uint64_t *)vthr;
   int *cnt = (int *)vcnt;
   int *flag = cnt + qb + 32;
   vflag = flag;
-  // Sketch: every lane's minimum over its rows of a 1-in-stride sample of
-  // base blocks, with >= k splits -> >= 2k distinct real keys per query.
-  // The k-th smallest of 2k subset minima sits near global rank
-  // 1.4 * stride * k, the expected candidate count; ~800 keeps the filtered
-  // pass's append path rare and the list far from overflow.
-  long stride = 600L / k;
-  if (stride > 8) stride = 8;
-  if (stride < 1) stride = 1;
-  const long nblocks = (base.nb + 31) / 32;
-  while (stride > 1 && nblocks / stride < 2L * k) stride >>= 1;
+  const long stride = yk_sketch_stride(base.nb, k);
   const long min_splits = k;
 
   for (long q0 = 0; q0 < nq_all; q0 += QB) {
@@ -739,7 +772,8 @@ static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *d
     a.nq = nq;
     // (A) sketch: per-lane minima over a strided sample
     YkPlan pa = yk_plan(nq, a.nb, stride, min_splits, min_splits > 256 ? min_splits : 256);
-    YK_TRY(yk_scratch(3, sizeof(uint64_t) * nq * pa.nsplit * 2, &vpart));
+    if (ar) ar->used = fixed_used;  // each pass reuses the region's partial-key piece
+    YK_TRY(yk_scratch(ar, 3, sizeof(uint64_t) * nq * pa.nsplit * 2, &vpart));
     a.part = (uint64_t *)vpart;
     YK_TRY((yk_launch_scan<1, 1>(pa, a, qres)));
     YK_TRY(yk_sortsel(a.part, nq, pa.nsplit * 2, k, nullptr, nullptr, thr));
@@ -773,17 +807,22 @@ static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *d
   return YK_OK;
 }
 
-extern "C" int yk_knn_f32(int type, long nq, long nb, int d, int k, const float *q,
-                          const double *qn, const float *b, const float *bn, const float *w,
-                          int *assign, float *dis) {
-  YK_TRY(yk_init());
-  if (type == 12) type = YK_L2;
-  if (type != YK_L2 && type != YK_DOT)
-    return yk_set_error(YK_EINVAL, "distance type %d has no GPU path (2, 12, 16 only)", type);
+// Checks shared by the FP32 entry points; 1 when there is nothing to do.
+static int yk_knn_f32_check(int *type, long nq, long nb, int k) {
+  if (*type == 12) *type = YK_L2;
+  if (*type != YK_L2 && *type != YK_DOT)
+    return yk_set_error(YK_EINVAL, "distance type %d has no GPU path (2, 12, 16 only)", *type);
   if (k < 1 || k > nb || k > YK_KMAX)
     return yk_set_error(YK_EINVAL, "k=%d out of range (nb=%ld, max %d)", k, nb, YK_KMAX);
-  if (nq <= 0) return YK_OK;
   if (nb > 0x7fffffffL || nq > 0x7fffffffL) return yk_set_error(YK_EINVAL, "sizes exceed int32");
+  return nq <= 0 ? 1 : YK_OK;
+}
+
+// The FP32 search of nq queries, scratch from ar (nullptr: the grow-only
+// buffers).
+static int yk_knn_f32_run(int type, long nq, long nb, int d, int k, const float *q,
+                          const double *qn, const float *b, const float *bn, const float *w,
+                          int *assign, float *dis, YkArena *ar) {
   ScanArgs a = {};
   a.q = q;
   a.qn = qn;
@@ -801,7 +840,7 @@ extern "C" int yk_knn_f32(int type, long nq, long nb, int d, int k, const float 
     YkPlan p = yk_plan(nq, nb, 1, 1, 4096);
     if (p.nsplit > 1) {
       void *buf;
-      YK_TRY(yk_scratch(3, sizeof(uint64_t) * nq * p.nsplit, &buf));
+      YK_TRY(yk_scratch(ar, 3, sizeof(uint64_t) * nq * p.nsplit, &buf));
       a.part = (uint64_t *)buf;
     }
     YK_TRY((yk_launch_scan<0, 1>(p, a, qres)));
@@ -817,12 +856,110 @@ extern "C" int yk_knn_f32(int type, long nq, long nb, int d, int k, const float 
   if (k <= YK_REG_R) {
     YkPlan p = yk_plan(nq, nb, 1, 1, 256);
     void *buf;
-    YK_TRY(yk_scratch(3, sizeof(uint64_t) * nq * p.nsplit * 2 * YK_REG_R, &buf));
+    YK_TRY(yk_scratch(ar, 3, sizeof(uint64_t) * nq * p.nsplit * 2 * YK_REG_R, &buf));
     a.part = (uint64_t *)buf;
     YK_TRY((yk_launch_scan<2, YK_REG_R>(p, a, qres)));
     return yk_sortsel(a.part, nq, p.nsplit * 2 * YK_REG_R, k, assign, dis, nullptr);
   }
-  return yk_knn_large_k(a, k, qres, assign, dis);
+  return yk_knn_large_k(a, k, qres, assign, dis, ar);
+}
+
+extern "C" int yk_knn_f32(int type, long nq, long nb, int d, int k, const float *q,
+                          const double *qn, const float *b, const float *bn, const float *w,
+                          int *assign, float *dis) {
+  YK_TRY(yk_init());
+  YkGuard guard;
+  const int c = yk_knn_f32_check(&type, nq, nb, k);
+  if (c != YK_OK) return c > 0 ? YK_OK : c;
+  return yk_knn_f32_run(type, nq, nb, d, k, q, qn, b, bn, w, assign, dis, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// the FP32 search without allocation: scratch in a static device array
+
+// Every GPU the library runs on holds one copy of this array for the life
+// of the process.  Host code never passes its name to a kernel (on the host
+// it is only a stand-in for the symbol): the device address of each GPU's
+// copy comes from hipGetSymbolAddress, taken once per device.
+static __device__ __align__(256) unsigned char scratch[YK_NOALLOC_SCRATCH_BYTES];
+
+#define YK_NOALLOC_MAX_DEV 64
+static void *g_noalloc_ptr[YK_NOALLOC_MAX_DEV];
+static pthread_mutex_t g_noalloc_mu = PTHREAD_MUTEX_INITIALIZER;
+
+// the current device's copy of the array (set-up on the first call there)
+static int yk_noalloc_region(char **out) {
+  int dev = -1;
+  YK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= YK_NOALLOC_MAX_DEV)
+    return yk_set_error(YK_EINVAL, "device %d beyond the %d the scratch table covers", dev,
+                        YK_NOALLOC_MAX_DEV);
+  pthread_mutex_lock(&g_noalloc_mu);
+  int rc = YK_OK;
+  if (!g_noalloc_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+      rc = yk_hip_fail(e, "hipGetSymbolAddress(scratch)");
+    else if (!ptr)
+      rc = yk_set_error(YK_EHIP, "hipGetSymbolAddress(scratch) gave a null address");
+    else
+      g_noalloc_ptr[dev] = ptr;
+  }
+  *out = (char *)g_noalloc_ptr[dev];
+  pthread_mutex_unlock(&g_noalloc_mu);
+  return rc;
+}
+
+// Scratch bytes one yk_knn_f32_run over nq queries takes from a region (the
+// same plans and pieces as the run itself, each rounded up to 256 bytes).
+static size_t yk_knn_f32_need(long nq, long nb, int k) {
+  if (k == 1) {
+    const YkPlan p = yk_plan(nq, nb, 1, 1, 4096);
+    return p.nsplit > 1 ? yk_align256(sizeof(uint64_t) * nq * p.nsplit) : 0;
+  }
+  if (k <= YK_REG_R) {
+    const YkPlan p = yk_plan(nq, nb, 1, 1, 256);
+    return yk_align256(sizeof(uint64_t) * nq * p.nsplit * 2 * YK_REG_R);
+  }
+  const long qb = nq < YK_LARGE_K_QB ? nq : YK_LARGE_K_QB;
+  const long stride = yk_sketch_stride(nb, k);
+  const YkPlan pa = yk_plan(qb, nb, stride, k, k > 256 ? k : 256);
+  return yk_align256(sizeof(uint64_t) * qb) + yk_align256(sizeof(int) * (qb + 64)) +
+         yk_align256(sizeof(uint64_t) * qb * YK_CAND_CAP) +
+         yk_align256(sizeof(uint64_t) * qb * pa.nsplit * 2);
+}
+
+extern "C" int yk_knn_f32_noalloc(int type, long nq, long nb, int d, int k, const float *q,
+                                  const double *qn, const float *b, const float *bn,
+                                  const float *w, int *assign, float *dis) {
+  YK_TRY(yk_init());
+  YkGuard guard;
+  const int c = yk_knn_f32_check(&type, nq, nb, k);
+  if (c != YK_OK) return c > 0 ? YK_OK : c;
+  char *region = nullptr;
+  YK_TRY(yk_noalloc_region(&region));
+  // Queries go in passes small enough for the region: the largest
+  // power-of-two pass (at most 16384 queries) whose scratch fits.  Each
+  // query's result depends only on that query, so the passes give the
+  // single call's results.
+  long pass = 16384;
+  while (pass > 32 && yk_knn_f32_need(pass < nq ? pass : nq, nb, k) > YK_NOALLOC_SCRATCH_BYTES)
+    pass >>= 1;
+  const size_t need = yk_knn_f32_need(pass < nq ? pass : nq, nb, k);
+  if (need > YK_NOALLOC_SCRATCH_BYTES)
+    return yk_set_error(YK_EINVAL,
+                        "k=%d, nb=%ld needs %zu scratch bytes even for %ld queries at a time; "
+                        "the no-allocation search holds %zu",
+                        k, nb, need, pass, (size_t)YK_NOALLOC_SCRATCH_BYTES);
+  const int ld = yk_ld(d);
+  for (long q0 = 0; q0 < nq; q0 += pass) {
+    const long n = nq - q0 < pass ? nq - q0 : pass;
+    YkArena ar = {region, (size_t)YK_NOALLOC_SCRATCH_BYTES, 0};
+    YK_TRY(yk_knn_f32_run(type, n, nb, d, k, q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, w,
+                          assign + q0 * k, dis + q0 * k, &ar));
+  }
+  return YK_OK;
 }
 
 // ---------------------------------------------------------------------------
