@@ -76,6 +76,12 @@ float *knn_thread(int nq, int nb, int d, int k, const float *b, const float *v, 
 void knn_reorder_shortlist(int n, int nb, int d, int k, const float *b, const float *v,
                            int *idx, float *dis);
 
+/* replaces nn.h:127-130 (nn.c:583-600): the shortlist distances of a
+ * partial base b (labels label0 .. label0 + nb - 1), per query from entry
+ * kp[q] until the first entry past that window; kp advances there */
+void knn_recompute_exact_dists(int nq, int nb, int d, int k, const float *b, const float *v,
+                               int label0, int *kp, const int *idx, float *dis);
+
 /* replaces nn.h:141-143 (nn.c:92-97) */
 void compute_cross_distances(int d, int na, int nb, const float *a, const float *b,
                              float *dist2);
@@ -184,6 +190,9 @@ int yael_kmeans_device(int d, int n, int k, int niter, const float *v_dev, int f
 int yael_kmeans_last_stats(long *iters, long *assigned, long *fallback);
 int yael_knn_reorder_shortlist_checked(int n, int nb, int d, int k, const float *b,
                                        const float *v, int *idx, float *dis);
+int yael_knn_recompute_exact_dists_checked(int nq, int nb, int d, int k, const float *b,
+                                           const float *v, int label0, int *kp, const int *idx,
+                                           float *dis);
 
 /* Batched form: ninst independent k-means instances, instance i being
  * kmeans(d, n_inst[i], k, niter, v_i, flags, seeds[i], 1, ...) on the rows

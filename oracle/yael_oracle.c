@@ -398,6 +398,35 @@ void oy_knn_reorder_shortlist(int n, int nb, int d, int k, const float *b,
   free(sub);
 }
 
+/* nn.c:583-600: distances of the shortlist entries that fall in a window of
+ * the base (labels label0 .. label0+nb-1, rows of b), per query from entry
+ * kp[q] on until the first entry past the window; difference form in double
+ * (vector.c:2348-2359), rounded to float.  kp[q] advances to that entry.
+ * Returns -1 where the reference's assert(j >= 0) would fire (an entry below
+ * the window before the first one past it), 0 otherwise. */
+int oy_knn_recompute_exact_dists(int nq, int nb, int d, int k, const float *b,
+                                 const float *v, int label0, int *kp, const int *idx,
+                                 float *dis) {
+  for (long q = 0; q < nq; q++) {
+    const float *vq = v + (long)d * q;
+    long i;
+    for (i = kp[q]; i < k; i++) {
+      long j = (long)idx[q * k + i] - label0;
+      if (j < 0) return -1;
+      if (j >= nb) break;
+      const float *bj = b + j * d;
+      double s = 0;
+      for (long c = 0; c < d; c++) {
+        double a = (double)vq[c] - bj[c];
+        s += a * a;
+      }
+      dis[q * k + i] = (float)s;
+    }
+    kp[q] = (int)i;
+  }
+  return 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* FP64 brute force (not a reference function: the exact-arithmetic check)  */
 

@@ -207,7 +207,23 @@ def shortlist_cases(ref, rng):
     a[3, 5:] = -1
     d = np.zeros((40, 9), np.float32)
     a2, d2 = ref.knn_reorder_shortlist(b, q, a, d)
-    return {"shortlist": dict(b=b, q=q, assign_in=a, assign=a2, dis=d2)}
+    out = {"shortlist": dict(b=b, q=q, assign_in=a, assign=a2, dis=d2)}
+    # knn_recompute_exact_dists over three windows of the base: sorted
+    # shortlists, each query starting where the previous window stopped
+    bw = rng.standard_normal((600, 33)).astype(np.float32)
+    qw = rng.standard_normal((50, 33)).astype(np.float32)
+    idx = np.sort(np.stack([rng.choice(600, 12, replace=False) for _ in range(50)]), 1)
+    idx = idx.astype(np.int32)
+    kp = np.zeros(50, np.int32)
+    dis = np.full((50, 12), -1.0, np.float32)
+    steps = []
+    for label0, nbw in ((0, 200), (200, 250), (450, 150)):
+        kp, dis = ref.knn_recompute_exact_dists(bw[label0:label0 + nbw], qw, label0, kp, idx, dis)
+        steps.append(np.concatenate([kp[:, None].astype(np.float32), dis], 1))
+    out["recompute"] = dict(b=bw, q=qw, idx=idx, windows=np.array([[0, 200], [200, 250],
+                                                                    [450, 150]], np.int32),
+                            after=np.stack(steps))
+    return out
 
 
 def main():

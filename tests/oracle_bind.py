@@ -85,6 +85,22 @@ class _Api:
               redo, _p(cent, _f32p), _p(dis, _f32p), _p(assign, _i32p), _p(nassign, _i32p))
         return float(r), cent, dis, assign, nassign
 
+    def knn_recompute_exact_dists(self, b, v, label0, kp, idx, dis):
+        """The window re-computation (nn.c:583-600); returns (kp, dis) as
+        updated (copies).  The oracle's twin returns -1 instead of asserting
+        on an entry below the window; that is raised as ValueError."""
+        b, v = _f32(b), _f32(v)
+        idx = np.ascontiguousarray(idx, np.int32)
+        kp = np.ascontiguousarray(kp, np.int32).copy()
+        dis = np.ascontiguousarray(dis, np.float32).copy()
+        nq, k = idx.shape
+        f = self.fn("knn_recompute_exact_dists")
+        r = f(nq, b.shape[0], b.shape[1], k, _p(b, _f32p), _p(v, _f32p), ctypes.c_int(label0),
+              _p(kp, _i32p), _p(idx, _i32p), _p(dis, _f32p))
+        if self.prefix and r != 0:
+            raise ValueError("an entry below the window")
+        return kp, dis
+
     def knn_reorder_shortlist(self, b, v, assign, dis):
         b, v = _f32(b), _f32(v)
         assign = np.ascontiguousarray(assign, np.int32).copy()

@@ -321,3 +321,69 @@ def test_knn_full_precision_form_large_k(ya, k):
     for i in bad:
         for dv in np.unique(d32[i][i16[i] != i32[i]]):
             assert sorted(i16[i][d16[i] == dv]) == sorted(i32[i][d32[i] == dv])
+
+
+def test_reorder_shortlist_golden(ya):
+    """knn_reorder_shortlist (nn.c:528-580) on the reference's golden case:
+    float data, a shortlist cut short by -1 entries."""
+    g = load("shortlist")
+    a, d = ya.knn_reorder_shortlist(g["b"], g["q"], g["assign_in"], np.zeros_like(g["dis"]))
+    assert np.array_equal(a, g["assign"])
+    valid = g["assign"] >= 0
+    assert np.array_equal(d.view(np.int32)[valid], g["dis"].view(np.int32)[valid])
+
+
+def _recompute(ya, b, q, label0, kp, idx, dis, checked=True):
+    L = ya.lib()
+    b = np.ascontiguousarray(b, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    idx = np.ascontiguousarray(idx, np.int32)
+    kp = np.ascontiguousarray(kp, np.int32).copy()
+    dis = np.ascontiguousarray(dis, np.float32).copy()
+    nq, k = idx.shape
+    rc = L.yael_knn_recompute_exact_dists_checked(nq, b.shape[0], b.shape[1], k,
+                                                   b.ctypes.data, q.ctypes.data, label0,
+                                                   kp.ctypes.data, idx.ctypes.data,
+                                                   dis.ctypes.data)
+    return rc, kp, dis
+
+
+def test_recompute_exact_dists_golden(ya):
+    """knn_recompute_exact_dists (nn.c:583-600) over three windows of the
+    base in turn, against the reference's golden values: the advanced kp and
+    every distance, bit for bit."""
+    g = load("recompute")
+    kp = np.zeros(len(g["q"]), np.int32)
+    dis = np.full(g["idx"].shape, -1.0, np.float32)
+    for w, (label0, nbw) in enumerate(g["windows"]):
+        rc, kp, dis = _recompute(ya, g["b"][label0:label0 + nbw], g["q"], int(label0), kp,
+                                 g["idx"], dis)
+        assert rc == 0, ya.lib().yk_last_error()
+        assert np.array_equal(kp, g["after"][w][:, 0].astype(np.int32)), w
+        assert np.array_equal(dis.view(np.int32), g["after"][w][:, 1:].view(np.int32)), w
+
+
+def test_recompute_exact_dists_vs_oracle(ya, oracle):
+    """Long shortlists (k=300, beyond one wave's 64 lanes), d=200, random
+    windows and start positions: the oracle's kp and distance bits; an
+    entry below the window is an error, as the reference asserts."""
+    rng = np.random.default_rng(17)
+    nb_all, d, nq, k = 5000, 200, 70, 300
+    b = rng.standard_normal((nb_all, d)).astype(np.float32)
+    q = rng.standard_normal((nq, d)).astype(np.float32)
+    idx = np.sort(np.stack([rng.choice(nb_all, k, replace=False) for _ in range(nq)]), 1)
+    idx = idx.astype(np.int32)
+    dis = rng.standard_normal((nq, k)).astype(np.float32)
+    label0, nbw = 1200, 2100
+    kp = np.array([np.searchsorted(r, label0) for r in idx], np.int32)
+    kp[::7] = np.minimum(kp[::7] + 5, k)  # some queries start further in
+    rc, gkp, gdis = _recompute(ya, b[label0:label0 + nbw], q, label0, kp, idx, dis)
+    assert rc == 0
+    okp, odis = oracle.knn_recompute_exact_dists(b[label0:label0 + nbw], q, label0, kp, idx, dis)
+    assert np.array_equal(gkp, okp)
+    assert np.array_equal(gdis.view(np.int32), odis.view(np.int32))
+    bad = kp.copy()
+    bad[3] = 0  # query 3 starts below the window
+    assert idx[3, 0] < label0
+    rc, _, _ = _recompute(ya, b[label0:label0 + nbw], q, label0, bad, idx, dis)
+    assert rc != 0 and b"below" in ya.lib().yk_last_error()

@@ -13,6 +13,7 @@
    dropped term, a wrong sign or a transposed operand fails.
 """
 import glob
+import math
 import os
 
 import numpy as np
@@ -148,6 +149,47 @@ def test_shortlist_golden(oracle_blas):
     assert np.array_equal(a, g["assign"])
     valid = g["assign"] >= 0
     assert np.array_equal(bits(d)[valid], bits(g["dis"])[valid])
+
+
+def test_recompute_exact_dists_golden(oracle):
+    """Three windows of the base in turn (nn.c:583-600): after each, the
+    advanced kp and the distances equal the reference's bit for bit."""
+    g = load("recompute")
+    kp = np.zeros(len(g["q"]), np.int32)
+    dis = np.full(g["idx"].shape, -1.0, np.float32)
+    for w, (label0, nbw) in enumerate(g["windows"]):
+        kp, dis = oracle.knn_recompute_exact_dists(g["b"][label0:label0 + nbw], g["q"], int(label0),
+                                                   kp, g["idx"], dis)
+        assert np.array_equal(kp, g["after"][w][:, 0].astype(np.int32)), w
+        assert np.array_equal(bits(dis), bits(g["after"][w][:, 1:])), w
+
+
+def test_recompute_exact_dists_closed_form(oracle):
+    """The recomputed values are the FP64 difference-form distances rounded
+    to float; entries outside the window keep their value; an entry below
+    the window (the reference's assert) is reported."""
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal((100, 7)).astype(np.float32)
+    q = rng.standard_normal((5, 7)).astype(np.float32)
+    idx = np.sort(np.stack([rng.choice(100, 6, replace=False) for _ in range(5)]), 1)
+    idx = idx.astype(np.int32)
+    dis = np.full((5, 6), 7.0, np.float32)
+    # kp at 0: an entry below the window comes first (the reference asserts)
+    assert (idx[:, 0] < 40).any()
+    with pytest.raises(ValueError):
+        oracle.knn_recompute_exact_dists(b[40:70], q, 40, np.zeros(5, np.int32), idx, dis)
+    start = np.array([np.searchsorted(r, 40) for r in idx], np.int32)
+    kp, out = oracle.knn_recompute_exact_dists(b[40:70], q, 40, start, idx, dis)
+    for i in range(5):
+        stop = np.searchsorted(idx[i], 70)
+        assert kp[i] == stop
+        for j in range(6):
+            if start[i] <= j < stop:
+                x = q[i].astype(np.float64) - b[idx[i, j]].astype(np.float64)
+                exact = math.fsum((x * x).tolist())  # correctly rounded sum
+                assert abs(float(out[i, j]) - exact) <= 6e-8 * exact
+            else:
+                assert out[i, j] == 7.0
 
 
 def test_reference_build_matches_fixtures():

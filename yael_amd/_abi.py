@@ -59,6 +59,9 @@ PROTOS = {
     "yk_gather_rows": (I, [vp, vp, L, I, vp]),
     "yk_rerank_shortlist": (I, [L, I, I, vp, vp, vp, vp]),
     "knn_reorder_shortlist": (None, [I, I, I, I, vp, vp, vp, vp]),
+    "knn_recompute_exact_dists": (None, [I, I, I, I, vp, vp, I, vp, vp, vp]),
+    "yael_knn_recompute_exact_dists_checked": (I, [I, I, I, I, vp, vp, I, vp, vp, vp]),
+    "yk_recompute_exact_dists": (I, [L, L, I, I, vp, vp, I, vp, vp, vp, vp]),
     # device-level C-ABI (yk.h)
     "yk_last_error": (C.c_char_p, []),
     "yk_set_error_c": (I, [I, C.c_char_p]),

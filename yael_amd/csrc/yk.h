@@ -122,6 +122,16 @@ int yk_pair_dist_f64(int type, long nq, int d, int k, const float *q,
 int yk_rerank_shortlist(long n, int d, int k, const float *b, const float *v, int *idx,
                         float *dis);
 
+/* Window re-computation (nn.c:583-600): for each query q, entries kp[q] ..
+ * of its sorted shortlist idx (nq x k labels) that lie in [label0, label0 +
+ * nb) -- b holds those nb rows, padded -- get their difference-form distance
+ * (double accumulation, rounded to float) in dis; kp[q] advances to the
+ * first entry past the window.  *bad (device int, set to INT_MAX by the
+ * caller) receives the smallest query whose next entry lies below the
+ * window (the reference asserts there). */
+int yk_recompute_exact_dists(long nq, long nb, int d, int k, const float *b, const float *v,
+                             int label0, int *kp, const int *idx, float *dis, int *bad);
+
 /* k-means device steps (kmeans.c:242-310).
  * update: counts[k], centroids[k x ld] = (float)(FP64 sum / count) in point
  * order of each cluster; FP64 objective of dis.  ws: workspace from

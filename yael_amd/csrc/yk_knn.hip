@@ -310,3 +310,76 @@ extern "C" int yk_rerank_shortlist(long n, int d, int k, const float *b, const f
   YK_HIP(hipGetLastError());
   return YK_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Distances of the shortlist entries inside a window of the base
+// (knn_recompute_exact_dists, nn.c:583-600): for query q, from entry kp[q]
+// on, every entry whose label lies in [label0, label0 + nb) up to the first
+// one past the window gets the difference-form distance, accumulated in
+// double in component order and rounded to float (vector.c:2348-2359); kp[q]
+// advances to that first entry.  An entry below the window met before it is
+// the reference's assert: the smallest such query is recorded in *bad (the
+// host reports it).  One wave per query; a lane per entry, its row read 16
+// bytes at a time (the zero pad columns add exact zeros).
+
+__global__ __launch_bounds__(256) void yk_recompute_kernel(const float *__restrict__ b,
+                                                          const float *__restrict__ v, long nq,
+                                                          long nb, int ld, int k, int label0,
+                                                          int *__restrict__ kp,
+                                                          const int *__restrict__ idx,
+                                                          float *__restrict__ dis, int *bad) {
+  const int lane = threadIdx.x & 63;
+  const long q = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= nq) return;
+  const int k0 = kp[q];
+  const int *row = idx + q * k;
+  // first entry from k0 on that leaves the window (below or past it)
+  int stop = k;
+  bool below = false;
+  for (int i0 = k0; i0 < k; i0 += 64) {
+    const int i = i0 + lane;
+    const long j = i < k ? (long)row[i] - label0 : 0;
+    const unsigned long long m = __ballot(i < k && (j < 0 || j >= nb));
+    if (m) {
+      const int f = __ffsll((long long)m) - 1;
+      stop = i0 + f;
+      below = __shfl(j, f) < 0;
+      break;
+    }
+  }
+  if (below) {
+    if (lane == 0) atomicMin(bad, (int)q);
+    return;
+  }
+  const float *vq = v + q * ld;
+  for (int i = k0 + lane; i < stop; i += 64) {
+    const float *bj = b + ((long)row[i] - label0) * ld;
+    double s = 0.0;
+    for (int c = 0; c < ld; c += 4) {
+      const float4 x = *(const float4 *)(vq + c), y = *(const float4 *)(bj + c);
+      double a = (double)x.x - (double)y.x;
+      s = __dadd_rn(s, __dmul_rn(a, a));
+      a = (double)x.y - (double)y.y;
+      s = __dadd_rn(s, __dmul_rn(a, a));
+      a = (double)x.z - (double)y.z;
+      s = __dadd_rn(s, __dmul_rn(a, a));
+      a = (double)x.w - (double)y.w;
+      s = __dadd_rn(s, __dmul_rn(a, a));
+    }
+    dis[q * k + i] = (float)s;
+  }
+  if (lane == 0 && stop > k0) kp[q] = stop;
+}
+
+extern "C" int yk_recompute_exact_dists(long nq, long nb, int d, int k, const float *b,
+                                        const float *v, int label0, int *kp, const int *idx,
+                                        float *dis, int *bad) {
+  YK_TRY(yk_init());
+  if (nq <= 0 || k <= 0) return YK_OK;
+  int tok = yk_prof_begin(YK_PROF_RERANK);
+  hipLaunchKernelGGL(yk_recompute_kernel, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0,
+                     yk_stream(), b, v, nq, nb, yk_ld(d), k, label0, kp, idx, dis, bad);
+  yk_prof_end(tok);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
