@@ -105,6 +105,26 @@ def test_full_runs_vs_reference_golden(ya, case):
     assert agree > 0.98, agree
 
 
+@pytest.mark.parametrize("d", [1000, 1100, 2053])
+def test_normalized_step_long_vectors(ya, oracle, d):
+    """KMEANS_NORMALIZE_CENTS on rows longer than the update kernel's
+    1024-column LDS row (a separate normalisation pass): one user-initialised
+    step on integer data in [0, 63] (every distance below 2^24, so exact in
+    FP32) gives the oracle's bits, normalised centroids included."""
+    rng = np.random.default_rng(d)
+    cen = rng.uniform(0, 63, (30, d))
+    v = np.clip(np.rint(cen[rng.integers(0, 30, 3000)] + 8 * rng.standard_normal((3000, d))),
+                0, 63).astype(np.float32)
+    init = v[rng.choice(len(v), 40, replace=False)].copy()
+    print("normalized step d=%d" % d, flush=True)
+    g = gpu_kmeans(ya, v, 40, 1, init=init, normalize=True)
+    o = oracle.kmeans(v, 40, 1, flags=QUIET | INIT_USER | NORMALIZE, seed=1, init=init)
+    assert np.array_equal(g[3], o[3]) and np.array_equal(g[4], o[4])
+    assert np.array_equal(g[2].view(np.int32), o[2].view(np.int32))
+    assert np.array_equal(g[1].view(np.int32), o[1].view(np.int32))
+    np.testing.assert_allclose(np.linalg.norm(g[1].astype(np.float64), axis=1), 1.0, rtol=1e-6)
+
+
 def test_split_step_bitexact_golden(ya):
     """Ten empty clusters split in one step on integer data: the assignment,
     the update and the host-side split (rand_r / gaussrand_r streams) give the

@@ -189,6 +189,28 @@ __global__ __launch_bounds__(256) void yk_segsum_kernel(const float *__restrict_
   }
 }
 
+// L2 normalisation of k rows of d > 1024 floats (the segmented-sum kernel
+// keeps at most 1024 columns of a row in LDS): one wave per row, 64 columns
+// per coalesced load, the squares then added by lane 0 in column order
+// through lane reads, so the double sum rounds as fvec_norm's sequential
+// loop does (vector.c:2180-2197); c *= 1/nr as fvec_mul_by (vector.c:1792).
+__global__ __launch_bounds__(256) void yk_normalize_rows_kernel(float *__restrict__ cent, int d,
+                                                                int ld, int k) {
+  const int lane = threadIdx.x & 63;
+  const long c = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= k) return;
+  float *row = cent + c * ld;
+  double nr = 0.0;
+  for (int j0 = 0; j0 < d; j0 += 64) {
+    const float x = j0 + lane < d ? row[j0 + lane] : 0.0f;
+    const float sq = __fmul_rn(x, x);
+    const int m = d - j0 < 64 ? d - j0 : 64;
+    for (int t = 0; t < m; t++) nr = __dadd_rn(nr, (double)__shfl(sq, t));
+  }
+  const double sc = 1.0 / sqrt(nr);
+  for (int j = lane; j < d; j += 64) row[j] = (float)((double)row[j] * sc);
+}
+
 // FP64 sum of n floats, fixed reduction shape.
 __global__ __launch_bounds__(256) void yk_sum_stage1(const float *__restrict__ x, long n,
                                                      double *__restrict__ part) {
@@ -272,6 +294,9 @@ extern "C" int yk_kmeans_update(long n, int k, int d, const float *v, const int 
                                    (size_t)n, 0, yk_bits_for(k), st));
   hipLaunchKernelGGL(yk_segsum_kernel, dim3((k + 3) / 4), dim3(256), 0, st, v, d, ld, k,
                      w.offsets, w.perm, cent, normalize);
+  if (normalize && d > 1024)
+    hipLaunchKernelGGL(yk_normalize_rows_kernel, dim3((k + 3) / 4), dim3(256), 0, st, cent, d,
+                       ld, k);
   yk_prof_end(tok);
   YK_HIP(hipGetLastError());
   int err = 0;
