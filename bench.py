@@ -33,6 +33,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from yael_amd.workloads import mixture, zipf_sizes  # noqa: E402
+
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 (matrix)
 PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16
 PEAK_HBM_GBS = 8000.0
@@ -40,23 +42,6 @@ PEAK_HBM_GBS = 8000.0
 (PROF_FUSED, PROF_NORM, PROF_UPDATE, PROF_MERGE, PROF_RERANK, PROF_BATCH, PROF_BF16,
  PROF_BF16_SKETCH, PROF_F32_SKETCH) = range(9)
 NSLOTS = 9
-
-
-def mixture(seed, n, d, ncomp, sigma, integer=True, centres_seed=None):
-    """Gaussian-mixture vectors in [0,255]^d (float32)."""
-    crng = np.random.default_rng(centres_seed if centres_seed is not None else seed)
-    cen = crng.uniform(0, 255, (ncomp, d)).astype(np.float32)
-    rng = np.random.default_rng(seed)
-    out = np.empty((n, d), np.float32)
-    step = 1 << 18
-    for i in range(0, n, step):
-        m = min(step, n - i)
-        x = cen[rng.integers(0, ncomp, m)] + np.float32(sigma) * rng.standard_normal((m, d), np.float32)
-        np.clip(x, 0, 255, out=x)
-        if integer:
-            np.rint(x, out=x)
-        out[i:i + m] = x
-    return out
 
 
 def traffic_from_profile(name):
@@ -142,26 +127,45 @@ def bench_knn(ya, dev, args, base, queries, k, kp):
                 sorted_ok=bool(np.all(np.diff(d16, axis=1) >= 0)))
 
 
-def bench_kmeans(ya, dev, v, k, niter, seed):
+def bench_kmeans(ya, dev, v, k, niter, env=None):
+    """k-means on device-resident points, seed 0 (kmeans() then draws its
+    seed from lrand48; srand48(0) before every run makes that draw the same
+    each time): iterations counted by the library, median of 3 runs; with
+    env, the reference's kmeans() on the same points and seed."""
     lib = ya.lib()
+    libc = ctypes.CDLL(None)
     V = dev.DeviceSet(v)
     n, d = v.shape
     cent = np.empty((k, d), np.float32)
+    asg = np.empty(n, np.int32)
     ret = ctypes.c_float()
     flags = 0x10000  # quiet
 
     def run(it):
-        ya.check(lib.yael_kmeans_device(d, n, k, it, V.vec.ptr, flags, seed, 1,
-                                        cent.ctypes.data_as(ctypes.c_void_p), None, None, None,
+        libc.srand48(ctypes.c_long(0))
+        ya.check(lib.yael_kmeans_device(d, n, k, it, V.vec.ptr, flags, 0, 1,
+                                        cent.ctypes.data_as(ctypes.c_void_p), None,
+                                        asg.ctypes.data_as(ctypes.c_void_p), None,
                                         ctypes.byref(ret)))
+        lib.yk_sync()
+        st = [ctypes.c_long() for _ in range(3)]
+        lib.yael_kmeans_last_stats(*[ctypes.byref(x) for x in st])
+        return st[0].value
     run(2)
-    lib.yk_sync()
-    t0 = time.perf_counter()
-    run(niter)
-    lib.yk_sync()
-    t1 = time.perf_counter()
+    recs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        iters = run(niter)
+        recs.append((time.perf_counter() - t0, iters))
     V.free()
-    return dict(sec=t1 - t0, iters=niter, obj=float(ret.value))
+    sec, iters = sorted(recs)[1]
+    rec = dict(sec=sec, iters=iters, obj=float(ret.value))
+    if env:
+        cb = cpu_baseline_kmeans(v, k, niter, env)
+        rec["cpu"] = {key: val for key, val in cb.items() if key != "assign"}
+        rec["cpu"]["assignment_agreement_with_gpu"] = float(np.mean(cb["assign"] == asg))
+        rec["cpu"]["objective_rel_diff_gpu_vs_cpu"] = float(ret.value) / cb["objective"] - 1.0
+    return rec
 
 
 def bench_kmeanspp(ya, dev, v, k, seed, cpu):
@@ -193,7 +197,7 @@ def bench_kmeanspp(ya, dev, v, k, seed, cpu):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_bind import Reference, have_ref
         if have_ref():
-            cores = min(len(os.sched_getaffinity(0)), 16)
+            cores = cpu_env()["threads"]
             ref = Reference()
             t0 = time.perf_counter()
             ref.kmeans(v, k, 1, flags=0x10000 | cores, seed=seed)
@@ -232,7 +236,7 @@ def bench_hkm(ya, base, cpu, n=200_000, nlevel=2, bf=32, niter=10, nq_cpu=20_000
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_bind import Reference, have_ref, make_hkm
         if have_ref():
-            cores = min(len(os.sched_getaffinity(0)), 16)
+            cores = cpu_env()["threads"]
             os.environ["YAEL_COUNT_CPU"] = str(cores)
             ref = Reference()
             libc.srand48(ctypes.c_long(1))
@@ -286,7 +290,7 @@ def bench_dropin(ya, base, queries, k):
 
 
 def bench_kmeans_large(ya, dev, n=10_000_000, d=128, k=65536, niter=20, seed=1, nsub=8,
-                       ksub=256):
+                       ksub=256, env=None, cpu_points=8192):
     """n=1e7, d=128, k=65,536 k-means with the BF16 assign form (BF16 first
     pass, FP32 re-rank of each point's 8 best centroids, certificate, FP32
     search of the uncertified points), FP64 sums and objective; data from a
@@ -368,6 +372,18 @@ def bench_kmeans_large(ya, dev, n=10_000_000, d=128, k=65536, niter=20, seed=1, 
           "objective rel err vs fp64 %.2e" % (s16, s32, s32 / s16, out["assign"]["assign_identical"],
                                               out["assign"]["objective_rel_err_vs_fp64"]),
           file=sys.stderr, flush=True)
+    if env:
+        # the assign step of the reference on a sample of the points against
+        # the same final centroids; its per-iteration time at n follows
+        vs = np.empty((cpu_points, d), np.float32)
+        ya.check(lib.yk_download_vectors(vs.ctypes.data_as(ctypes.c_void_p), V.vec.ptr,
+                                         cpu_points, d))
+        cb = cpu_baseline_assign(vs, cent, env, "the first %d points" % cpu_points)
+        cb["index_agreement_with_gpu"] = float(np.mean(cb.pop("idx")[:, 0] == i16[:cpu_points]))
+        cb.pop("dis")
+        cb["assign_step_s_at_full_n"] = n / cb["value"]
+        cb["gpu_assign_step_s"] = s16
+        out["assign"]["cpu_baseline"] = cb
     for b in (oi16, od16, oi32, od32, o64):
         b.free()
     C.free()
@@ -398,9 +414,12 @@ def bench_kmeans_large(ya, dev, n=10_000_000, d=128, k=65536, niter=20, seed=1, 
     return out
 
 
-def bench_precision_d960(ya, dev, nb=1_000_000, nq=10_000, d=960, k=10, kp=64, steps=3):
+def bench_precision_d960(ya, dev, nb=1_000_000, nq=10_000, d=960, k=10, kp=64, steps=3,
+                         env=None):
     """Precision form at d=960: |N(0,1)| vectors scaled to unit norm,
-    generated in HBM; k=10 from k'=64 BF16 candidates."""
+    generated in HBM; k=10 from k'=64 BF16 candidates.  With env, the
+    reference's knn_full on a sample of the same queries against the whole
+    base (copied to the host)."""
     lib = ya.lib()
     B = dev.DeviceSet.synth(nb, d, 2, 51, unit_rows=True)
     Q = dev.DeviceSet.synth(nq, d, 2, 52, unit_rows=True)
@@ -430,6 +449,17 @@ def bench_precision_d960(ya, dev, nb=1_000_000, nq=10_000, d=960, k=10, kp=64, s
     d64 = o64.to_host(np.float64, (nq, k))
     rel = np.abs(d16.astype(np.float64) - d64) / np.maximum(np.abs(d64), 1e-30)
     flops = 2.0 * nq * nb * d
+    cpu = None
+    if env:
+        bh_ = np.empty((nb, d), np.float32)
+        ya.check(lib.yk_download_vectors(bh_.ctypes.data_as(ctypes.c_void_p), B.vec.ptr, nb, d))
+        qh_ = np.empty((nq, d), np.float32)
+        ya.check(lib.yk_download_vectors(qh_.ctypes.data_as(ctypes.c_void_p), Q.vec.ptr, nq, d))
+        cb = cpu_baseline_knn(bh_, qh_, k, env)
+        cpu = {key: val for key, val in cb.items() if key not in ("idx", "dis", "nq")}
+        cpu["index_agreement_with_gpu"] = float(np.mean(np.all(cb["idx"] == i16[:cb["nq"]],
+                                                               axis=1)))
+        del bh_
     for b in (oi, od, oi16, od16, o64):
         b.free()
     B.free()
@@ -444,32 +474,18 @@ def bench_precision_d960(ya, dev, nb=1_000_000, nq=10_000, d=960, k=10, kp=64, s
          "distances_bit_identical_to_fp32": bool(np.array_equal(d16.view(np.int32),
                                                                 d32.view(np.int32))),
          "max_rel_err_vs_fp64": float(np.nanmax(rel)), "fallback_queries": nfb.value}
+    if cpu:
+        r["cpu_baseline"] = cpu
     print("# d=960: bf16+rerank %.1f q/s, fp32 %.1f q/s (x%.2f), agreement %.4f, fallback %d"
           % (nq / s16, nq / s32, s32 / s16, r["index_agreement_with_fp32"], nfb.value),
           file=sys.stderr, flush=True)
     return r
 
 
-def zipf_sizes(ninst, total, lo, hi, s=1.1):
-    """Instance sizes proportional to rank^-s, water-filled into [lo, hi] so
-    that they sum to total."""
-    w = 1.0 / np.arange(1, ninst + 1) ** s
-    a, b = 0.0, float(total) / w.min()
-    for _ in range(200):
-        lam = 0.5 * (a + b)
-        if np.clip(lam * w, lo, hi).sum() > total:
-            b = lam
-        else:
-            a = lam
-    n = np.floor(np.clip(a * w, lo, hi)).astype(np.int64)
-    free = np.nonzero(n < hi)[0]
-    n[free[0]] += total - n.sum()
-    return n.astype(np.int32)
-
-
-def bench_batched(ya, dev, ninst=256, total=2_000_000, d=128, k=64, niter=15):
+def bench_batched(ya, dev, ninst=256, total=2_000_000, d=128, k=64, niter=15, env=None):
     """hkm level-2 pattern: 256 independent k-means instances of Zipf sizes,
-    one batched run (grouped launches, graph-replayed iteration)."""
+    one batched run (grouped launches, graph-replayed iteration); with env,
+    the reference's loop of kmeans() calls over the same instances."""
     lib = ya.lib()
     sizes = zipf_sizes(ninst, total, 256, 120_000)
     v = mixture(4, int(sizes.sum()), d, 1024, 20, centres_seed=0)
@@ -508,33 +524,149 @@ def bench_batched(ya, dev, ninst=256, total=2_000_000, d=128, k=64, niter=15):
     lib.yk_sync()
     t3 = time.perf_counter()
     V.free()
-    return dict(sizes=sizes, call_s=t1 - t0, loop_ms=loop_ms, seq_s=t3 - t2, chunk=chunk.value,
-                iters_max=int(iters.max()), iters_min=int(iters.min()))
+    out = dict(sizes=sizes, call_s=t1 - t0, loop_ms=loop_ms, seq_s=t3 - t2, chunk=chunk.value,
+               iters_max=int(iters.max()), iters_min=int(iters.min()))
+    if env:
+        cb = cpu_baseline_batched(v, sizes, k, niter, seeds, env)
+        rel = np.asarray(ret, np.float64) / np.asarray(cb.pop("objectives")) - 1.0
+        cb["objective_rel_diff_gpu_vs_cpu"] = {"mean": float(rel.mean()),
+                                               "max_abs": float(np.abs(rel).max())}
+        out["cpu"] = cb
+    return out
 
 
-def cpu_baseline_knn(base, queries, k, budget_s=12.0):
-    """yael's own knn_full_thread (compiled reference, OpenBLAS + OpenMP) on a
-    bounded sample of the queries against the full base."""
-    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+# ---------------------------------------------------------------------------
+# CPU baselines: yael's own C path (oracle/_ref, the reference's sources
+# compiled here against OpenBLAS, OpenMP threads), or the oracle's
+# restatement of it where the compiled reference is absent.
+
+
+def cpu_env():
+    """Facts of the host the CPU baselines run on.  threads = the OpenMP
+    thread count the run uses: OMP_NUM_THREADS when set (a shared GPU host
+    gives each GPU's jobs that share of its cores), else every CPU this
+    process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    logical = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(logical, int(omp)) if omp.isdigit() and int(omp) > 0 else logical
+    blas = "unknown"
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import openblas_path
+    path = openblas_path()
+    if path:
+        try:
+            ob = ctypes.CDLL(path)
+            ob.scipy_openblas_get_config.restype = ctypes.c_char_p
+            blas = "%s (%s)" % (ob.scipy_openblas_get_config().decode(), os.path.basename(path))
+        except (OSError, AttributeError):
+            blas = os.path.basename(path)
+    return dict(cpu_model=model, logical_cpus_visible=logical, threads=threads,
+                omp_num_threads=omp or None, blas=blas, blas_threads_inside_yael=1)
+
+
+def cpu_api():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_bind import Reference, Oracle, have_ref
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"  # BLAS single-threaded inside yael's threads
     if have_ref():
-        api, kind = Reference(), "reference"
-    else:
-        api, kind = Oracle(), "port"
-    nq_cal = cores * 32
+        return Reference(), "reference"
+    o = Oracle()
+    o.set_blas(0, 0)
+    return o, "port"
+
+
+def median_runs(fn, reps=3, budget_s=40.0):
+    """Median wall time of up to reps runs of fn (fewer when one run alone
+    exceeds budget_s / reps); returns (median seconds, runs, last result)."""
+    times, out = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+        if sum(times) > budget_s or times[0] > budget_s / reps:
+            break
+    return float(np.median(times)), len(times), out
+
+
+def cpu_baseline_knn(base, queries, k, env, budget_s=4.0):
+    """knn_full_thread on a bounded sample of the queries against the full
+    base (about budget_s per run), median of 3 runs."""
+    api, kind = cpu_api()
+    nt = env["threads"]
+    nq_cal = nt * 8
     t0 = time.perf_counter()
-    api.knn_full(2, base, queries[:nq_cal], k, nt=cores)
+    api.knn_full(2, base, queries[:nq_cal], k, nt=nt)
     cal = time.perf_counter() - t0
     nq_s = int(min(len(queries), max(nq_cal, nq_cal * budget_s / max(cal, 1e-3))))
-    t0 = time.perf_counter()
-    idx, dis = api.knn_full(2, base, queries[:nq_s], k, nt=cores)
-    sec = time.perf_counter() - t0
-    return dict(value=nq_s / sec, unit="queries/s", cores=cores, kind=kind,
-                sample="%d of the %d queries against all %d base vectors, k=%d, %d OpenMP threads"
-                       % (nq_s, len(queries), len(base), k, cores),
+    sec, runs, (idx, dis) = median_runs(lambda: api.knn_full(2, base, queries[:nq_s], k, nt=nt))
+    return dict(value=nq_s / sec, unit="queries/s", cores=nt, kind=kind,
+                sample="%d of the %d queries against all %d base vectors, d=%d, k=%d; %d "
+                       "OpenMP threads; median of %d runs"
+                       % (nq_s, len(queries), len(base), base.shape[1], k, nt, runs),
                 idx=idx, dis=dis, nq=nq_s)
+
+
+def cpu_baseline_kmeans(v, k, niter, env):
+    """kmeans() of the reference on the same points, seed 0 (its seed drawn
+    from lrand48 after srand48(0), as the GPU run draws it), median of 3."""
+    api, kind = cpu_api()
+    libc = ctypes.CDLL(None)
+    nt = env["threads"]
+
+    def run():
+        libc.srand48(ctypes.c_long(0))
+        return api.kmeans(v, k, niter, flags=0x10000 | nt, seed=0)
+    sec, runs, (obj, cent, dis, asg, nas) = median_runs(run)
+    return dict(value=niter / sec, unit="iterations/s", cores=nt, kind=kind, objective=obj,
+                sample="the whole workload: n=%d d=%d k=%d niter=%d (a fixed-iteration run: "
+                       "the reference stops only on a repeated objective), %d OpenMP threads; "
+                       "median of %d runs" % (len(v), v.shape[1], k, niter, nt, runs),
+                assign=asg)
+
+
+def cpu_baseline_batched(v, sizes, k, niter, seeds, env):
+    """The reference's hkm loop: one kmeans() call per instance (hkm.c:66-103),
+    each with its own seed; grouped iterations/s = niter / the loop's time."""
+    api, kind = cpu_api()
+    nt = env["threads"]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+
+    def run():
+        objs = []
+        for i in range(len(sizes)):
+            objs.append(api.kmeans(v[off[i]:off[i + 1]], k, niter, flags=0x10000 | nt,
+                                   seed=int(seeds[i]))[0])
+        return objs
+    sec, runs, objs = median_runs(run)
+    return dict(value=niter / sec, unit="grouped iterations/s", cores=nt, kind=kind,
+                seconds=sec, objectives=objs,
+                sample="all %d instances (sum n=%d, d=%d, k=%d, niter=%d), one kmeans() call "
+                       "each, %d OpenMP threads; median of %d runs"
+                       % (len(sizes), int(sizes.sum()), v.shape[1], k, niter, nt, runs))
+
+
+def cpu_baseline_assign(points, cent, env, what):
+    """One k-means assign step (nn(): the nearest centroid of each point,
+    knn_full_thread with k=1) of a sample of the points against all the
+    centroids; the unit is points/s, from which the per-iteration time of
+    the whole set follows."""
+    api, kind = cpu_api()
+    nt = env["threads"]
+    sec, runs, (idx, dis) = median_runs(lambda: api.knn_full(2, cent, points, 1, nt=nt))
+    return dict(value=len(points) / sec, unit="points/s (assign step)", cores=nt, kind=kind,
+                sample="%s: %d points against all %d centroids (d=%d), k=1; %d OpenMP "
+                       "threads; median of %d runs" % (what, len(points), len(cent),
+                                                       cent.shape[1], nt, runs),
+                idx=idx, dis=dis)
 
 
 def main():
@@ -564,6 +696,7 @@ def main():
 
     import yael_amd as ya
     from yael_amd import device as dev
+    env = None if args.no_cpu else cpu_env()
     ya.init()
     print("# device:", ya.device_info(), file=sys.stderr, flush=True)
 
@@ -656,12 +789,18 @@ def main():
 
     if not args.no_extras:
         v = mixture(3, 100_000, 128, 1024, 20, centres_seed=0)
-        km = bench_kmeans(ya, dev, v, 1024, 10, 1)
+        km = bench_kmeans(ya, dev, v, 1024, 10, env)
         out["extras"] = {"kmeans_iter_per_s": km["iters"] / km["sec"],
-                         "kmeans_config": "n=100000 d=128 k=1024 niter=10, FP32 assign, "
-                                          "device-resident",
-                         "kmeans_obj": km["obj"]}
-        print("# kmeans: %.2f iter/s" % (km["iters"] / km["sec"]), file=sys.stderr, flush=True)
+                         "kmeans_config": "n=100000 d=128 k=1024 niter=10 seed 0, FP32 assign, "
+                                          "device-resident; iterations counted by the library, "
+                                          "median of 3 runs",
+                         "kmeans_iterations": km["iters"], "kmeans_obj": km["obj"]}
+        if "cpu" in km:
+            out["extras"]["kmeans_cpu_baseline"] = km["cpu"]
+        print("# kmeans: %.2f iter/s (%d iterations)%s" % (
+            km["iters"] / km["sec"], km["iters"],
+            ", cpu reference %.2f iter/s" % km["cpu"]["value"] if "cpu" in km else ""),
+            file=sys.stderr, flush=True)
         kpp = bench_kmeanspp(ya, dev, v, 1024, 1, cpu=not args.no_cpu)
         out["extras"]["kmeanspp"] = dict(
             config="k-means++ seeding (KMEANS_INIT_BERKELEY), n=100000 d=128 k=1024: draws "
@@ -682,7 +821,7 @@ def main():
             (" (cpu reference: learn %.2f s, quantize %.3g points/s)"
              % (hk["cpu_reference_learn_s"], hk["cpu_reference_quantize_points_per_s"]))
             if "cpu_reference_learn_s" in hk else ""), file=sys.stderr, flush=True)
-        bt = bench_batched(ya, dev)
+        bt = bench_batched(ya, dev, env=env)
         out["extras"]["batched"] = {
             "config": "256 k-means instances (hkm level 2), Zipf(1.1) sizes sum 2e6 in [256, "
                       "120000], d=128, k=64, niter=15, FP32, one batched run, graph-replayed "
@@ -693,14 +832,16 @@ def main():
             "speedup_vs_sequential_calls": bt["seq_s"] / bt["call_s"],
             "iterations_run": [bt["iters_min"], bt["iters_max"]],
         }
+        if "cpu" in bt:
+            out["extras"]["batched"]["cpu_baseline"] = bt["cpu"]
         print("# batched: %.1f grouped iter/s (loop %.2f ms, call %.3f s, sequential %.3f s)"
               % (15.0 / (bt["loop_ms"] * 1e-3), bt["loop_ms"], bt["call_s"], bt["seq_s"]),
               file=sys.stderr, flush=True)
 
     if not args.no_large:
         out.setdefault("extras", {})
-        out["extras"]["precision_d960"] = bench_precision_d960(ya, dev)
-        kl = bench_kmeans_large(ya, dev)
+        out["extras"]["precision_d960"] = bench_precision_d960(ya, dev, env=env)
+        kl = bench_kmeans_large(ya, dev, env=env)
         out["extras"]["kmeans_large"] = {
             "config": "n=1e7 d=128 k=65536 niter=20 seed 1, 65,536-component mixture sigma 15 in "
                       "[0,255] generated in HBM; BF16 first pass + FP32 re-rank of the 8 best "
@@ -715,7 +856,7 @@ def main():
         }
 
     if not args.no_cpu:
-        cb = cpu_baseline_knn(base, queries, k)
+        cb = cpu_baseline_knn(base, queries, k, env)
         g_idx = r["i16"][:cb["nq"]]
         agree = float(np.mean(np.sort(g_idx, 1) == np.sort(cb["idx"], 1)))
         # integer-valued data: every distance is exact in both, so the
@@ -727,7 +868,7 @@ def main():
         out["cpu_baseline"] = {"value": cb["value"], "unit": cb["unit"], "cores": cb["cores"],
                                "kind": cb["kind"], "sample": cb["sample"],
                                "index_agreement_with_gpu": agree,
-                               "distance_lists_identical_to_gpu": dagree}
+                               "distance_lists_identical_to_gpu": dagree, "host": env}
         print("# cpu baseline: %.1f queries/s on %d cores" % (cb["value"], cb["cores"]),
               file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)

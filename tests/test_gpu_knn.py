@@ -325,12 +325,24 @@ def test_knn_full_precision_form_large_k(ya, k):
 
 def test_reorder_shortlist_golden(ya):
     """knn_reorder_shortlist (nn.c:528-580) on the reference's golden case:
-    float data, a shortlist cut short by -1 entries."""
+    float data, a shortlist cut short by -1 entries.  The reference forms
+    each dot product with its BLAS's sgemv, whose summation order is that
+    library's own; the GPU sums in component order.  So the distances agree
+    within the FP32 rounding bound of the expanded form, and the order
+    equals the reference's except between values within that bound."""
     g = load("shortlist")
     a, d = ya.knn_reorder_shortlist(g["b"], g["q"], g["assign_in"], np.zeros_like(g["dis"]))
-    assert np.array_equal(a, g["assign"])
     valid = g["assign"] >= 0
-    assert np.array_equal(d.view(np.int32)[valid], g["dis"].view(np.int32)[valid])
+    assert np.array_equal(a < 0, ~valid)
+    tol = parity.tolerance(2, g["q"], g["b"])
+    err = np.abs(d.astype(np.float64) - g["dis"].astype(np.float64))
+    assert np.all(err[valid] <= np.broadcast_to(tol[:, None], err.shape)[valid])
+    for i in np.nonzero((a != g["assign"]).any(1))[0]:
+        # a different order only among values within the bound of each other
+        j = np.nonzero(a[i] != g["assign"][i])[0]
+        assert np.ptp(d[i, j].astype(np.float64)) <= 2 * tol[i], i
+    print("shortlist: max |gpu - reference| %.3g, order differs in %d of %d lists"
+          % (err[valid].max(), int((a != g["assign"]).any(1).sum()), len(a)), flush=True)
 
 
 def _recompute(ya, b, q, label0, kp, idx, dis, checked=True):
