@@ -415,7 +415,7 @@ def bench_kmeans_large(ya, dev, n=10_000_000, d=128, k=65536, niter=20, seed=1, 
 
 
 def bench_precision_d960(ya, dev, nb=1_000_000, nq=10_000, d=960, k=10, kp=64, steps=3,
-                         env=None):
+                         env=None, fp32=True):
     """Precision form at d=960: |N(0,1)| vectors scaled to unit norm,
     generated in HBM; k=10 from k'=64 BF16 candidates.  With env, the
     reference's knn_full on a sample of the same queries against the whole
@@ -440,8 +440,17 @@ def bench_precision_d960(ya, dev, nb=1_000_000, nq=10_000, d=960, k=10, kp=64, s
         ya.check(lib.yk_knn_bf16(2, nq, nb, d, k, kp, Q.vec.ptr, Q.dnorm.ptr, qh.ptr, qst.ptr,
                                  B.vec.ptr, B.fnorm.ptr, bh.ptr, bmx.ptr, oi16.ptr, od16.ptr,
                                  ctypes.byref(nfb)))
-    s32, sl32 = timed(lib, f32, 2, 1)
     s16, sl16 = timed(lib, b16, steps, 1)
+    if not fp32:  # profile runs: the precision form alone
+        for b in (oi, od, oi16, od16):
+            b.free()
+        B.free()
+        Q.free()
+        return {"bf16_rerank_queries_per_s": nq / s16,
+                "bf16_scan_ms_per_step": sl16[PROF_BF16][0] / steps,
+                "bf16_sketch_ms_per_step": sl16[PROF_BF16_SKETCH][0] / steps,
+                "rerank_ms_per_step": sl16[PROF_RERANK][0] / steps, "fallback_queries": nfb.value}
+    s32, sl32 = timed(lib, f32, 2, 1)
     i32, d32 = oi.to_host(np.int32, (nq, k)), od.to_host(np.float32, (nq, k))
     i16, d16 = oi16.to_host(np.int32, (nq, k)), od16.to_host(np.float32, (nq, k))
     o64 = dev.DeviceBuffer(nq * k * 8)
@@ -682,7 +691,8 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="profile runs: first copy this many MiB three times with a plain "
                          "copy kernel (the reference point of the HBM traffic counters)")
-    ap.add_argument("--only", choices=["kmeans-large", "d960", "batched"], default=None,
+    ap.add_argument("--only", choices=["kmeans-large", "d960", "d960-bf16", "batched"],
+                    default=None,
                     help="profile runs: run one extra workload alone and print its record")
     ap.add_argument("--nb", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
@@ -712,6 +722,7 @@ def main():
 
     if args.only:
         only = {"kmeans-large": bench_kmeans_large, "d960": bench_precision_d960,
+                "d960-bf16": lambda ya_, dev_: bench_precision_d960(ya_, dev_, fp32=False),
                 "batched": bench_batched}[args.only]
         rec = only(ya, dev)
         print(json.dumps(rec, default=lambda x: x.tolist() if hasattr(x, "tolist") else float(x)),

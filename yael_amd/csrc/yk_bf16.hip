@@ -37,6 +37,7 @@
 // with gamma = 2 (d' + 8) 2^-23 (d' = padded length, truncating adds allowed).
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdlib.h>
 
 #include "yk.h"
 #include "yk_dev.h"
@@ -359,7 +360,7 @@ __device__ __forceinline__ void yk_reg_insert16(uint64_t (&t)[R], uint64_t x) {
 
 // Per-wave selection state over NB 32-query column tiles: lane (r, h) owns
 // query r of each tile and, per 32-row block, rows 8g + 4h + e (g, e < 4).
-template <int NB, int MODE, int R>
+template <int NB, int MODE, int R, int CST = YK_CSTAGE>
 struct BfSel {
   long qrow[NB];
   bool qok[NB];
@@ -367,7 +368,7 @@ struct BfSel {
   float bv[NB][MODE == 1 ? 2 : 1];  // MODE 1: minima of the even and odd blocks
   uint64_t T[NB];
   float Tv[NB];
-  // MODE 3: candidate keys are staged in LDS, YK_CSTAGE per (query tile,
+  // MODE 3: candidate keys are staged in LDS, CST per (query tile,
   // lane), and appended to the query's global list a region at a time (one
   // atomic per region instead of one per key)
   uint64_t *cs;
@@ -472,9 +473,9 @@ struct BfSel {
           if (key < top[j][R - 1]) yk_reg_insert16<R>(top[j], key);
         } else if (key <= T[j]) {
           const int n = ncs[j];
-          cs[(j * 64 + lane) * YK_CSTAGE + n] = key;
-          if (n + 1 == YK_CSTAGE) {
-            flush(a, j, YK_CSTAGE);
+          cs[(j * 64 + lane) * CST + n] = key;
+          if (n + 1 == CST) {
+            flush(a, j, CST);
             ncs[j] = 0;
           } else {
             ncs[j] = n + 1;
@@ -488,7 +489,7 @@ struct BfSel {
   // growing past YK_CAND_CAP: the select kernel detects the overflow)
   __device__ __forceinline__ void flush(const Bf16Args &a, int j, int n) {
     const int base = atomicAdd(&a.cnt[qrow[j]], n);
-    const uint64_t *src = cs + (j * 64 + lane) * YK_CSTAGE;
+    const uint64_t *src = cs + (j * 64 + lane) * CST;
     for (int s = 0; s < n; s++)
       if (base + s < YK_CAND_CAP) a.cand[qrow[j] * YK_CAND_CAP + base + s] = src[s];
   }
@@ -691,53 +692,80 @@ yk_scan_bf16_w2(Bf16Args a) {
 }
 
 
-// Long vectors (d'/16 > 16 chunks): neither operand fits in registers.  Each
-// wave owns NB query tiles (NB = 3: 96 queries) and takes the base in groups
-// of G = 2 blocks: the accumulators of all G x NB tiles stay in registers
-// while the chunk loop walks K, so the NB + G fragments loaded per chunk feed
-// G x NB MFMAs (0.83 KB per MFMA at NB = 3; the register-resident kernel's
-// streaming at long d loaded three fragments per two MFMAs).  The queries
-// come from a fragment-major copy (qf), so every load is one contiguous 1 KB
-// fragment.  Chunk c + 1's fragments load while chunk c's MFMAs run (two
-// register sets used in turn, the loop unrolled by two).  Selection per
-// block as in yk_scan_bf16.  The kernel is held to two waves per SIMD, which
-// at NB = 3 spills a few selection registers to scratch; measured at d=960:
-// 156K queries/s against 140K with NB = 2 (NB = 4 spills 450 bytes per lane;
-// groups of 3 or 4 blocks measured no faster and need one wave per SIMD).
-template <int MODE, int R, int NB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
-yk_scan_bf16_long(Bf16Args a) {
-  constexpr int G = 2;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+// Long vectors, LDS-tiled (d'/16 > 16 chunks; the sketch and the filtered
+// pass).  Neither operand fits in registers for the whole K, so the
+// workgroup's YK_TW = 8 waves share YK_TQ = 4 query tiles (128 queries)
+// staged through LDS YK_TS = 3 chunks at a time, in two buffers used in
+// turn (one barrier per stage: a stage's buffer is written only after every
+// wave has left the stage that last read it), and each wave streams its own
+// YK_TG = 2 base blocks per step (the workgroup 16 blocks, 512 rows) straight
+// to registers with buffer loads issued a stage ahead, in a ring of YK_TS
+// slots whose slot is the chunk's position in its stage (a compile-time
+// index): a slot reloads as soon as the MFMAs reading it have issued.  Per
+// chunk a wave reads 4 fragments from LDS and 2 from memory for 8 MFMAs
+// (the per-wave kernel this replaces read 5 fragments from memory for 6
+// MFMAs, and its workgroup's waves read the same base blocks through the
+// vector L1, whose line rate bounded it at 15% of the MFMA peak).  What
+// bounds this kernel is the data brought to the CUs: every 512-row step
+// re-reads the 250 KB query tile (8 waves per workgroup: 19.4 ms per
+// filtered pass at d=960 against 21.4 ms with 4, the query tile staged once
+// per 256 rows).  The queries of a stage come from the fragment-major copy
+// (qf), wave w < 4 loading tile w's fragments one stage ahead into
+// registers and writing them to LDS at the stage's end.  Selection as in
+// yk_scan_bf16: every wave of the workgroup holds the same queries (so
+// MODE 3's "any active" test is uniform over the workgroup); the sketch's
+// disjoint row subsets are (split, wave, lane half, block parity), so
+// MODE 1 writes nsplit x 8 (the waves) partial minima per query and half.
+#define YK_TQ 4
+#define YK_TG 2
+#define YK_TS 3
+#define YK_TCST 4  // MODE 3 keys staged per (query tile, lane)
+#define YK_TW 8    // waves per workgroup
+
+template <int MODE, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+yk_scan_bf16_tiled(Bf16Args a) {
+  constexpr int NB = YK_TQ, G = YK_TG, S = YK_TS;
+  // the wave index as a scalar: the base offsets derived from it then stay
+  // in scalar registers (a per-lane offset turns every buffer load into a
+  // readfirstlane loop)
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int r = lane & 31, h = lane >> 5;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, wpx = gridDim.x >> 3;
   const int ibeg = xcd * a.per_xcd, iend = min(a.total, ibeg + a.per_xcd);
   const int nch = a.ldh >> 4;
-  __shared__ uint64_t cstage[MODE == 3 ? 4 * NB * 64 * YK_CSTAGE : 1];
-  __shared__ float vstage[MODE >= 2 ? 4 * 64 * 16 : 1];
+  const int nstage = (nch + S - 1) / S;
+  __shared__ bf16x8 qs[2][S][NB][64];
+  __shared__ uint64_t cstage[MODE == 3 ? W * NB * 64 * YK_TCST : 1];
+  __shared__ float vstage[MODE == 3 ? W * 64 * 16 : 1];
+  // the sketch's partial minima: nsplit x W subsets per query and half
+  Bf16Args aw = a;
+  aw.nsplit = a.nsplit * W;
+  // items split-major: the workgroups an XCD runs at once share one base
+  // range (measured: grouping a few query tiles per XCD instead, to keep
+  // their fragments in L2, fetched 23-49% more bytes and ran 8-21% slower)
   for (int item = ibeg + local; item < iend; item += wpx) {
     const int tile = item % a.tiles_x, split = item / a.tiles_x;
-    const long qbase = ((long)tile * nwave + wave) * (32 * NB);
+    const long qbase = (long)tile * (32 * NB);
     const long bbeg = (long)split * a.brange;
     const long bend = min(a.nb, bbeg + a.brange);
-    BfSel<NB, MODE, R> sel;
-    sel.init(a, qbase, r, lane, cstage + wave * (MODE == 3 ? NB * 64 * YK_CSTAGE : 0),
-             vstage + (MODE >= 2 ? (wave * 64 + lane) * 16 : 0));
-    if (MODE == 3 && !__any(sel.active())) continue;
-    // query tiles qbase/32 .. qbase/32 + NB - 1 of the fragment-major copy
-    // (its rows are padded to the launch's query span, the pad rows zero)
-    const uint16_t *qf0 = a.qf + ((qbase >> 5) * nch * 64 + lane) * 8;
-    const long qtstride = (long)nch * 512;  // elements between query tiles
+    BfSel<NB, MODE, 1, YK_TCST> sel;
+    sel.init(a, qbase, r, lane, cstage + wave * (MODE == 3 ? NB * 64 * YK_TCST : 0),
+             vstage + (MODE == 3 ? (wave * 64 + lane) * 16 : 0));
+    if (MODE == 3 && !__any(sel.active())) continue;  // the same in every wave
+    // waves 0 .. NB-1 stage the queries, wave w tile (qbase/32 + w)
+    const bool qloader = wave < NB;
+    const uint16_t *qfw = a.qf + (((qbase >> 5) + (qloader ? wave : 0)) * nch * 64 + lane) * 8;
     const long nbk = (a.nb + 31) >> 5;
     const long rbytes = (nbk - (bbeg >> 5)) * (long)nch * 1024;
     const uint16_t *ibase = a.b + (bbeg >> 5) * (long)nch * 512;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)ibase, 0, (int)min(rbytes, 0x7fffffffL), YK_BUF_DWORD3);
     const int voff = lane * 16;
-    const int gstride = (int)((a.bstep >> 5) * nch * 1024);  // bytes between a group's blocks
-    int sub = 0;
-    for (long bb = bbeg; bb < bend; bb += G * a.bstep) {
-      const int sblk = (int)(((bb - bbeg) >> 5) * nch * 1024);
+    const int bstride = (int)((a.bstep >> 5) * nch * 1024);  // bytes between sampled blocks
+    for (long bb0 = bbeg; bb0 < bend; bb0 += W * G * a.bstep) {
+      const long bw = bb0 + (long)wave * G * a.bstep;  // this wave's first block
+      const int sw = (int)(((bw - bbeg) >> 5) * nch * 1024);
       f32x16 acc[G][NB];
 #pragma unroll
       for (int g = 0; g < G; g++)
@@ -745,40 +773,81 @@ yk_scan_bf16_long(Bf16Args a) {
         for (int j = 0; j < NB; j++)
 #pragma unroll
           for (int e = 0; e < 16; e++) acc[g][j][e] = 0.f;
-      bf16x8 qa[NB], ba[G], qb[NB], bv2[G];
-      // past the range, the base loads return zeros (buffer bounds)
-      auto ld = [&](int c, bf16x8(&q)[NB], bf16x8(&bv)[G]) {
+      // query fragments of stage st for this wave's tile (chunk indices
+      // clamped: a stage past the last chunk reloads it, never used)
+      bf16x8 qn[S];
+      auto qload = [&](int st) {
+        if (qloader) {
 #pragma unroll
-        for (int j = 0; j < NB; j++) q[j] = *(const bf16x8 *)(qf0 + j * qtstride + c * 512);
+          for (int t = 0; t < S; t++) {
+            const int c = min(st * S + t, nch - 1);
+            qn[t] = *(const bf16x8 *)(qfw + c * 512);
+          }
+        }
+      };
+      auto qstore = [&](int buf) {
+        if (qloader) {
+#pragma unroll
+          for (int t = 0; t < S; t++) qs[buf][t][wave][lane] = qn[t];
+        }
+      };
+      // base fragments, a ring of S chunk slots, chunk c in slot c % S (past
+      // the base end the buffer bounds return zeros; past the range, rows
+      // are not selected)
+      bf16x8 ring[S][G];
+      auto bload = [&](int slot, int c) {
+        const int cc = min(c, nch - 1);
 #pragma unroll
         for (int g = 0; g < G; g++)
-          bv[g] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
-                                                 rs, voff, sblk + g * gstride + c * 1024, 0));
+          ring[slot][g] = __builtin_bit_cast(
+              bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, voff, sw + g * bstride + cc * 1024,
+                                                            0));
       };
-      auto mm = [&](const bf16x8(&q)[NB], const bf16x8(&bv)[G]) {
+      qload(0);
 #pragma unroll
-        for (int g = 0; g < G; g++)
+      for (int t = 0; t < S; t++) bload(t, t);
+      qstore(0);
+      qload(1);
+      __syncthreads();
+      for (int st = 0; st < nstage; st++) {
+        const int buf = st & 1;
+        // the stage's query fragments read from LDS one chunk ahead of the
+        // MFMAs that use them (two register sets used in turn)
+        bf16x8 qv[2][NB];
 #pragma unroll
-          for (int j = 0; j < NB; j++)
-            acc[g][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv[g], q[j], acc[g][j], 0, 0, 0);
-      };
-      ld(0, qa, ba);
-      for (int c = 0; c < nch; c += 2) {
-        const int c1 = c + 1 < nch ? c + 1 : c, c2 = c + 2 < nch ? c + 2 : c1;
-        ld(c1, qb, bv2);
-        mm(qa, ba);
-        if (c + 1 >= nch) break;
-        ld(c2, qa, ba);
-        mm(qb, bv2);
+        for (int j = 0; j < NB; j++) qv[0][j] = qs[buf][0][j][lane];
+#pragma unroll
+        for (int t = 0; t < S; t++) {
+          const int c = st * S + t;
+          if (t + 1 < S) {
+#pragma unroll
+            for (int j = 0; j < NB; j++) qv[(t + 1) & 1][j] = qs[buf][t + 1][j][lane];
+          }
+          if (c < nch) {
+#pragma unroll
+            for (int g = 0; g < G; g++)
+#pragma unroll
+              for (int j = 0; j < NB; j++)
+                acc[g][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ring[t][g], qv[t & 1][j],
+                                                                    acc[g][j], 0, 0, 0);
+          }
+          // slot t's MFMAs are issued: it takes chunk c + S, a whole stage
+          // ahead of its use
+          __builtin_amdgcn_sched_barrier(0);
+          bload(t, c + S);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        qstore(buf ^ 1);
+        qload(st + 2);
+        __syncthreads();
       }
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        const long bg = bb + g * a.bstep;
-        if (bg < bend) sel.block(a, acc[g], bg, bend, h, sub);
-        sub ^= 1;
+        const long bg = bw + g * a.bstep;
+        if (bg < bend) sel.block(a, acc[g], bg, bend, h, g & 1);
       }
     }
-    sel.finish(a, qbase, split, r, h);
+    sel.finish(aw, qbase, split * W + wave, r, h);
   }  // work items
 }
 
@@ -1389,7 +1458,7 @@ static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, 
 // faster than two tiles at k = 65536); else two.  Each base fragment loaded
 // then feeds three MFMAs instead of two.
 static int yk_bnb(int mode, int nch, int r = YK_REG_R) {
-  if (mode != 2 && nch > 16) return 3;  // yk_scan_bf16_long<MODE, R, 3>
+  if (mode != 2 && nch > 16) return 1;  // yk_scan_bf16_tiled: 128 queries per workgroup
   if (mode == 2 && r <= 8 && nch >= 1 && nch <= 9) return 3;  // yk_scan_bf16_w2
   return mode != 2 && nch >= 1 && nch <= 9 ? 3 : 2;
 }
@@ -1442,7 +1511,7 @@ static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
     }
   }
   if constexpr (MODE != 2) {
-    if ((a.ldh >> 4) > 16) kern = yk_scan_bf16_long<MODE, R, 3>;
+    if ((a.ldh >> 4) > 16) kern = yk_scan_bf16_tiled<MODE, YK_TW>;
   }
   a.brange = p.brange;
   a.bstep = p.bstep;
@@ -1450,8 +1519,9 @@ static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
   a.tiles_x = (int)((a.nq + 128L * nbq - 1) / (128L * nbq));
   a.total = a.tiles_x * p.nsplit;
   a.per_xcd = (a.total + 7) / 8;
-  const int per_res = yk_resident_wgs((const void *)kern, 256) / 8;
-  const dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), blk(256);
+  const int nthr = (MODE != 2 && (a.ldh >> 4) > 16) ? 64 * YK_TW : 256;
+  const int per_res = yk_resident_wgs((const void *)kern, nthr) / 8;
+  const dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), blk(nthr);
   int tok = yk_prof_begin(MODE == 1 ? YK_PROF_BF16_SKETCH : YK_PROF_BF16);
   hipLaunchKernelGGL(kern, grid, blk, 0, yk_stream(), a);
   yk_prof_end(tok);
@@ -1521,10 +1591,13 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   while (stride > 1 && nblocks / stride < 2L * kp) stride >>= 1;
   BPlan pa = yk_bplan(nq, a.nb, yk_bnb(1, a.ldh >> 4), stride, kp, kp > 256 ? kp : 256,
                       64L * a.ldh);
-  YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * pa.nsplit * 4, &vpart));
+  // per query and split: 2 lane halves x 2 block parities of minima, and
+  // for long vectors that per wave of the tiled kernel's workgroup
+  const int nsub = (a.ldh >> 4) > 16 ? 4 * YK_TW : 4;
+  YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * pa.nsplit * nsub, &vpart));
   a.part = (uint64_t *)vpart;
   YK_TRY((yk_launch_bscan<1, 1>(pa, a)));
-  YK_TRY(yk_bf_sortsel(a.part, nq, pa.nsplit * 4, kp, nullptr, thr));
+  YK_TRY(yk_bf_sortsel(a.part, nq, pa.nsplit * nsub, kp, nullptr, thr));
   hipLaunchKernelGGL(yk_bf_zero_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
                      yk_stream(), cnt, nq);
   hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), flag, 1L);
