@@ -206,3 +206,46 @@ def test_profiled_batched_run_leaves_no_error(ya):
     compare(b, single(ya, v, sizes, 8, 4, [3, 4, 5]), sizes)
     idx, dis = ya.knn(v[:5], v, 1)
     assert np.array_equal(idx[:, 0], np.arange(5))
+
+
+@pytest.mark.parametrize("k,d", [(256, 128), (170, 100), (600, 36)])
+def test_tables_larger_than_lds_match_single_runs(ya, k, d):
+    """k x d beyond one LDS table (bf = 256 at d = 128 is 264 KB of FP64
+    sums): the partial sums run in cluster slices, each instance still
+    equal to its single-instance kmeans() bit for bit (d = 128: 32-column
+    groups; d = 100: a thread per dimension; d = 36: float4 columns)."""
+    rng = np.random.default_rng(k + d)
+    sizes = zipf_sizes(6, 6 * 4 * k, 2 * k, 10 * k)
+    v = mixture(rng, int(sizes.sum()), d, 400, 18)
+    seeds = np.arange(11, 11 + len(sizes))
+    print("k", k, "d", d, "sizes", sizes.tolist(), flush=True)
+    b = ya.kmeans_batched(v, sizes, k, niter=6, seeds=seeds)
+    compare(b, single(ya, v, sizes, k, 6, seeds), sizes)
+
+
+def test_niter_zero_runs_to_convergence(ya):
+    """niter = 0 means until convergence (at most 1,000,000 iterations,
+    kmeans.c:345): the batched run goes window by window until every
+    instance has stopped, and each instance equals its single-instance
+    kmeans(niter = 0) bit for bit."""
+    import ctypes
+    rng = np.random.default_rng(77)
+    sizes = zipf_sizes(10, 30000, 500, 9000)
+    v = mixture(rng, int(sizes.sum()), 24, 60, 25)
+    seeds = np.arange(21, 21 + len(sizes))
+    b = ya.kmeans_batched(v, sizes, 20, niter=0, seeds=seeds)
+    compare(b, single(ya, v, sizes, 20, 0, seeds), sizes)
+    # the device-level entry reports the iterations each instance ran:
+    # past one window of 16, and fewer than the cap
+    from yael_amd.device import DeviceSet
+    V = DeviceSet(v)
+    n = int(sizes.sum())
+    iters = np.empty(len(sizes), np.int32)
+    chunk = ctypes.c_int(0)
+    ya.check(ya.lib().yael_kmeans_batched_device(24, len(sizes), sizes.ctypes.data, 20, 0,
+                                                 V.vec.ptr, 0x10000, seeds.ctypes.data, None,
+                                                 None, None, None, None, iters.ctypes.data, 1,
+                                                 ctypes.byref(chunk)))
+    V.free()
+    print("iterations per instance", iters.tolist(), flush=True)
+    assert iters.max() > 16 and iters.max() < 1000

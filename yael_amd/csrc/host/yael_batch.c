@@ -9,10 +9,14 @@
  * calls would).  The host does what depends only on the seeds: the seed
  * chains (kmeans.c:379-409), the random initialisation (kmeans.c:15-20) and
  * the split streams (kmeans.c:183-195) -- for every instance and iteration
- * the first YH_ECAP (draw, gaussian vector) pairs are precomputed -- plus the
- * work-item tables.  The device then runs every iteration without the host
- * (yk_batch.hip), captured once in a hipGraph.  An instance that empties more
- * than YH_ECAP clusters in one iteration is flagged and recomputed by the
+ * the first YH_ECAP (draw, gaussian vector) pairs are precomputed, a window
+ * of iterations at a time -- plus the work-item tables.  The device then
+ * runs the window's iterations without the host (yk_batch.hip), captured
+ * once in a hipGraph and replayed; between windows the host reads the
+ * instances' done flags and stops once every instance has stopped, so
+ * niter = 0 ("until convergence", at most 1,000,000 iterations, kmeans.c:345)
+ * costs the iterations actually run.  An instance that empties more than
+ * YH_ECAP clusters in one iteration is flagged and recomputed by the
  * single-instance path, so the results never depend on the cap.
  *
  * The chunk size of the update's partial sums is chosen by a small
@@ -31,6 +35,12 @@
 
 #define YH_ECAP 4
 #define YH_TILE 128 /* points per assign work item (4 waves x 32) */
+/* split-table window: at most this many bytes of draws on the device, and
+   when niter exceeds YH_WIN_LONG, windows of YH_WIN_CHECK iterations (how
+   often a long run checks whether every instance has stopped) */
+#define YH_WIN_BYTES (256L << 20)
+#define YH_WIN_LONG 64
+#define YH_WIN_CHECK 16
 
 typedef struct {
   void *p[32];
@@ -96,7 +106,7 @@ void yh_tune_put(const char *key, int val) {
 /* ---- the batched run ---- */
 
 typedef struct {
-  int ninst, d, ld, k, kpad, niter;
+  int ninst, d, ld, k, kpad, niter, win;
   long ntot;
   long *off; /* host offsets */
   /* device */
@@ -135,7 +145,8 @@ static int yh_batch_bind(yh_bstate *s, int P, yh_allocs *al, yk_batch **out) {
   free(uf);
   free(ui);
   if (rc != YK_OK) return rc;
-  return yk_batch_create(out, ninst, s->d, s->k, s->kpad, s->niter, YH_ECAP, P, s->ntot, s->v,
+  return yk_batch_create(out, ninst, s->d, s->k, s->kpad, s->niter, s->win, YH_ECAP, P, s->ntot,
+                         s->v,
                          s->vn, s->cent, s->cn, s->assign, s->dis, s->d_off, s->nitems,
                          s->item_inst, s->item_tile, nu, d_ui, d_uf, psum, pcnt, s->nassign,
                          s->pbuf, s->rd, s->eps, s->flags, s->qerr2);
@@ -146,14 +157,17 @@ static int yh_batch_init_state(yh_bstate *s) {
   return rc;
 }
 
-/* host precomputation of instances [i0, ninst) in steps of step: the
-   initial selection (kmeans.c:15-20, 402) and, per iteration, the split
-   stream's first YH_ECAP (draw, gaussian vector) pairs (kmeans.c:183-195,
-   296, 408-409) */
+/* host precomputation for instances [i0, ninst) in steps of step.  First
+   call (win == 0): the initial selection (kmeans.c:15-20, 402) and the
+   state of the instance's per-iteration seed chain (kmeans.c:408-409).
+   Window calls: for the next win iterations, the split stream's first
+   YH_ECAP (draw, gaussian vector) pairs (kmeans.c:183-195, 296), advancing
+   the chain */
 typedef struct {
-  int ninst, niter, k, kpad, d;
+  int ninst, win, k, kpad, d;
   const int *n_inst;
   const long *off, *seed_used;
+  unsigned *chain;
   int *sel;
   double *rd;
   float *eps;
@@ -162,29 +176,36 @@ typedef struct {
 
 static void *yh_prep_worker(void *arg) {
   yh_prep *p = (yh_prep *)arg;
-  const int niter = p->niter, d = p->d;
-  int *tmp = (int *)malloc(sizeof(int) * p->k);
-  if (!tmp) {
-    p->failed = 1;
+  const int win = p->win, d = p->d;
+  if (win == 0) {
+    int *tmp = (int *)malloc(sizeof(int) * p->k);
+    if (!tmp) {
+      p->failed = 1;
+      return NULL;
+    }
+    for (int i = p->i0; i < p->ninst; i += p->step) {
+      unsigned seed = (unsigned)p->seed_used[i];
+      yh_random_init(p->n_inst[i], p->k, tmp, (unsigned)rand_r(&seed));
+      for (int c = 0; c < p->kpad; c++)
+        p->sel[(long)i * p->kpad + c] = (int)(p->off[i] + tmp[c < p->k ? c : 0]);
+      p->chain[i] = (unsigned)rand_r(&seed);
+    }
+    free(tmp);
     return NULL;
   }
   for (int i = p->i0; i < p->ninst; i += p->step) {
-    unsigned seed = (unsigned)p->seed_used[i];
-    yh_random_init(p->n_inst[i], p->k, tmp, (unsigned)rand_r(&seed));
-    for (int c = 0; c < p->kpad; c++)
-      p->sel[(long)i * p->kpad + c] = (int)(p->off[i] + tmp[c < p->k ? c : 0]);
-    unsigned cs = (unsigned)rand_r(&seed);
-    for (int it = 0; it < niter; it++) {
+    unsigned cs = p->chain[i];
+    for (int it = 0; it < win; it++) {
       unsigned S = (unsigned)rand_r(&cs); /* kmeans.c:296 */
       for (int e = 0; e < YH_ECAP; e++) {
-        long t = ((long)i * niter + it) * YH_ECAP + e;
+        long t = ((long)i * win + it) * YH_ECAP + e;
         p->rd[t] = yh_drand_r(&S);
         unsigned gs = (unsigned)rand_r(&S);
         for (int l = 0; l < d; l++) p->eps[t * d + l] = (float)yh_gaussrand_r(&gs);
       }
     }
+    p->chain[i] = cs;
   }
-  free(tmp);
   return NULL;
 }
 
@@ -194,7 +215,7 @@ static void *yh_prep_worker(void *arg) {
 static void yh_prep_run(yh_prep *p) {
   long work = 0;
   for (int i = 0; i < p->ninst; i++)
-    work += p->n_inst[i] + (long)p->niter * YH_ECAP * p->d;
+    work += p->win ? (long)p->win * YH_ECAP * p->d : p->n_inst[i];
   int nt = work < 200000 ? 1 : 16;
   if (nt > p->ninst) nt = p->ninst;
   yh_prep job[16];
@@ -227,8 +248,8 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
     return yk_set_error_c(YK_EINVAL, "batched k-means: only random init, L2, plain centroids");
   if (ninst < 1 || k < 1 || d < 1 || niter < 0)
     return yk_set_error_c(YK_EINVAL, "batched k-means: bad sizes");
-  if (niter == 0) niter = 1000000;
-  if (niter > 100000) return yk_set_error_c(YK_EINVAL, "batched k-means: niter too large");
+  const int niter_in = niter;
+  if (niter == 0) niter = 1000000; /* kmeans.c:345 */
   int rc = yk_init();
   if (rc != YK_OK) return rc;
   yh_bstate s;
@@ -251,24 +272,37 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   }
   s.ntot = s.off[ninst];
   const int ld = s.ld, kpad = s.kpad;
+  /* the split-table window */
+  {
+    const long per_iter = (long)ninst * YH_ECAP * (long)(sizeof(double) + sizeof(float) * d);
+    long w = YH_WIN_BYTES / per_iter;
+    if (niter > YH_WIN_LONG && w > YH_WIN_CHECK) w = YH_WIN_CHECK;
+    if (w > niter) w = niter;
+    s.win = w < 1 ? 1 : (int)w;
+  }
+  const int win = s.win;
 
   /* seed chains (lrand48 in instance order, as successive kmeans() calls
      draw them), then per instance, on up to 16 host threads, the initial
      selection and the split streams (each a function of its seed only) */
   int *sel = (int *)malloc(sizeof(int) * (size_t)ninst * kpad);
-  double *rd = (double *)malloc(sizeof(double) * (size_t)ninst * niter * YH_ECAP);
-  float *eps = (float *)malloc(sizeof(float) * (size_t)ninst * niter * YH_ECAP * d);
+  double *rd = (double *)malloc(sizeof(double) * (size_t)ninst * win * YH_ECAP);
+  float *eps = (float *)malloc(sizeof(float) * (size_t)ninst * win * YH_ECAP * d);
   long *seed_used = (long *)malloc(sizeof(long) * ninst);
-  int host_ok = sel && rd && eps && seed_used;
+  unsigned *chain = (unsigned *)malloc(sizeof(unsigned) * ninst);
+  int host_ok = sel && rd && eps && seed_used && chain;
+  yh_prep prep = {ninst, 0, k, kpad, d, n_inst, s.off, seed_used, chain, sel, rd, eps, 0, 1, 0};
   if (host_ok) {
     for (int i = 0; i < ninst; i++) {
       long si = seeds ? seeds[i] : 0;
       if (si == 0) si = lrand48(); /* kmeans.c:379-380 */
       seed_used[i] = si;
     }
-    yh_prep prep = {ninst, niter, k, kpad, d, n_inst, s.off, seed_used, sel, rd, eps, 0, 1, 0};
-    yh_prep_run(&prep);
+    yh_prep_run(&prep); /* selections and seed chains */
     host_ok = !prep.failed;
+    prep.win = win;
+    if (host_ok) yh_prep_run(&prep); /* the first window's split draws */
+    host_ok = host_ok && !prep.failed;
   }
 
   /* assign work items: (instance, tile of 128 points) */
@@ -297,8 +331,8 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   s.nassign = (int *)yh_dmalloc(&al, sizeof(int) * (size_t)ninst * k);
   s.flags = (int *)yh_dmalloc(&al, sizeof(int) * (7L * ninst + 8));
   s.qerr2 = (double *)yh_dmalloc(&al, sizeof(double) * 2L * ninst);
-  s.rd = (double *)yh_dmalloc(&al, sizeof(double) * (size_t)ninst * niter * YH_ECAP);
-  s.eps = (float *)yh_dmalloc(&al, sizeof(float) * (size_t)ninst * niter * YH_ECAP * d);
+  s.rd = (double *)yh_dmalloc(&al, sizeof(double) * (size_t)ninst * win * YH_ECAP);
+  s.eps = (float *)yh_dmalloc(&al, sizeof(float) * (size_t)ninst * win * YH_ECAP * d);
   s.pbuf = (float *)yh_dmalloc(&al, sizeof(float) * (size_t)ninst * k);
   s.item_inst = (int *)yh_dmalloc(&al, sizeof(int) * nitems);
   s.item_tile = (int *)yh_dmalloc(&al, sizeof(int) * nitems);
@@ -310,17 +344,15 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   if (rc == YK_OK) rc = yk_memcpy_h2d(s.d_off, s.off, sizeof(long) * (ninst + 1));
   if (rc == YK_OK && !v_on_device) rc = yk_upload_vectors(vdev, v, s.ntot, d);
   if (rc == YK_OK) rc = yk_memcpy_h2d(s.sel, sel, sizeof(int) * (size_t)ninst * kpad);
-  if (rc == YK_OK) rc = yk_memcpy_h2d(s.rd, rd, sizeof(double) * (size_t)ninst * niter * YH_ECAP);
+  if (rc == YK_OK) rc = yk_memcpy_h2d(s.rd, rd, sizeof(double) * (size_t)ninst * win * YH_ECAP);
   if (rc == YK_OK)
-    rc = yk_memcpy_h2d(s.eps, eps, sizeof(float) * (size_t)ninst * niter * YH_ECAP * d);
+    rc = yk_memcpy_h2d(s.eps, eps, sizeof(float) * (size_t)ninst * win * YH_ECAP * d);
   if (rc == YK_OK) rc = yk_memcpy_h2d(s.item_inst, ii, sizeof(int) * nitems);
   if (rc == YK_OK) rc = yk_memcpy_h2d(s.item_tile, it_, sizeof(int) * nitems);
   if (rc == YK_OK) rc = yk_norms(vdev, s.ntot, d, NULL, s.vn);
   free(ii);
   free(it_);
   free(sel);
-  free(rd);
-  free(eps);
 
   /* autotune the update chunk (timed from the initial state, then reset) */
   int P = 0;
@@ -365,11 +397,35 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   if (rc == YK_OK) rc = yh_batch_bind(&s, P, &bal, &bt);
   if (rc == YK_OK) rc = yh_batch_init_state(&s);
   if (rc == YK_OK) rc = yk_batch_reset(bt);
-  if (rc == YK_OK) rc = yk_batch_run(bt, niter, use_graph);
+  /* window by window: while the device runs one, the host draws the next
+     one's splits; then, past the last window or once every instance has
+     stopped, the run ends */
+  int *hflags = (int *)malloc(sizeof(int) * (7L * ninst + 8));
+  if (rc == YK_OK && !hflags) rc = yk_set_error_c(YK_ENOMEM, "batched k-means: host malloc");
+  for (long w0 = 0; rc == YK_OK && w0 < niter; w0 += win) {
+    const int nw = (int)(niter - w0 < win ? niter - w0 : win);
+    if (w0 > 0) {
+      int w0i = (int)w0;
+      rc = yk_memcpy_h2d(s.rd, rd, sizeof(double) * (size_t)ninst * win * YH_ECAP);
+      if (rc == YK_OK)
+        rc = yk_memcpy_h2d(s.eps, eps, sizeof(float) * (size_t)ninst * win * YH_ECAP * d);
+      if (rc == YK_OK) rc = yk_memcpy_h2d(s.flags + 6L * ninst + 1, &w0i, sizeof(int));
+    }
+    if (rc == YK_OK) rc = yk_batch_run(bt, nw, use_graph);
+    if (rc != YK_OK || w0 + nw >= niter) break;
+    yh_prep_run(&prep); /* the next window's draws, while the device runs */
+    if (prep.failed) rc = yk_set_error_c(YK_ENOMEM, "batched k-means: host malloc");
+    if (rc == YK_OK) rc = yk_memcpy_d2h(hflags, s.flags, sizeof(int) * (size_t)ninst);
+    int all_done = 1;
+    for (int i = 0; rc == YK_OK && i < ninst && all_done; i++) all_done = hflags[i] != 0;
+    if (all_done) break;
+  }
   if (rc == YK_OK) rc = yk_sync();
+  free(rd);
+  free(eps);
+  free(chain);
 
   /* results */
-  int *hflags = (int *)malloc(sizeof(int) * (7L * ninst + 8));
   double *hq = (double *)malloc(sizeof(double) * 2L * ninst);
   if (rc == YK_OK) rc = yk_memcpy_d2h(hflags, s.flags, sizeof(int) * (7L * ninst + 8));
   if (rc == YK_OK) rc = yk_memcpy_d2h(hq, s.qerr2, sizeof(double) * 2L * ninst);
@@ -403,7 +459,7 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
       float *ci = (float *)malloc(sizeof(float) * (size_t)k * d);
       float r = 0;
       if (rc == YK_OK)
-        rc = yh_kmeans(d, (int)n, k, niter, vi, 0, flags | KMEANS_QUIET, seed_used[i], 1, ci,
+        rc = yh_kmeans(d, (int)n, k, niter_in, vi, 0, flags | KMEANS_QUIET, seed_used[i], 1, ci,
                        dis_out ? dis_out + s.off[i] : NULL,
                        assign_out ? assign_out + s.off[i] : NULL,
                        nassign_out ? nassign_out + (long)i * k : NULL, &r);

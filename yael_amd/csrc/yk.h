@@ -163,16 +163,20 @@ int yk_gather_rows(const float *src, const int *idx, long n, int d, float *dst);
  * tile), centroid sets kpad rows apart.  yk_batch_create binds device
  * arrays laid out by the caller (see host/yael_batch.c; psum holds
  * nuitems * (k * d + 1) doubles: the chunks' partial sums, then their
- * objective partials); yk_batch_run queues
+ * objective partials; flags holds 7 x ninst ints, then the iteration
+ * counter, then w0).  The split draws rd / eps cover a window of win
+ * iterations per instance (row it - w0 of instance i, w0 the window's first
+ * iteration); the caller refills them and w0 between windows.  k * d tables
+ * larger than the LDS are summed in cluster slices.  yk_batch_run queues
  * niter iterations, replaying one captured hipGraph when use_graph. */
 typedef struct yk_batch yk_batch;
 int yk_batch_assign(long nitems, const int *item_inst, const int *item_tile, const long *inst_off,
                     int d, int kc, int kpad, const float *v, const double *vn, const float *cent,
                     const float *cn, const int *done, int *assign, float *dis);
 size_t yk_batch_lds_bytes(int k, int d);
-int yk_batch_create(yk_batch **out, int ninst, int d, int k, int kpad, int niter, int ecap,
-                    int chunk, long ntot, const float *v, const double *vn, float *cent, float *cn,
-                    int *assign, float *dis, const long *inst_off, long nitems,
+int yk_batch_create(yk_batch **out, int ninst, int d, int k, int kpad, int niter, int win,
+                    int ecap, int chunk, long ntot, const float *v, const double *vn, float *cent,
+                    float *cn, int *assign, float *dis, const long *inst_off, long nitems,
                     const int *item_inst, const int *item_tile, long nuitems,
                     const int *uitem_inst, const int *uitem_first, double *psum, int *pcnt,
                     int *nassign, float *pbuf, const double *rd, const float *eps, int *flags,

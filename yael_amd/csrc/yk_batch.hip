@@ -35,6 +35,8 @@
 
 struct BatchDev {
   int ninst, d, ld, k, kpad, niter, ecap, chunk;
+  int kslice, nslice;  // the update's clusters per LDS table, and the tables per chunk
+  int win;             // iterations per window of the split tables
   long ntot;
   const float *v;   // ntot x ld
   const double *vn;  // ntot
@@ -52,11 +54,12 @@ struct BatchDev {
   int *pcnt;     // nuitems x k
   int *nassign;  // ninst x k
   float *pbuf;   // ninst x k (split probabilities)
-  const double *rd;   // ninst x niter x ecap
-  const float *eps;   // ninst x niter x ecap x d
+  const double *rd;   // ninst x win x ecap: the current window's draws
+  const float *eps;   // ninst x win x ecap x d
   int *done, *overflow, *nre, *tot, *iters, *abandoned;
   double *qerr, *qerr_old;
   int *it;  // iteration counter
+  int *w0;  // first iteration of the current window (set by the host)
 };
 
 // Row stride (doubles) of the LDS table: the column paths (d < 64 with d a
@@ -82,19 +85,26 @@ __host__ __device__ static inline int yk_bupd_stride(int d) {
 // sum is accumulated in point order.  Deterministic, whatever the grid.  The chunk's objective (its points' distances, FP64,
 // thread-strided then a fixed tree) goes to pobj[u].
 __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
-  extern __shared__ double lsum[];  // k x sd
+  extern __shared__ double lsum[];  // ks x sd
   __shared__ double wobj[4];
   const int d = b.d, ld = b.ld, sd = yk_bupd_stride(d);
-  int *lcnt = (int *)(lsum + (size_t)b.k * sd);
   const int tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63, wave = tid >> 6;
   const long u = blockIdx.x;
+  // this workgroup's clusters [kc0, kc0 + ks): tables larger than LDS are
+  // taken a slice at a time, each slice's workgroup reading every
+  // assignment of the chunk and only its own clusters' rows
+  const int kc0 = blockIdx.y * b.kslice, ks = min(b.kslice, b.k - kc0);
+  int *lcnt = (int *)(lsum + (size_t)ks * sd);
   const int inst = b.uitem_inst[u];
   if (b.done[inst]) return;
   const long off = b.inst_off[inst], n = b.inst_off[inst + 1] - off;
   const long c0 = (u - b.uitem_first[inst]) * (long)b.chunk;
   const long p0 = off + c0, p1 = off + (c0 + b.chunk < n ? c0 + b.chunk : n);
-  for (int i = tid; i < b.k * sd; i += nthr) lsum[i] = 0.0;
-  for (int i = tid; i < b.k; i += nthr) lcnt[i] = 0;
+  // a point's row in this slice's table, or -1 when its cluster is another
+  // slice's
+  auto slot = [&](int a) { return (unsigned)(a - kc0) < (unsigned)ks ? a - kc0 : -1; };
+  for (int i = tid; i < ks * sd; i += nthr) lsum[i] = 0.0;
+  for (int i = tid; i < ks; i += nthr) lcnt[i] = 0;
   // a point without a nearest centroid (NaN input) has assign -1: the
   // instance is handed to the single-instance path, which reports it
   bool bad = false;
@@ -103,7 +113,7 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
     if (tid == 0) b.overflow[inst] = 1;
     return;
   }
-  {  // the chunk objective
+  if (blockIdx.y == 0) {  // the chunk objective
     double obj = 0.0;
     for (long p = p0 + tid; p < p1; p += nthr) obj += (double)b.dis[p];
     for (int o = 32; o > 0; o >>= 1) obj += __shfl_xor(obj, o);
@@ -119,8 +129,8 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
 #pragma unroll
         for (int s = 0; s < 8; s++) {
           const long q = p + 8 * s;
-          a[s] = q < p1 ? b.assign[q] : -1;
-          x[s] = q < p1 ? *(const float4 *)(b.v + q * ld + j0) : make_float4(0.f, 0.f, 0.f, 0.f);
+          a[s] = q < p1 ? slot(b.assign[q]) : -1;
+          x[s] = a[s] >= 0 ? *(const float4 *)(b.v + q * ld + j0) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int s = 0; s < 8; s++) {
@@ -144,23 +154,23 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
     for (int t = tid; t < d; t += nthr) {
       long p = p0;
       for (; p + 4 <= p1; p += 4) {
-        const int a0 = b.assign[p], a1 = b.assign[p + 1], a2 = b.assign[p + 2],
-                  a3 = b.assign[p + 3];
-        const float x0 = b.v[p * ld + t], x1 = b.v[(p + 1) * ld + t], x2 = b.v[(p + 2) * ld + t],
-                    x3 = b.v[(p + 3) * ld + t];
-        lsum[a0 * d + t] += (double)x0;
-        lsum[a1 * d + t] += (double)x1;
-        lsum[a2 * d + t] += (double)x2;
-        lsum[a3 * d + t] += (double)x3;
-        if (t == 0) {
-          lcnt[a0]++;
-          lcnt[a1]++;
-          lcnt[a2]++;
-          lcnt[a3]++;
+        int a[4];
+        float x[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          a[s] = slot(b.assign[p + s]);
+          x[s] = a[s] >= 0 ? b.v[(p + s) * ld + t] : 0.f;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          if (a[s] < 0) continue;
+          lsum[a[s] * d + t] += (double)x[s];
+          if (t == 0) lcnt[a[s]]++;
         }
       }
       for (; p < p1; p++) {
-        const int a = b.assign[p];
+        const int a = slot(b.assign[p]);
+        if (a < 0) continue;
         lsum[a * d + t] += (double)b.v[p * ld + t];
         if (t == 0) lcnt[a]++;
       }
@@ -175,8 +185,8 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
 #pragma unroll
         for (int s = 0; s < 4; s++) {
           const long q = p + 64 * s;
-          a[s] = q < p1 ? b.assign[q] : -1;
-          x[s] = q < p1 ? *(const float4 *)(b.v + q * ld + j0) : make_float4(0.f, 0.f, 0.f, 0.f);
+          a[s] = q < p1 ? slot(b.assign[q]) : -1;
+          x[s] = a[s] >= 0 ? *(const float4 *)(b.v + q * ld + j0) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int s = 0; s < 4; s++) {
@@ -198,7 +208,8 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
     const int ppp = 64 / d, g = tid / d, t = tid - g * d;
     if (g < ppp) {
       for (long p = p0 + g; p < p1; p += ppp) {
-        const int a = b.assign[p];
+        const int a = slot(b.assign[p]);
+        if (a < 0) continue;
         __hip_atomic_fetch_add(&lsum[a * d + t], (double)b.v[p * ld + t], __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
         if (t == 0)
@@ -207,14 +218,14 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
     }
   }
   __syncthreads();
-  if (tid == 0) {
+  if (tid == 0 && blockIdx.y == 0) {
     double obj = 0.0;
     for (int w = 0; w < (nthr >> 6); w++) obj += wobj[w];
     b.pobj[u] = obj;
   }
-  double *ps = b.psum + u * (size_t)b.k * b.d;
-  for (int i = tid; i < b.k * d; i += nthr) ps[i] = lsum[(i / d) * sd + i % d];
-  for (int i = tid; i < b.k; i += nthr) b.pcnt[u * b.k + i] = lcnt[i];
+  double *ps = b.psum + (u * (size_t)b.k + kc0) * b.d;
+  for (int i = tid; i < ks * d; i += nthr) ps[i] = lsum[(i / d) * sd + i % d];
+  for (int i = tid; i < ks; i += nthr) b.pcnt[u * b.k + kc0 + i] = lcnt[i];
 }
 
 // One wave per (instance, cluster): chunk sums in chunk order, then
@@ -253,7 +264,9 @@ __device__ void yk_l1_normalize(float *p, int k) {
 __global__ void yk_bsplit_kernel(BatchDev b) {
   const int inst = blockIdx.x * blockDim.x + threadIdx.x;
   if (inst >= b.ninst || b.done[inst]) return;
-  const int it = min(*b.it, b.niter - 1);  // tuning runs may go past niter
+  // the iteration's row of the current window's tables (tuning runs, which
+  // may go past the window, reuse its last row)
+  const int it = min(max(*b.it - *b.w0, 0), b.win - 1);
   const int k = b.k, d = b.d;
   const int *na = b.nassign + (long)inst * k;
   float *p = b.pbuf + (long)inst * k;
@@ -262,7 +275,7 @@ __global__ void yk_bsplit_kernel(BatchDev b) {
     p[c] = na[c] < 2 ? 0.f : (float)(int)(u * u - 1u);  // 32-bit wrap as in the reference
   }
   yk_l1_normalize(p, k);
-  const long tab = ((long)inst * b.niter + it) * b.ecap;
+  const long tab = ((long)inst * b.win + it) * b.ecap;
   int e = 0;
   for (int c = 0; c < k; c++) {
     if (na[c] != 0) continue;
@@ -341,7 +354,10 @@ __global__ void yk_bstep_kernel(int *it) {
 
 __global__ void yk_binit_kernel(BatchDev b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *b.it = 0;
+  if (i == 0) {
+    *b.it = 0;
+    *b.w0 = 0;
+  }
   if (i < b.ninst) {
     b.done[i] = 0;
     b.overflow[i] = 0;
@@ -365,39 +381,59 @@ struct yk_batch {
   int captured;
 };
 
+// the update's LDS table: clusters per slice and bytes of a slice of ks
+#define YK_BUPD_LDS_MAX (160 * 1024)
+static size_t yk_bupd_lds(int ks, int d) {
+  return sizeof(double) * (size_t)ks * yk_bupd_stride(d) + sizeof(int) * (size_t)ks;
+}
+
+// clusters per slice: all k when the table fits, else k split into equal
+// slices that do
+static int yk_bupd_kslice(int k, int d) {
+  const int kmax = (int)(YK_BUPD_LDS_MAX / yk_bupd_lds(1, d));
+  if (kmax < 1) return 0;
+  const int nslice = (k + kmax - 1) / kmax;
+  return (k + nslice - 1) / nslice;
+}
+
 extern "C" size_t yk_batch_lds_bytes(int k, int d) {
-  return sizeof(double) * (size_t)k * yk_bupd_stride(d) + sizeof(int) * (size_t)k;
+  const int ks = yk_bupd_kslice(k, d);
+  return ks ? yk_bupd_lds(ks, d) : (size_t)-1;
 }
 
 // The device state of one batched run.  All pointers are device pointers
 // laid out by the caller (host/yael_batch.c) in one allocation.
 extern "C" int yk_batch_create(yk_batch **out, int ninst, int d, int k, int kpad, int niter,
-                               int ecap, int chunk, long ntot, const float *v, const double *vn,
-                               float *cent, float *cn, int *assign, float *dis,
+                               int win, int ecap, int chunk, long ntot, const float *v,
+                               const double *vn, float *cent, float *cn, int *assign, float *dis,
                                const long *inst_off, long nitems, const int *item_inst,
                                const int *item_tile, long nuitems, const int *uitem_inst,
                                const int *uitem_first, double *psum, int *pcnt, int *nassign,
                                float *pbuf, const double *rd, const float *eps, int *flags,
                                double *qerr2) {
   YK_TRY(yk_init());
-  if (yk_batch_lds_bytes(k, d) > 160 * 1024)
-    return yk_set_error(YK_EINVAL, "batched k-means: k*d = %d too large for the LDS table", k * d);
+  const int kslice = yk_bupd_kslice(k, d);
+  if (kslice < 1)
+    return yk_set_error(YK_EINVAL, "batched k-means: d = %d too large for one LDS table row", d);
+  if (win < 1) return yk_set_error(YK_EINVAL, "batched k-means: window of %d iterations", win);
   yk_batch *bt = new yk_batch();
   BatchDev &b = bt->dev;
   b.ninst = ninst; b.d = d; b.ld = yk_ld(d); b.k = k; b.kpad = kpad; b.niter = niter;
   b.ecap = ecap; b.chunk = chunk; b.ntot = ntot;
+  b.kslice = kslice; b.nslice = (k + kslice - 1) / kslice; b.win = win;
   b.v = v; b.vn = vn; b.cent = cent; b.cn = cn; b.assign = assign; b.dis = dis;
   b.inst_off = inst_off; b.nitems = nitems; b.item_inst = item_inst; b.item_tile = item_tile;
   b.nuitems = nuitems; b.uitem_inst = uitem_inst; b.uitem_first = uitem_first;
   b.psum = psum; b.pobj = psum + (size_t)nuitems * k * d; b.pcnt = pcnt; b.nassign = nassign; b.pbuf = pbuf; b.rd = rd; b.eps = eps;
-  // flags: 7 int arrays of ninst + the counter
+  // flags: 7 int arrays of ninst, the iteration counter, the window start
   b.done = flags; b.overflow = flags + ninst; b.nre = flags + 2L * ninst;
   b.tot = flags + 3L * ninst; b.iters = flags + 4L * ninst; b.abandoned = flags + 5L * ninst;
   b.it = flags + 6L * ninst;
+  b.w0 = flags + 6L * ninst + 1;
   b.qerr = qerr2; b.qerr_old = qerr2 + ninst;
   bt->captured = 0;
   *out = bt;
-  size_t lds = yk_batch_lds_bytes(k, d);
+  size_t lds = yk_bupd_lds(kslice, d);
   if (lds > 65536)
     YK_HIP(hipFuncSetAttribute((const void *)yk_bupd_partial_kernel,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -425,8 +461,8 @@ static int yk_batch_enqueue(yk_batch *bt) {
   const int pthreads = (b.d >= 64 && (b.d & 31) == 0) ? 64 * (b.d / 32 < 4 ? b.d / 32 : 4)
                        : b.d < 64      ? ((b.d & 3) == 0 ? 64 * (b.d / 4 < 4 ? b.d / 4 : 4) : 64)
                                        : (b.d > 256 ? 256 : ((b.d + 63) / 64) * 64);
-  hipLaunchKernelGGL(yk_bupd_partial_kernel, dim3((unsigned)b.nuitems), dim3(pthreads),
-                     yk_batch_lds_bytes(b.k, b.d), st, b);
+  hipLaunchKernelGGL(yk_bupd_partial_kernel, dim3((unsigned)b.nuitems, (unsigned)b.nslice),
+                     dim3(pthreads), yk_bupd_lds(b.kslice, b.d), st, b);
   hipLaunchKernelGGL(yk_bupd_reduce_kernel, dim3((unsigned)(((long)b.ninst * b.k + 3) / 4)),
                      dim3(256), 0, st, b);
   hipLaunchKernelGGL(yk_bsplit_kernel, dim3((b.ninst + 63) / 64), dim3(64), 0, st, b);
