@@ -29,22 +29,23 @@ def zipf_sizes(ninst, total, lo, hi, s=1.1):
     return n.astype(np.int32)
 
 
-def single(ya, v, sizes, k, niter, seeds):
+def single(ya, v, sizes, k, niter, seeds, init=None, flags=0x80000):
     """kmeans() per instance through the C-ABI (not ynumpy, which raises on
     the slightly negative objective the expanded distance can give on
-    duplicated points)."""
+    duplicated points); init: per-instance initial centroids
+    (KMEANS_INIT_USER)."""
     import ctypes
     out = []
     off = 0
     d = v.shape[1]
-    for n, s in zip(sizes, seeds):
+    for i, (n, s) in enumerate(zip(sizes, seeds)):
         vi = np.ascontiguousarray(v[off:off + n])
-        c = np.empty((k, d), np.float32)
+        c = np.empty((k, d), np.float32) if init is None else init[i].copy()
         dis = np.empty(n, np.float32)
         asg = np.empty(n, np.int32)
         nas = np.empty(k, np.int32)
         q = ctypes.c_float()
-        rc = ya.lib().yael_kmeans_checked(d, int(n), k, niter, vi.ctypes.data, 0x10000 | 0x80000,
+        rc = ya.lib().yael_kmeans_checked(d, int(n), k, niter, vi.ctypes.data, 0x10000 | flags,
                                          int(s), 1, c.ctypes.data, dis.ctypes.data, asg.ctypes.data,
                                          nas.ctypes.data, ctypes.byref(q))
         ya.check(rc, "kmeans")
@@ -249,3 +250,23 @@ def test_niter_zero_runs_to_convergence(ya):
     V.free()
     print("iterations per instance", iters.tolist(), flush=True)
     assert iters.max() > 16 and iters.max() < 1000
+
+
+@pytest.mark.parametrize("d", [48, 1100])
+def test_user_init_and_normalized_centroids_match_single_runs(ya, d):
+    """KMEANS_INIT_USER (the caller's initial centroids; no initialisation
+    draw from the seed) with KMEANS_NORMALIZE_CENTS (unit-norm centroids
+    after every update, before the split): each instance equals its
+    single-instance kmeans() with the same flags bit for bit (d = 1100: a
+    row longer than the single path's 1024-column LDS row)."""
+    rng = np.random.default_rng(d)
+    sizes = zipf_sizes(5, 4000, 300, 2000)
+    v = np.clip(np.rint(rng.uniform(0, 63, (int(sizes.sum()), d))), 0, 63).astype(np.float32)
+    k = 12
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    init = np.stack([v[off[i]:off[i] + k] for i in range(len(sizes))])
+    seeds = np.arange(3, 3 + len(sizes))
+    b = ya.kmeans_batched(v, sizes, k, niter=5, seeds=seeds, init_centroids=init,
+                          normalize=True)
+    compare(b, single(ya, v, sizes, k, 5, seeds, init=init, flags=0x100000 | 0x40000), sizes)
+    np.testing.assert_allclose(np.linalg.norm(b[0].astype(np.float64), axis=2), 1.0, rtol=1e-6)

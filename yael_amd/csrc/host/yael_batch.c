@@ -106,9 +106,10 @@ void yh_tune_put(const char *key, int val) {
 /* ---- the batched run ---- */
 
 typedef struct {
-  int ninst, d, ld, k, kpad, niter, win;
+  int ninst, d, ld, k, kpad, niter, win, normalize;
   long ntot;
   long *off; /* host offsets */
+  float *cent_user; /* KMEANS_INIT_USER: the initial centroids (device) */
   /* device */
   long *d_off;
   const float *v;
@@ -149,12 +150,16 @@ static int yh_batch_bind(yh_bstate *s, int P, yh_allocs *al, yk_batch **out) {
                          s->v,
                          s->vn, s->cent, s->cn, s->assign, s->dis, s->d_off, s->nitems,
                          s->item_inst, s->item_tile, nu, d_ui, d_uf, psum, pcnt, s->nassign,
-                         s->pbuf, s->rd, s->eps, s->flags, s->qerr2);
+                         s->pbuf, s->rd, s->eps, s->flags, s->qerr2, s->normalize);
 }
 
+/* the initial centroids: the caller's (KMEANS_INIT_USER) or the rows the
+   seeded random initialisation selected */
 static int yh_batch_init_state(yh_bstate *s) {
-  int rc = yk_gather_rows(s->v, s->sel, (long)s->ninst * s->kpad, s->d, s->cent);
-  return rc;
+  if (s->cent_user)
+    return yk_memcpy_d2d(s->cent, s->cent_user,
+                         sizeof(float) * (size_t)s->ninst * s->kpad * s->ld);
+  return yk_gather_rows(s->v, s->sel, (long)s->ninst * s->kpad, s->d, s->cent);
 }
 
 /* host precomputation for instances [i0, ninst) in steps of step.  First
@@ -164,7 +169,7 @@ static int yh_batch_init_state(yh_bstate *s) {
    YH_ECAP (draw, gaussian vector) pairs (kmeans.c:183-195, 296), advancing
    the chain */
 typedef struct {
-  int ninst, win, k, kpad, d;
+  int ninst, win, k, kpad, d, user_init;
   const int *n_inst;
   const long *off, *seed_used;
   unsigned *chain;
@@ -185,9 +190,11 @@ static void *yh_prep_worker(void *arg) {
     }
     for (int i = p->i0; i < p->ninst; i += p->step) {
       unsigned seed = (unsigned)p->seed_used[i];
-      yh_random_init(p->n_inst[i], p->k, tmp, (unsigned)rand_r(&seed));
-      for (int c = 0; c < p->kpad; c++)
-        p->sel[(long)i * p->kpad + c] = (int)(p->off[i] + tmp[c < p->k ? c : 0]);
+      if (!p->user_init) { /* user centroids draw nothing (kmeans.c:388-389) */
+        yh_random_init(p->n_inst[i], p->k, tmp, (unsigned)rand_r(&seed));
+        for (int c = 0; c < p->kpad; c++)
+          p->sel[(long)i * p->kpad + c] = (int)(p->off[i] + tmp[c < p->k ? c : 0]);
+      }
       p->chain[i] = (unsigned)rand_r(&seed);
     }
     free(tmp);
@@ -243,9 +250,12 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
                       int v_on_device, int flags, const long *seeds, float *cent_out,
                       float *dis_out, int *assign_out, int *nassign_out, float *ret_out,
                       int *iters_out, int use_graph, int *chunk_used) {
-  if (flags & (KMEANS_L1 | KMEANS_CHI2 | KMEANS_INIT_BERKELEY | KMEANS_INIT_USER |
-               KMEANS_NORMALIZE_CENTS))
-    return yk_set_error_c(YK_EINVAL, "batched k-means: only random init, L2, plain centroids");
+  if (flags & (KMEANS_L1 | KMEANS_CHI2 | KMEANS_INIT_BERKELEY))
+    return yk_set_error_c(YK_EINVAL,
+                          "batched k-means: L2 with random or user initialisation only");
+  const int user_init = (flags & KMEANS_INIT_USER) != 0;
+  if (user_init && !cent_out)
+    return yk_set_error_c(YK_EINVAL, "batched k-means: KMEANS_INIT_USER needs the centroids");
   if (ninst < 1 || k < 1 || d < 1 || niter < 0)
     return yk_set_error_c(YK_EINVAL, "batched k-means: bad sizes");
   const int niter_in = niter;
@@ -260,6 +270,7 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   s.k = k;
   s.kpad = (k + 31) & ~31;
   s.niter = niter;
+  s.normalize = (flags & KMEANS_NORMALIZE_CENTS) != 0;
   s.off = (long *)malloc(sizeof(long) * (ninst + 1));
   s.off[0] = 0;
   for (int i = 0; i < ninst; i++) {
@@ -291,7 +302,8 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   long *seed_used = (long *)malloc(sizeof(long) * ninst);
   unsigned *chain = (unsigned *)malloc(sizeof(unsigned) * ninst);
   int host_ok = sel && rd && eps && seed_used && chain;
-  yh_prep prep = {ninst, 0, k, kpad, d, n_inst, s.off, seed_used, chain, sel, rd, eps, 0, 1, 0};
+  yh_prep prep = {ninst,    0,         k,     kpad, d,   user_init, n_inst, s.off,
+                  seed_used, chain, sel, rd, eps, 0, 1, 0};
   if (host_ok) {
     for (int i = 0; i < ninst; i++) {
       long si = seeds ? seeds[i] : 0;
@@ -344,6 +356,24 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   if (rc == YK_OK) rc = yk_memcpy_h2d(s.d_off, s.off, sizeof(long) * (ninst + 1));
   if (rc == YK_OK && !v_on_device) rc = yk_upload_vectors(vdev, v, s.ntot, d);
   if (rc == YK_OK) rc = yk_memcpy_h2d(s.sel, sel, sizeof(int) * (size_t)ninst * kpad);
+  float *cent_init = NULL; /* KMEANS_INIT_USER: the caller's centroids, kept for the
+                              single-instance path (cent_out receives the results) */
+  if (rc == YK_OK && user_init) {
+    cent_init = (float *)malloc(sizeof(float) * (size_t)ninst * k * d);
+    if (cent_init) memcpy(cent_init, cent_out, sizeof(float) * (size_t)ninst * k * d);
+    /* padded rows; the pad rows past k repeat centroid 0 of their
+       instance, as the random initialisation's selection does */
+    float *hc = (float *)malloc(sizeof(float) * (size_t)ninst * kpad * d);
+    s.cent_user = (float *)yh_dmalloc(&al, sizeof(float) * (size_t)ninst * kpad * ld);
+    if (!hc || !s.cent_user || !cent_init)
+      rc = yk_set_error_c(YK_ENOMEM, "batched k-means: user centroids");
+    for (int i = 0; rc == YK_OK && i < ninst; i++)
+      for (int c = 0; c < kpad; c++)
+        memcpy(hc + ((size_t)i * kpad + c) * d, cent_out + ((size_t)i * k + (c < k ? c : 0)) * d,
+               sizeof(float) * d);
+    if (rc == YK_OK) rc = yk_upload_vectors(s.cent_user, hc, (long)ninst * kpad, d);
+    free(hc);
+  }
   if (rc == YK_OK) rc = yk_memcpy_h2d(s.rd, rd, sizeof(double) * (size_t)ninst * win * YH_ECAP);
   if (rc == YK_OK)
     rc = yk_memcpy_h2d(s.eps, eps, sizeof(float) * (size_t)ninst * win * YH_ECAP * d);
@@ -457,6 +487,7 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
       else
         memcpy(vi, v + s.off[i] * d, sizeof(float) * n * d);
       float *ci = (float *)malloc(sizeof(float) * (size_t)k * d);
+      if (ci && cent_init) memcpy(ci, cent_init + (size_t)i * k * d, sizeof(float) * (size_t)k * d);
       float r = 0;
       if (rc == YK_OK)
         rc = yh_kmeans(d, (int)n, k, niter_in, vi, 0, flags | KMEANS_QUIET, seed_used[i], 1, ci,
@@ -475,6 +506,7 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   free(hflags);
   free(hq);
   free(seed_used);
+  free(cent_init);
   yk_batch_destroy(bt);
   yk_sync();
   yh_dfree_all(&bal);

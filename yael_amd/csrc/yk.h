@@ -167,7 +167,8 @@ int yk_gather_rows(const float *src, const int *idx, long n, int d, float *dst);
  * counter, then w0).  The split draws rd / eps cover a window of win
  * iterations per instance (row it - w0 of instance i, w0 the window's first
  * iteration); the caller refills them and w0 between windows.  k * d tables
- * larger than the LDS are summed in cluster slices.  yk_batch_run queues
+ * larger than the LDS are summed in cluster slices; normalize scales every
+ * centroid to unit L2 norm after its update.  yk_batch_run queues
  * niter iterations, replaying one captured hipGraph when use_graph. */
 typedef struct yk_batch yk_batch;
 int yk_batch_assign(long nitems, const int *item_inst, const int *item_tile, const long *inst_off,
@@ -180,7 +181,7 @@ int yk_batch_create(yk_batch **out, int ninst, int d, int k, int kpad, int niter
                     const int *item_inst, const int *item_tile, long nuitems,
                     const int *uitem_inst, const int *uitem_first, double *psum, int *pcnt,
                     int *nassign, float *pbuf, const double *rd, const float *eps, int *flags,
-                    double *qerr2);
+                    double *qerr2, int normalize);
 int yk_batch_reset(yk_batch *bt);
 int yk_batch_run(yk_batch *bt, int niter, int use_graph);
 int yk_batch_destroy(yk_batch *bt);

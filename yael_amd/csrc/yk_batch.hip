@@ -37,6 +37,7 @@ struct BatchDev {
   int ninst, d, ld, k, kpad, niter, ecap, chunk;
   int kslice, nslice;  // the update's clusters per LDS table, and the tables per chunk
   int win;             // iterations per window of the split tables
+  int normalize;       // KMEANS_NORMALIZE_CENTS: centroids scaled to unit L2 norm
   long ntot;
   const float *v;   // ntot x ld
   const double *vn;  // ntot
@@ -249,6 +250,21 @@ __global__ __launch_bounds__(256) void yk_bupd_reduce_kernel(BatchDev b) {
     for (long u = u0; u < u1; u++) s += b.psum[(u * b.k + c) * (size_t)b.d + t];
     crow[t] = (float)(s * inv);
   }
+  if (b.normalize) {
+    // L2 normalisation with the single-instance update's arithmetic: float
+    // squares added in double in column order (each lane re-reads only its
+    // own columns; lane reads hand the squares over in order), c *= 1/nr
+    // (vector.c:1792-1797, 2016-2025, 2180-2197)
+    double nr = 0.0;
+    for (int j0 = 0; j0 < b.d; j0 += 64) {
+      const float x = j0 + lane < b.d ? crow[j0 + lane] : 0.0f;
+      const float sq = __fmul_rn(x, x);
+      const int m = b.d - j0 < 64 ? b.d - j0 : 64;
+      for (int t = 0; t < m; t++) nr = __dadd_rn(nr, (double)__shfl(sq, t));
+    }
+    const double sc = 1.0 / sqrt(nr);
+    for (int t = lane; t < b.d; t += 64) crow[t] = (float)((double)crow[t] * sc);
+  }
   if (lane == 0) b.nassign[(long)inst * b.k + c] = cnt;
 }
 
@@ -410,7 +426,7 @@ extern "C" int yk_batch_create(yk_batch **out, int ninst, int d, int k, int kpad
                                const int *item_tile, long nuitems, const int *uitem_inst,
                                const int *uitem_first, double *psum, int *pcnt, int *nassign,
                                float *pbuf, const double *rd, const float *eps, int *flags,
-                               double *qerr2) {
+                               double *qerr2, int normalize) {
   YK_TRY(yk_init());
   const int kslice = yk_bupd_kslice(k, d);
   if (kslice < 1)
@@ -421,6 +437,7 @@ extern "C" int yk_batch_create(yk_batch **out, int ninst, int d, int k, int kpad
   b.ninst = ninst; b.d = d; b.ld = yk_ld(d); b.k = k; b.kpad = kpad; b.niter = niter;
   b.ecap = ecap; b.chunk = chunk; b.ntot = ntot;
   b.kslice = kslice; b.nslice = (k + kslice - 1) / kslice; b.win = win;
+  b.normalize = normalize;
   b.v = v; b.vn = vn; b.cent = cent; b.cn = cn; b.assign = assign; b.dis = dis;
   b.inst_off = inst_off; b.nitems = nitems; b.item_inst = item_inst; b.item_tile = item_tile;
   b.nuitems = nuitems; b.uitem_inst = uitem_inst; b.uitem_first = uitem_first;

@@ -111,12 +111,14 @@ def kmeans(v, k, distance_type=2, nt=1, niter=30, seed=0, redo=1, verbose=True,
     return cent, qerr, dis, assign, nassign
 
 
-def kmeans_batched(v, sizes, k, niter=30, seeds=None, verbose=False):
+def kmeans_batched(v, sizes, k, niter=30, seeds=None, verbose=False, init_centroids=None,
+                   normalize=False):
     """ninst independent k-means instances on consecutive row ranges of v
     (instance i: the next sizes[i] rows), each with kmeans() semantics and its
     own seed (seed 0: lrand48(), in instance order).  One batched GPU run.
-    Returns (centroids (ninst, k, d), qerr (ninst,), dis, assign, nassign
-    (ninst, k))."""
+    init_centroids (ninst, k, d) selects KMEANS_INIT_USER; normalize
+    KMEANS_NORMALIZE_CENTS.  Returns (centroids (ninst, k, d), qerr (ninst,),
+    dis, assign, nassign (ninst, k))."""
     v = _rowf32(v, "v")
     sizes = np.ascontiguousarray(sizes, np.int32)
     ninst = len(sizes)
@@ -129,7 +131,14 @@ def kmeans_batched(v, sizes, k, niter=30, seeds=None, verbose=False):
     assign = np.empty(n, np.int32)
     nassign = np.empty((ninst, k), np.int32)
     ret = np.empty(ninst, np.float32)
-    flags = KMEANS_INIT_RANDOM | (0 if verbose else KMEANS_QUIET)
+    flags = 0 if verbose else KMEANS_QUIET
+    if init_centroids is not None:
+        cent[:] = np.asarray(init_centroids, np.float32).reshape(ninst, k, d)
+        flags |= KMEANS_INIT_USER
+    else:
+        flags |= KMEANS_INIT_RANDOM
+    if normalize:
+        flags |= KMEANS_NORMALIZE_CENTS
     rc = lib().yael_kmeans_batched(d, ninst, _ptr(sizes), k, niter, _ptr(v), flags, _ptr(seeds),
                                    _ptr(cent), _ptr(dis), _ptr(assign), _ptr(nassign), _ptr(ret))
     check(rc, "kmeans_batched")
