@@ -96,3 +96,36 @@ def test_learn_rejects_small_node(hk):
     v = np.concatenate([v, v + 100])
     with pytest.raises(YaelGPUError):
         hk.hkm_learn(v, 3, 4, niter=2)
+
+
+def _tree_file(path, levels, bf):
+    """A tree in the reference's file format (hkm.c:181-199): nlevel, bf, d,
+    then per level the element count and the centroids."""
+    d = levels[0].shape[1]
+    with open(path, "wb") as f:
+        f.write(np.array([len(levels), bf, d], np.int32).tobytes())
+        for c in levels:
+            c = np.ascontiguousarray(c, np.float32)
+            f.write(np.array([c.size], np.int32).tobytes() + c.tobytes())
+
+
+@pytest.mark.parametrize("nlevel,bf,d,n", [(3, 10, 40, 50000), (2, 32, 128, 200000),
+                                           (2, 45, 20, 30011)])
+def test_quantize_levels_batched_vs_oracle(hk, tmp_path, nlevel, bf, d, n):
+    """The level-at-a-time descent (points ordered by node, each node one
+    instance of the batched FP32 assign) on integer-valued trees and points
+    (every distance exact): the oracle's leaves bit for bit, with branching
+    factors on and off the 32-row block size."""
+    from oracle_bind import Oracle
+    rng = np.random.default_rng(nlevel * 1000 + bf)
+    levels = [np.rint(rng.uniform(0, 60, (bf ** (l + 1), d))).astype(np.float32)
+              for l in range(nlevel)]
+    v = np.rint(rng.uniform(0, 60, (n, d))).astype(np.float32)
+    path = tmp_path / "int.hkm"
+    _tree_file(path, levels, bf)
+    t = hk.hkm_read(path)
+    print("hkm_quantize nlevel=%d bf=%d d=%d n=%d" % (nlevel, bf, d, n), flush=True)
+    got = t.quantize(v)
+    want = Oracle().hkm_quantize(levels, bf, v)
+    assert np.array_equal(got, want)
+    assert got.min() >= 0 and got.max() < bf ** nlevel

@@ -29,6 +29,14 @@ struct YkGuard {
   YkGuard &operator=(const YkGuard &) = delete;
 };
 
+// The batched assign (yk_batch_assign) with an optional index table: point
+// j of the instances' concatenation is row qidx[j] of v / vn (outputs stay
+// in concatenation order).
+int yk_batch_assign_idx(long nitems, const int *item_inst, const int *item_tile,
+                        const long *inst_off, int d, int kc, int kpad, const float *v,
+                        const double *vn, const int *qidx, const float *cent, const float *cn,
+                        const int *done, int *assign, float *dis);
+
 // profiling slots
 enum { YK_PROF_FUSED = 0, YK_PROF_NORM = 1, YK_PROF_UPDATE = 2, YK_PROF_MERGE = 3,
        YK_PROF_RERANK = 4, YK_PROF_BATCH = 5, YK_PROF_BF16 = 6, YK_PROF_BF16_SKETCH = 7,

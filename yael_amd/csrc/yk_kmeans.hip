@@ -348,44 +348,111 @@ extern "C" int yk_kpp_update(const float *v, long n, int d, const double *vn, lo
   return YK_OK;
 }
 
-// hkm tree descent (hkm.c:131-150), one thread per point: at each level the
-// bf children of the point's node compared as nn() does (nn.c:404-440):
-// float(centroid float norm + point double norm), then -2 * dot added in
-// float, strict '<' from (-1, 1e30).  Levels are stacked in cent/cn (level
-// l: bf^(l+1) padded rows).  A point with no child below 1e30 at some level
-// gets -1 (the reference would index outside the level).
-__global__ __launch_bounds__(256) void yk_hkm_quantize_kernel(
-    const float *__restrict__ v, const double *__restrict__ vn, long n, int ld, int d, int nlevel,
-    int bf, const float *__restrict__ cent, const float *__restrict__ cn, int *__restrict__ idx) {
+// hkm tree descent (hkm_quantize, hkm.c:131-150), a level at a time: the
+// points are ordered by their current node (a counting sort on the device:
+// histogram, exclusive scan, scatter of point indices), every node is one
+// instance of the batched FP32 assign (yk_batch_assign_idx: k = 1 over the
+// node's bf children with the FP32 form's arithmetic, nn.c:383-446, points
+// read through the index table), and each point's node becomes
+// node * bf + child.  A point with no child below 1e30 gets -1 and leaves
+// the descent (the reference would index outside the level).
+
+__global__ void yk_hq_hist_kernel(const int *__restrict__ node, long n, int *__restrict__ cnt) {
   const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float *x = v + i * ld;
-  const double xn = vn[i];
-  long node = 0, lvl = 0, width = 1;  // node of the point, first row of the level, nodes
-  for (int l = 0; l < nlevel; l++) {
-    const long row0 = lvl + node * bf;
-    int best = -1;
-    float bv = 1e30f;
-    for (int c = 0; c < bf; c++) {
-      const float *cc = cent + (row0 + c) * ld;
-      float dot = 0.f;
-      for (int j = 0; j < d; j++) dot = dot + cc[j] * x[j];
-      float dist = (float)((double)cn[row0 + c] + xn);
-      dist = dist + -2.0f * dot;
-      if (dist < bv) {
-        bv = dist;
-        best = c;
-      }
-    }
-    if (best < 0) {
-      node = -1;
-      break;
-    }
-    node = node * bf + best;
-    lvl += width * bf;
-    width *= bf;
+  if (i < n && node[i] >= 0) atomicAdd(&cnt[node[i]], 1);
+}
+
+// one workgroup: off[p] (points before node p, p <= nn), cursor = off, and
+// the work items: itoff[p] = items before node p (ceil(count / 128) each)
+__global__ __launch_bounds__(1024) void yk_hq_scan_kernel(const int *__restrict__ cnt, long nn,
+                                                          long *__restrict__ off,
+                                                          long *__restrict__ cursor,
+                                                          long *__restrict__ itoff) {
+  __shared__ long sp[1024], si[1024];
+  const int t = threadIdx.x;
+  const long per = (nn + 1023) / 1024, lo = t * per, hi = min(nn, lo + per);
+  long a = 0, b = 0;
+  for (long p = lo; p < hi; p++) {
+    a += cnt[p];
+    b += (cnt[p] + 127) / 128;
   }
-  idx[i] = (int)node;
+  sp[t] = a;
+  si[t] = b;
+  __syncthreads();
+  if (t == 0) {  // exclusive prefix over the 1024 partials
+    long ra = 0, rb = 0;
+    for (int u = 0; u < 1024; u++) {
+      const long xa = sp[u], xb = si[u];
+      sp[u] = ra;
+      si[u] = rb;
+      ra += xa;
+      rb += xb;
+    }
+    off[nn] = ra;
+    itoff[nn] = rb;
+  }
+  __syncthreads();
+  a = sp[t];
+  b = si[t];
+  for (long p = lo; p < hi; p++) {
+    off[p] = a;
+    cursor[p] = a;
+    itoff[p] = b;
+    a += cnt[p];
+    b += (cnt[p] + 127) / 128;
+  }
+}
+
+__global__ void yk_hq_scatter_kernel(const int *__restrict__ node, long n, long *cursor,
+                                     int *__restrict__ perm) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n && node[i] >= 0) perm[atomicAdd((unsigned long long *)&cursor[node[i]], 1ull)] = (int)i;
+}
+
+// work item j -> (node, tile): the node by binary search over itoff
+__global__ void yk_hq_items_kernel(const long *__restrict__ itoff, long nn, long nitems,
+                                   int *__restrict__ item_inst, int *__restrict__ item_tile) {
+  const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (j >= nitems) return;
+  long lo = 0, hi = nn;  // itoff[lo] <= j < itoff[hi]
+  while (hi - lo > 1) {
+    const long mid = (lo + hi) >> 1;
+    if (itoff[mid] <= j)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  item_inst[j] = (int)lo;
+  item_tile[j] = (int)(j - itoff[lo]);
+}
+
+// the level's centroids with each node's bf children padded to kpad rows
+// (pad rows repeat child 0; the assign never selects them), and their norms
+__global__ void yk_hq_pad_kernel(const float *__restrict__ cl, const float *__restrict__ cnl,
+                                 long nn, int bf, int kpad, int ld, float *__restrict__ dst,
+                                 float *__restrict__ dstn) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long rows = nn * kpad, row = t / (ld / 4);
+  if (row >= rows) return;
+  const int c4 = (int)(t % (ld / 4));
+  const long p = row / kpad;
+  const int c = (int)(row % kpad);
+  const long src = p * bf + (c < bf ? c : 0);
+  ((float4 *)(dst + row * ld))[c4] = ((const float4 *)(cl + src * ld))[c4];
+  if (c4 == 0) dstn[row] = cnl[src];
+}
+
+__global__ void yk_hq_update_kernel(const int *__restrict__ perm, const int *__restrict__ child,
+                                    long m, int bf, int *__restrict__ node) {
+  const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  const int i = perm[j], c = child[j];
+  node[i] = c < 0 ? -1 : node[i] * bf + c;
+}
+
+__global__ void yk_hq_fill_kernel(int *p, long n, int v) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 extern "C" int yk_hkm_quantize(const float *v, const double *vn, long n, int d, int nlevel, int bf,
@@ -393,10 +460,66 @@ extern "C" int yk_hkm_quantize(const float *v, const double *vn, long n, int d, 
   YK_TRY(yk_init());
   if (n <= 0) return YK_OK;
   if (nlevel < 1 || bf < 1) return yk_set_error(YK_EINVAL, "yk_hkm_quantize: bad tree");
-  hipLaunchKernelGGL(yk_hkm_quantize_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     yk_stream(), v, vn, n, yk_ld(d), d, nlevel, bf, cent, cn, idx);
-  YK_HIP(hipGetLastError());
-  return YK_OK;
+  if (n > 0x7fffffffL) return yk_set_error(YK_EINVAL, "yk_hkm_quantize: n exceeds int32");
+  const int ld = yk_ld(d), kpad = (bf + 31) & ~31;
+  hipStream_t st = yk_stream();
+  // the widest level's parent nodes: bf^(nlevel-1)
+  long nn_max = 1;
+  for (int l = 1; l < nlevel; l++) nn_max *= bf;
+  void *mem = nullptr;
+  const size_t n_i = sizeof(int) * (size_t)n, nn_l = sizeof(long) * (size_t)(nn_max + 1);
+  const size_t items_max = (size_t)(n / 128 + nn_max + 1);
+  const size_t bytes = 4 * n_i + 3 * nn_l + sizeof(int) * (size_t)nn_max +
+                       2 * sizeof(int) * items_max +
+                       sizeof(float) * (size_t)nn_max * kpad * (ld + 1) + 16 * 256;
+  YK_TRY(yk_malloc(&mem, bytes));
+  char *q = (char *)mem;
+  auto carve = [&](size_t b) {
+    char *r = q;
+    q += (b + 255) & ~(size_t)255;
+    return (void *)r;
+  };
+  int *perm = (int *)carve(n_i), *child = (int *)carve(n_i);
+  float *dis = (float *)carve(n_i);
+  int *cnt = (int *)carve(sizeof(int) * (size_t)nn_max);
+  long *off = (long *)carve(nn_l), *cursor = (long *)carve(nn_l), *itoff = (long *)carve(nn_l);
+  int *item_inst = (int *)carve(sizeof(int) * items_max);
+  int *item_tile = (int *)carve(sizeof(int) * items_max);
+  float *pc = (float *)carve(sizeof(float) * (size_t)nn_max * kpad * ld);
+  float *pcn = (float *)carve(sizeof(float) * (size_t)nn_max * kpad);
+  int rc = YK_OK;
+  const unsigned gb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(yk_hq_fill_kernel, dim3(gb), dim3(256), 0, st, idx, n, 0);  // node 0
+  long lvl = 0, nn = 1;  // first centroid row of the level, its parent nodes
+  for (int l = 0; l < nlevel && rc == YK_OK; l++) {
+    hipLaunchKernelGGL(yk_hq_fill_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st,
+                       cnt, nn, 0);
+    hipLaunchKernelGGL(yk_hq_hist_kernel, dim3(gb), dim3(256), 0, st, idx, n, cnt);
+    hipLaunchKernelGGL(yk_hq_scan_kernel, dim3(1), dim3(1024), 0, st, cnt, nn, off, cursor,
+                       itoff);
+    hipLaunchKernelGGL(yk_hq_scatter_kernel, dim3(gb), dim3(256), 0, st, idx, n, cursor, perm);
+    long tot[2] = {0, 0};  // points still descending, work items
+    rc = yk_memcpy_d2h(&tot[0], off + nn, sizeof(long));
+    if (rc == YK_OK) rc = yk_memcpy_d2h(&tot[1], itoff + nn, sizeof(long));
+    if (rc != YK_OK || tot[0] == 0) break;
+    hipLaunchKernelGGL(yk_hq_items_kernel, dim3((unsigned)((tot[1] + 255) / 256)), dim3(256), 0,
+                       st, itoff, nn, tot[1], item_inst, item_tile);
+    const long prow = nn * kpad * (ld / 4);
+    hipLaunchKernelGGL(yk_hq_pad_kernel, dim3((unsigned)((prow + 255) / 256)), dim3(256), 0, st,
+                       cent + lvl * ld, cn + lvl, nn, bf, kpad, ld, pc, pcn);
+    YK_HIP(hipGetLastError());
+    rc = yk_batch_assign_idx(tot[1], item_inst, item_tile, off, d, bf, kpad, v, vn, perm, pc,
+                             pcn, nullptr, child, dis);
+    if (rc != YK_OK) break;
+    hipLaunchKernelGGL(yk_hq_update_kernel, dim3((unsigned)((tot[0] + 255) / 256)), dim3(256), 0,
+                       st, perm, child, tot[0], bf, idx);
+    YK_HIP(hipGetLastError());
+    lvl += nn * bf;
+    nn *= bf;
+  }
+  const int rs = yk_sync();
+  yk_free(mem);
+  return rc != YK_OK ? rc : rs;
 }
 
 __global__ void yk_gather_kernel(const float *__restrict__ src, const int *__restrict__ idx,

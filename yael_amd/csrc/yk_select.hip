@@ -90,6 +90,7 @@ struct ScanArgs {
   const long *inst_off;  // ninst + 1 point offsets
   int kc, kpad;
   const int *done;  // instances already finished (skipped)
+  const int *qidx;  // optional: point j of the concatenation is row qidx[j] of q / qn
 };
 
 // The hardware hands consecutive block ids to the 8 XCDs in turn; give each
@@ -189,8 +190,12 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
     tile = a0.item_tile[item];
     split = 0;
     const long off = a0.inst_off[inst];
-    a.q = a0.q + off * a0.ld;
-    a.qn = a0.qn + off;
+    if (a0.qidx) {
+      a.qidx = a0.qidx + off;  // rows through the index table, q and qn as given
+    } else {
+      a.q = a0.q + off * a0.ld;
+      a.qn = a0.qn + off;
+    }
     a.nq = a0.inst_off[inst + 1] - off;
     a.b = a0.b + (long)inst * a0.kpad * a0.ld;
     a.bn = a0.bn + (long)inst * a0.kpad;
@@ -208,7 +213,8 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
   const int ld = a.ld;
   const bool dot = (a.type == YK_DOT);
 
-  long qrow[NB];
+  long qrow[NB];  // the query's position (outputs)
+  long qsrc[NB];  // its row of q / qn
   bool qok[NB];
   double qnd[NB];
 #pragma unroll
@@ -216,7 +222,8 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
     long qi = qbase + j * 32 + r;
     qok[j] = qi < a.nq;
     qrow[j] = qok[j] ? qi : a.nq - 1;
-    qnd[j] = dot ? 0.0 : a.qn[qrow[j]];
+    qsrc[j] = a.qidx ? (long)a.qidx[qrow[j]] : qrow[j];
+    qnd[j] = dot ? 0.0 : a.qn[qsrc[j]];
   }
 
   // selection state
@@ -256,7 +263,7 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
     for (int j = 0; j < NB; j++)
 #pragma unroll
       for (int c = 0; c < NCH; c++) {
-        const float *p = a.q + qrow[j] * ld + c * 16 + h * 8;
+        const float *p = a.q + qsrc[j] * ld + c * 16 + h * 8;
         qreg[j][2 * c] = *(const float4 *)p;
         qreg[j][2 * c + 1] = *(const float4 *)(p + 4);
       }
@@ -350,7 +357,7 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
         float4 a1 = *(const float4 *)(bp + c * 16 + 4);
 #pragma unroll
         for (int j = 0; j < NB; j++) {
-          const float *qp = a.q + qrow[j] * ld + c * 16 + h * 8;
+          const float *qp = a.q + qsrc[j] * ld + c * 16 + h * 8;
           float4 b0 = *(const float4 *)qp, b1 = *(const float4 *)(qp + 4);
           acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc[j], 0, 0, 0);
           acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc[j], 0, 0, 0);
@@ -969,6 +976,14 @@ extern "C" int yk_batch_assign(long nitems, const int *item_inst, const int *ite
                                const long *inst_off, int d, int kc, int kpad, const float *v,
                                const double *vn, const float *cent, const float *cn,
                                const int *done, int *assign, float *dis) {
+  return yk_batch_assign_idx(nitems, item_inst, item_tile, inst_off, d, kc, kpad, v, vn, nullptr,
+                             cent, cn, done, assign, dis);
+}
+
+int yk_batch_assign_idx(long nitems, const int *item_inst, const int *item_tile,
+                        const long *inst_off, int d, int kc, int kpad, const float *v,
+                        const double *vn, const int *qidx, const float *cent, const float *cn,
+                        const int *done, int *assign, float *dis) {
   YK_TRY(yk_init());
   if (nitems <= 0) return YK_OK;
   ScanArgs a = {};
@@ -990,6 +1005,7 @@ extern "C" int yk_batch_assign(long nitems, const int *item_inst, const int *ite
   a.kc = kc;
   a.kpad = kpad;
   a.done = done;
+  a.qidx = qidx;
   a.total = (int)nitems;
   a.per_xcd = (a.total + 7) / 8;
   auto kern = yk_f32_pick<0, 1>(a.ld, false);
