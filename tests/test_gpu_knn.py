@@ -399,3 +399,39 @@ def test_recompute_exact_dists_vs_oracle(ya, oracle):
     assert idx[3, 0] < label0
     rc, _, _ = _recompute(ya, b[label0:label0 + nbw], q, label0, bad, idx, dis)
     assert rc != 0 and b"below" in ya.lib().yk_last_error()
+
+
+@pytest.mark.parametrize("typ,k", [(2, 1025), (2, 3000), (16, 1500)])
+def test_knn_large_k_vs_oracle(ya, oracle, typ, k):
+    """k beyond the on-chip selection (k > 1024; the reference allows any
+    k <= nb): distance rows, radix select, segmented sort.  Integer data:
+    every distance exact, so the oracle's lists bit for bit (exact ties in
+    its heap order aside)."""
+    rng = np.random.default_rng(k + typ)
+    b = mixture(rng, 30011, 64, 128, 20)
+    q = mixture(rng, 257, 64, 128, 20)
+    print("knn large k", typ, k, flush=True)
+    gi, gd = ya.knn(q, b, k, distance_type=typ)
+    oi, od = oracle.knn_full(typ, b, q, k, nt=16)
+    parity.check_knn(typ, q, b, k, gi, gd, oi, od, exact=True, label="large k%d" % k)
+
+
+def test_knn_large_k_weights_nan_and_k_equals_nb(ya, oracle):
+    """Weighted search, NaN base rows (never selected) with fewer finite
+    values than k (the tail padded with -1 and NaN bits 0xffffffff,
+    nn.c:515-518), and k = nb."""
+    rng = np.random.default_rng(9)
+    b = mixture(rng, 1500, 16, 30, 20)
+    b[::7, 3] = np.nan
+    q = mixture(rng, 40, 16, 30, 20)
+    w = rng.integers(1, 4, len(b)).astype(np.float32)
+    for k in (1400, 1500):
+        print("knn large k with NaN rows, k", k, flush=True)
+        gi, gd = ya.knn(q, b, k, weights=w)
+        oi, od = oracle.knn_full(2, b, q, k, w=w)
+        nfin = int(np.isfinite(b).all(1).sum())
+        assert np.all(gi[:, nfin:] == -1)
+        assert np.all(gd[:, nfin:].view(np.uint32) == 0xffffffff)
+        assert np.array_equal(np.sort(gd[:, :nfin], 1).view(np.int32),
+                              np.sort(od[:, :nfin], 1).view(np.int32))
+        parity.check_knn(2, q, b, k, gi, gd, oi, od, w=w, exact=True, label="nan k%d" % k)

@@ -37,6 +37,11 @@ int yk_batch_assign_idx(long nitems, const int *item_inst, const int *item_tile,
                         const double *vn, const int *qidx, const float *cent, const float *cn,
                         const int *done, int *assign, float *dis);
 
+// Exact kNN for k > YK_KMAX (yk_bigk.hip): distance rows per query batch,
+// per-query radix select and segmented sort; arguments as yk_knn_f32.
+int yk_knn_bigk(int type, long nq, long nb, int d, int k, const float *q, const double *qn,
+                const float *b, const float *bn, const float *w, int *assign, float *dis);
+
 // profiling slots
 enum { YK_PROF_FUSED = 0, YK_PROF_NORM = 1, YK_PROF_UPDATE = 2, YK_PROF_MERGE = 3,
        YK_PROF_RERANK = 4, YK_PROF_BATCH = 5, YK_PROF_BF16 = 6, YK_PROF_BF16_SKETCH = 7,

@@ -819,8 +819,8 @@ static int yk_knn_f32_check(int *type, long nq, long nb, int k) {
   if (*type == 12) *type = YK_L2;
   if (*type != YK_L2 && *type != YK_DOT)
     return yk_set_error(YK_EINVAL, "distance type %d has no GPU path (2, 12, 16 only)", *type);
-  if (k < 1 || k > nb || k > YK_KMAX)
-    return yk_set_error(YK_EINVAL, "k=%d out of range (nb=%ld, max %d)", k, nb, YK_KMAX);
+  if (k < 1 || k > nb)
+    return yk_set_error(YK_EINVAL, "k=%d out of range (nb=%ld)", k, nb);
   if (nb > 0x7fffffffL || nq > 0x7fffffffL) return yk_set_error(YK_EINVAL, "sizes exceed int32");
   return nq <= 0 ? 1 : YK_OK;
 }
@@ -830,6 +830,10 @@ static int yk_knn_f32_check(int *type, long nq, long nb, int k) {
 static int yk_knn_f32_run(int type, long nq, long nb, int d, int k, const float *q,
                           const double *qn, const float *b, const float *bn, const float *w,
                           int *assign, float *dis, YkArena *ar) {
+  if (k > YK_KMAX) {  // beyond the on-chip selection: distance rows + radix select
+    if (ar) return yk_set_error(YK_EINVAL, "k=%d > %d needs allocated scratch", k, YK_KMAX);
+    return yk_knn_bigk(type, nq, nb, d, k, q, qn, b, bn, w, assign, dis);
+  }
   ScanArgs a = {};
   a.q = q;
   a.qn = qn;
