@@ -96,6 +96,9 @@ PROTOS = {
     "yk_prof_enable": (I, [I]),
     "yk_prof_get": (I, [I, f64p, C.POINTER(L)]),
     "yk_prof_reset": (I, []),
+    "yk_tune_get": (I, [C.c_char_p, i32p]),
+    "yk_tune_put": (None, [C.c_char_p, I]),
+    "yk_tune_enabled": (I, []),
     "yk_device_info": (I, [C.c_char_p, I]),
     "yk_knn_plan": (I, [L, L, I, I, i32p, i32p, i32p]),
 }

@@ -59,50 +59,6 @@ static void yh_dfree_all(yh_allocs *a) {
   a->n = 0;
 }
 
-/* ---- autotuner cache: one "key value" pair per line ---- */
-
-static const char *yh_tune_path(char *buf, size_t len) {
-  const char *e = getenv("YAEL_MI355X_TUNE_CACHE");
-  if (e && *e) return e;
-  const char *home = getenv("HOME");
-  snprintf(buf, len, "%s/.cache/yael_mi355x_tune.txt", home ? home : "/tmp");
-  return buf;
-}
-
-int yh_tune_get(const char *key, int *val) {
-  char pbuf[512];
-  FILE *f = fopen(yh_tune_path(pbuf, sizeof pbuf), "r");
-  if (!f) return 0;
-  char k[256];
-  int v, found = 0;
-  while (fscanf(f, "%255s %d", k, &v) == 2)
-    if (!strcmp(k, key)) {
-      *val = v;
-      found = 1;
-    }
-  fclose(f);
-  return found;
-}
-
-void yh_tune_put(const char *key, int val) {
-  char pbuf[512];
-  const char *path = yh_tune_path(pbuf, sizeof pbuf);
-  char dir[512];
-  snprintf(dir, sizeof dir, "%s", path);
-  char *slash = strrchr(dir, '/');
-  if (slash) {
-    *slash = 0;
-    char cmd[600];
-    snprintf(cmd, sizeof cmd, "mkdir -p '%s' 2>/dev/null", dir);
-    if (system(cmd)) { /* best effort */
-    }
-  }
-  FILE *f = fopen(path, "a");
-  if (!f) return;
-  fprintf(f, "%s %d\n", key, val);
-  fclose(f);
-}
-
 /* ---- the batched run ---- */
 
 typedef struct {
@@ -388,7 +344,7 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
   int P = 0;
   char key[256];
   snprintf(key, sizeof key, "batch_chunk:d=%d:k=%d:ninst=%d:ntot=%ld", d, k, ninst, s.ntot);
-  if (rc == YK_OK && !yh_tune_get(key, &P)) {
+  if (rc == YK_OK && !yk_tune_get(key, &P)) {
     /* candidates: enough chunks to give every CU work (>= 512), partial
        sums within 8 GB; the smallest chunk is always a candidate */
     const int cand[7] = {256, 512, 1024, 2048, 4096, 8192, 16384};
@@ -416,7 +372,7 @@ int yh_kmeans_batched(int d, int ninst, const int *n_inst, int k, int niter, con
         P = cand[c];
       }
     }
-    if (rc == YK_OK) yh_tune_put(key, P);
+    if (rc == YK_OK) yk_tune_put(key, P);
   }
   if (chunk_used) *chunk_used = P;
 

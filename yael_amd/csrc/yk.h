@@ -186,6 +186,15 @@ int yk_batch_reset(yk_batch *bt);
 int yk_batch_run(yk_batch *bt, int niter, int use_graph);
 int yk_batch_destroy(yk_batch *bt);
 
+/* Autotuning cache of launch plans: "key value" lines in a text file
+ * ($YAEL_MI355X_TUNE_CACHE, default ~/.cache/yael_mi355x_tune.txt), held in
+ * memory after the first read.  The plans it records are schedules (work
+ * splits, sampling strides, update chunks): every plan gives the same bits.
+ * yk_tune_enabled: 0 when YAEL_MI355X_TUNE=0 (default plans, no timing). */
+int yk_tune_get(const char *key, int *val);
+void yk_tune_put(const char *key, int val);
+int yk_tune_enabled(void);
+
 /* Timing of the library's stream (HIP events recorded on it). */
 int yk_timer_start(void);
 int yk_timer_stop(float *ms);

@@ -37,6 +37,7 @@
 // with gamma = 2 (d' + 8) 2^-23 (d' = padded length, truncating adds allowed).
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 
 #include "yk.h"
@@ -1425,6 +1426,16 @@ struct BPlan {
   long brange, bstep;
 };
 
+// Launch-plan knobs the autotuner chooses among (every choice gives the
+// same bits): the work-item target of the base split and a scale of the
+// sketch's sampling stride; cfg = target index + 3 x stride index, 0 the
+// default plan.  The plan of the search in progress (calls are serialised
+// by the library lock; the nested escalation search keeps its caller's).
+static const long yk_bitem_targets[3] = {4096, 2048, 8192};
+static const int yk_bstride_scale[3][2] = {{1, 1}, {1, 2}, {2, 1}};
+#define YK_BNCFG 9
+static int g_bcfg = 0;
+
 // bbytes: bytes of one 32-row block of the base copy (32 rows x ldh BF16)
 static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, long max_split,
                       long bbytes) {
@@ -1433,7 +1444,8 @@ static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, 
   p.tiles = (int)((nq + qt - 1) / qt);
   p.bstep = 32 * sample;
   const long nblocks = (nb + p.bstep - 1) / p.bstep;
-  long want = (4096 + p.tiles - 1) / p.tiles;  // ~8 items per resident workgroup
+  const long target = yk_bitem_targets[g_bcfg % 3];
+  long want = (target + p.tiles - 1) / p.tiles;  // ~8 items per resident workgroup
   long maxs = (nblocks + 15) / 16;
   if (want > maxs) want = maxs;
   if (want < min_splits) want = min_splits;
@@ -1584,7 +1596,8 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   int *cnt = (int *)vcnt, *flag = cnt + nq + 32;
   // sketch: per-lane minima over a 1-in-stride sample, >= kp splits (same
   // reasoning as the FP32 form in yk_select.hip)
-  long stride = 600L / kp;
+  const int *sc = yk_bstride_scale[(g_bcfg / 3) % 3];
+  long stride = 600L * sc[0] / sc[1] / kp;
   if (stride > 8) stride = 8;
   if (stride < 1) stride = 1;
   const long nblocks = (a.nb + 31) / 32;
@@ -1623,12 +1636,92 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   return YK_OK;
 }
 
+static int yk_knn_bf16_run(int type, long nq, long nb, int d, int k, int kp, const float *q,
+                           const double *qn, const uint16_t *qh, const float *qstats,
+                           const float *b, const float *bn, const uint16_t *bh,
+                           const float *bstats_max, int *assign, float *dis, int *n_fallback);
+
+// a top-level search through yk_time_ms, with plan cfg
+struct YkBf16Call {
+  int type;
+  long nq, nb;
+  int d, k, kp;
+  const float *q;
+  const double *qn;
+  const uint16_t *qh;
+  const float *qstats, *b, *bn;
+  const uint16_t *bh;
+  const float *bstats_max;
+  int *assign;
+  float *dis;
+  int *n_fallback;
+  int cfg;
+};
+static int yk_bf16_call(void *p) {
+  const YkBf16Call &c = *(const YkBf16Call *)p;
+  g_bcfg = c.cfg;
+  const int rc = yk_knn_bf16_run(c.type, c.nq, c.nb, c.d, c.k, c.kp, c.q, c.qn, c.qh, c.qstats,
+                                 c.b, c.bn, c.bh, c.bstats_max, c.assign, c.dis, c.n_fallback);
+  g_bcfg = 0;
+  return rc;
+}
+
+static int yk_blg(long x) {
+  int l = 0;
+  while (x > 1) {
+    x >>= 1;
+    l++;
+  }
+  return l;
+}
+
 extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, const float *q,
                            const double *qn, const uint16_t *qh, const float *qstats,
                            const float *b, const float *bn, const uint16_t *bh,
                            const float *bstats_max, int *assign, float *dis, int *n_fallback) {
   YK_TRY(yk_init());
   YkGuard guard;  // process-wide scratch (g_bs, the escalation level) is shared
+  YkBf16Call call = {type, nq, nb, d, k, kp, q, qn, qh, qstats, b, bn, bh, bstats_max,
+                     assign, dis, n_fallback, 0};
+  // The plan: the tuning cache's choice for the shape class; on a miss (a
+  // top-level search of at least 2^32 flop, where one timing is
+  // meaningful) every candidate is timed once after an untimed run of the
+  // default and the fastest recorded; the outputs then hold the last
+  // candidate's results, the same bits as any plan's.
+  if (g_bs_level == 0 && yk_tune_enabled() && nq > 0 &&
+      (double)nq * nb * d >= (double)(1L << 31)) {
+    const int kpc = kp <= 8 ? 8 : kp <= YK_REG_R ? 16 : 1 << yk_blg(kp);
+    char key[128];
+    snprintf(key, sizeof key, "bf16:t%d:kp%d:ldh%d:q%d:b%d", type == 12 ? YK_L2 : type, kpc,
+             yk_bf16_ld(d, type == 12 ? YK_L2 : type), yk_blg(nq), yk_blg(nb));
+    int cfg = 0;
+    if (yk_tune_get(key, &cfg)) {
+      call.cfg = cfg >= 0 && cfg < YK_BNCFG ? cfg : 0;
+      return yk_bf16_call(&call);
+    }
+    const int ncand = kp > YK_REG_R ? YK_BNCFG : 3;  // the stride matters for k' > 16 only
+    YK_TRY(yk_bf16_call(&call));  // warm-up, default plan
+    float best = 1e30f;
+    for (int i = 0; i < ncand; i++) {
+      call.cfg = i;
+      float ms = 0;
+      YK_TRY(yk_time_ms(yk_bf16_call, &call, &ms));
+      if (ms < best) {
+        best = ms;
+        cfg = i;
+      }
+    }
+    yk_tune_put(key, cfg);
+    return YK_OK;
+  }
+  return yk_knn_bf16_run(type, nq, nb, d, k, kp, q, qn, qh, qstats, b, bn, bh, bstats_max,
+                         assign, dis, n_fallback);
+}
+
+static int yk_knn_bf16_run(int type, long nq, long nb, int d, int k, int kp, const float *q,
+                           const double *qn, const uint16_t *qh, const float *qstats,
+                           const float *b, const float *bn, const uint16_t *bh,
+                           const float *bstats_max, int *assign, float *dis, int *n_fallback) {
   if (type == 12) type = YK_L2;
   if (type != YK_L2 && type != YK_DOT)
     return yk_set_error(YK_EINVAL, "distance type %d has no GPU path (2, 12, 16 only)", type);

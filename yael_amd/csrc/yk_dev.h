@@ -42,6 +42,9 @@ int yk_batch_assign_idx(long nitems, const int *item_inst, const int *item_tile,
 int yk_knn_bigk(int type, long nq, long nb, int d, int k, const float *q, const double *qn,
                 const float *b, const float *bn, const float *w, int *assign, float *dis);
 
+// Milliseconds of run(arg) on the library stream (HIP events around it).
+int yk_time_ms(int (*run)(void *), void *arg, float *ms);
+
 // profiling slots
 enum { YK_PROF_FUSED = 0, YK_PROF_NORM = 1, YK_PROF_UPDATE = 2, YK_PROF_MERGE = 3,
        YK_PROF_RERANK = 4, YK_PROF_BATCH = 5, YK_PROF_BF16 = 6, YK_PROF_BF16_SKETCH = 7,

@@ -5,6 +5,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include "yk.h"
 #include "yk_dev.h"
@@ -463,4 +464,113 @@ int yk_resident_wgs(const void *fn, int block) {
   }
   pthread_mutex_unlock(&g_occ_mu);
   return res;
+}
+
+// ---------------------------------------------------------------------------
+// autotuning cache: "key value" lines in a text file
+// ($YAEL_MI355X_TUNE_CACHE, default ~/.cache/yael_mi355x_tune.txt), read
+// once per process into memory; later lines of a key win.  Every choice it
+// holds is a launch plan whose results are the same bits whichever plan
+// runs (a schedule, never an arithmetic), so a stale or foreign file can
+// only cost time.
+
+#define YK_TUNE_MAX 512
+static char g_tune_key[YK_TUNE_MAX][128];
+static int g_tune_val[YK_TUNE_MAX];
+static int g_tune_n = 0, g_tune_loaded = 0;
+static pthread_mutex_t g_tune_mu = PTHREAD_MUTEX_INITIALIZER;
+
+static const char *yk_tune_path(char *buf, size_t len) {
+  const char *e = getenv("YAEL_MI355X_TUNE_CACHE");
+  if (e && *e) return e;
+  const char *home = getenv("HOME");
+  snprintf(buf, len, "%s/.cache/yael_mi355x_tune.txt", home ? home : "/tmp");
+  return buf;
+}
+
+static void yk_tune_set_locked(const char *key, int val) {
+  for (int i = 0; i < g_tune_n; i++)
+    if (!strcmp(g_tune_key[i], key)) {
+      g_tune_val[i] = val;
+      return;
+    }
+  if (g_tune_n == YK_TUNE_MAX) return;
+  snprintf(g_tune_key[g_tune_n], sizeof g_tune_key[0], "%s", key);
+  g_tune_val[g_tune_n++] = val;
+}
+
+static void yk_tune_load_locked(void) {
+  if (g_tune_loaded) return;
+  g_tune_loaded = 1;
+  char pbuf[512];
+  FILE *f = fopen(yk_tune_path(pbuf, sizeof pbuf), "r");
+  if (!f) return;
+  char k[128];
+  int v;
+  while (fscanf(f, "%127s %d", k, &v) == 2) yk_tune_set_locked(k, v);
+  fclose(f);
+}
+
+extern "C" int yk_tune_get(const char *key, int *val) {
+  pthread_mutex_lock(&g_tune_mu);
+  yk_tune_load_locked();
+  int found = 0;
+  for (int i = 0; i < g_tune_n && !found; i++)
+    if (!strcmp(g_tune_key[i], key)) {
+      *val = g_tune_val[i];
+      found = 1;
+    }
+  pthread_mutex_unlock(&g_tune_mu);
+  return found;
+}
+
+extern "C" void yk_tune_put(const char *key, int val) {
+  pthread_mutex_lock(&g_tune_mu);
+  yk_tune_load_locked();
+  yk_tune_set_locked(key, val);
+  char pbuf[512];
+  const char *path = yk_tune_path(pbuf, sizeof pbuf);
+  char dir[512];
+  snprintf(dir, sizeof dir, "%s", path);
+  char *slash = strrchr(dir, '/');
+  if (slash && slash != dir) {
+    *slash = 0;
+    mkdir(dir, 0755);  // one level (~/.cache); best effort
+  }
+  FILE *f = fopen(path, "a");
+  if (f) {
+    fprintf(f, "%s %d\n", key, val);
+    fclose(f);
+  }
+  pthread_mutex_unlock(&g_tune_mu);
+}
+
+// 0 when YAEL_MI355X_TUNE=0: every search then runs its default plan
+extern "C" int yk_tune_enabled(void) {
+  const char *e = getenv("YAEL_MI355X_TUNE");
+  return !(e && e[0] == '0');
+}
+
+// Milliseconds of run() on the library stream, between two events.
+int yk_time_ms(int (*run)(void *), void *arg, float *ms) {
+  YK_TRY(yk_init());
+  hipEvent_t e0, e1;
+  YK_HIP(hipEventCreate(&e0));
+  hipError_t e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return yk_hip_fail(e, "hipEventCreate");
+  }
+  int rc = YK_OK;
+  if ((e = hipEventRecord(e0, g_stream)) != hipSuccess) rc = yk_hip_fail(e, "hipEventRecord");
+  if (rc == YK_OK) rc = run(arg);
+  if (rc == YK_OK && (e = hipEventRecord(e1, g_stream)) != hipSuccess)
+    rc = yk_hip_fail(e, "hipEventRecord");
+  if (rc == YK_OK && (e = hipEventSynchronize(e1)) != hipSuccess)
+    rc = yk_hip_fail(e, "hipEventSynchronize");
+  if (rc == YK_OK && (e = hipEventElapsedTime(ms, e0, e1)) != hipSuccess)
+    rc = yk_hip_fail(e, "hipEventElapsedTime");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }

@@ -611,9 +611,19 @@ struct YkPlan {
   long bstep;
 };
 
-// Split the base so that tiles * nsplit reaches ~4 workgroups per CU, with at
-// least min_rows rows per split and at most max_split splits.
-static YkPlan yk_plan(long nq, long nb, long sample, long min_splits, long max_split) {
+// Launch-plan knobs the autotuner chooses among (every choice gives the
+// same bits): the number of work items the base split aims at, and a
+// scale of the k > 16 sketch's sampling stride.  cfg = target index + 3 x
+// stride index; 0 is the default plan.
+static const long yk_item_targets[3] = {4096, 2048, 8192};
+static const int yk_stride_scale[3][2] = {{1, 1}, {1, 2}, {2, 1}};  // numerator, denominator
+#define YK_NCFG 9
+
+// Split the base so that tiles * nsplit reaches the item target (~8 per
+// resident workgroup by default), with at least min_rows rows per split and
+// at most max_split splits.
+static YkPlan yk_plan(long nq, long nb, long sample, long min_splits, long max_split,
+                      int cfg = 0) {
   YkPlan p;
   p.nblk = 1;
   p.waves = 4;
@@ -623,7 +633,8 @@ static YkPlan yk_plan(long nq, long nb, long sample, long min_splits, long max_s
   const long nblocks = (nb + p.bstep - 1) / p.bstep;  // processed blocks
   // ~8 work items per resident workgroup (512 of them: 256 CUs x 2) keeps
   // the tail of the persistent grid short
-  long want = (4096 + p.tiles - 1) / p.tiles;
+  const long target = yk_item_targets[cfg % 3];
+  long want = (target + p.tiles - 1) / p.tiles;
   long maxs = (nblocks + 15) / 16;  // >= 16 processed blocks per split
   if (want > maxs) want = maxs;
   if (want < min_splits) want = min_splits;
@@ -740,8 +751,9 @@ static int yk_sortsel(const uint64_t *src, long nq, int n, int k, int *oi, float
 // real keys per query.  The k-th smallest of 2k subset minima sits near
 // global rank 1.4 * stride * k, the expected candidate count; ~800 keeps
 // the filtered pass's append path rare and the list far from overflow.
-static long yk_sketch_stride(long nb, int k) {
-  long stride = 600L / k;
+static long yk_sketch_stride(long nb, int k, int cfg = 0) {
+  const int *sc = yk_stride_scale[(cfg / 3) % 3];
+  long stride = 600L * sc[0] / sc[1] / k;
   if (stride > 8) stride = 8;
   if (stride < 1) stride = 1;
   const long nblocks = (nb + 31) / 32;
@@ -755,7 +767,7 @@ static long yk_sketch_stride(long nb, int k) {
 // k > 16: sketch threshold, filtered scan, per-query sort (queries in batches
 // that bound the candidate memory).
 static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *dis,
-                          YkArena *ar) {
+                          YkArena *ar, int cfg) {
   const long QB = YK_LARGE_K_QB;
   const long nq_all = base.nq;
   void *vthr, *vcnt, *vcand, *vpart, *vflag;
@@ -768,7 +780,7 @@ static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *d
   int *cnt = (int *)vcnt;
   int *flag = cnt + qb + 32;
   vflag = flag;
-  const long stride = yk_sketch_stride(base.nb, k);
+  const long stride = yk_sketch_stride(base.nb, k, cfg);
   const long min_splits = k;
 
   for (long q0 = 0; q0 < nq_all; q0 += QB) {
@@ -778,14 +790,14 @@ static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *d
     a.qn = base.qn ? base.qn + q0 : nullptr;
     a.nq = nq;
     // (A) sketch: per-lane minima over a strided sample
-    YkPlan pa = yk_plan(nq, a.nb, stride, min_splits, min_splits > 256 ? min_splits : 256);
+    YkPlan pa = yk_plan(nq, a.nb, stride, min_splits, min_splits > 256 ? min_splits : 256, cfg);
     if (ar) ar->used = fixed_used;  // each pass reuses the region's partial-key piece
     YK_TRY(yk_scratch(ar, 3, sizeof(uint64_t) * nq * pa.nsplit * 2, &vpart));
     a.part = (uint64_t *)vpart;
     YK_TRY((yk_launch_scan<1, 1>(pa, a, qres)));
     YK_TRY(yk_sortsel(a.part, nq, pa.nsplit * 2, k, nullptr, nullptr, thr));
     // (B) filtered scan + per-query sort, retried for overflowing queries
-    YkPlan pb = yk_plan(nq, a.nb, 1, 1, 4096);
+    YkPlan pb = yk_plan(nq, a.nb, 1, 1, 4096, cfg);
     a.thr = thr;
     a.cnt = cnt;
     a.cand = (uint64_t *)vcand;
@@ -829,7 +841,7 @@ static int yk_knn_f32_check(int *type, long nq, long nb, int k) {
 // buffers).
 static int yk_knn_f32_run(int type, long nq, long nb, int d, int k, const float *q,
                           const double *qn, const float *b, const float *bn, const float *w,
-                          int *assign, float *dis, YkArena *ar) {
+                          int *assign, float *dis, YkArena *ar, int cfg = 0) {
   if (k > YK_KMAX) {  // beyond the on-chip selection: distance rows + radix select
     if (ar) return yk_set_error(YK_EINVAL, "k=%d > %d needs allocated scratch", k, YK_KMAX);
     return yk_knn_bigk(type, nq, nb, d, k, q, qn, b, bn, w, assign, dis);
@@ -848,7 +860,7 @@ static int yk_knn_f32_run(int type, long nq, long nb, int d, int k, const float 
   a.out_d = dis;
   const bool qres = a.ld <= 128;
   if (k == 1) {
-    YkPlan p = yk_plan(nq, nb, 1, 1, 4096);
+    YkPlan p = yk_plan(nq, nb, 1, 1, 4096, cfg);
     if (p.nsplit > 1) {
       void *buf;
       YK_TRY(yk_scratch(ar, 3, sizeof(uint64_t) * nq * p.nsplit, &buf));
@@ -865,14 +877,80 @@ static int yk_knn_f32_run(int type, long nq, long nb, int d, int k, const float 
     return YK_OK;
   }
   if (k <= YK_REG_R) {
-    YkPlan p = yk_plan(nq, nb, 1, 1, 256);
+    YkPlan p = yk_plan(nq, nb, 1, 1, 256, cfg);
     void *buf;
     YK_TRY(yk_scratch(ar, 3, sizeof(uint64_t) * nq * p.nsplit * 2 * YK_REG_R, &buf));
     a.part = (uint64_t *)buf;
     YK_TRY((yk_launch_scan<2, YK_REG_R>(p, a, qres)));
     return yk_sortsel(a.part, nq, p.nsplit * 2 * YK_REG_R, k, assign, dis, nullptr);
   }
-  return yk_knn_large_k(a, k, qres, assign, dis, ar);
+  return yk_knn_large_k(a, k, qres, assign, dis, ar, cfg);
+}
+
+// a search through yk_time_ms
+struct YkF32Call {
+  int type;
+  long nq, nb;
+  int d, k;
+  const float *q;
+  const double *qn;
+  const float *b, *bn, *w;
+  int *assign;
+  float *dis;
+  int cfg;
+};
+static int yk_f32_call(void *p) {
+  const YkF32Call &c = *(const YkF32Call *)p;
+  return yk_knn_f32_run(c.type, c.nq, c.nb, c.d, c.k, c.q, c.qn, c.b, c.bn, c.w, c.assign, c.dis,
+                        nullptr, c.cfg);
+}
+
+// log2 bucket of a size (shape classes of the tuning cache)
+static int yk_lg(long x) {
+  int l = 0;
+  while (x > 1) {
+    x >>= 1;
+    l++;
+  }
+  return l;
+}
+
+// The plan: the tuning cache's choice for the shape class; on a miss (and
+// a search of at least 2^30 flop, where one timing is meaningful), every
+// candidate plan is timed once after an untimed run of the default, the
+// fastest recorded.  The outputs then hold the last candidate's results --
+// the same bits as any other plan's.  *ran = 1 when the outputs are already
+// written.
+static int yk_f32_plan(const YkF32Call &call, int *cfg, int *ran) {
+  *cfg = 0;
+  *ran = 0;
+  if (call.k > YK_KMAX || !yk_tune_enabled()) return YK_OK;
+  if ((double)call.nq * call.nb * call.d < (double)(1 << 29)) return YK_OK;
+  const int kc = call.k == 1 ? 1 : call.k <= YK_REG_R ? 16 : 1 << yk_lg(call.k);
+  char key[128];
+  snprintf(key, sizeof key, "f32:t%d:k%d:ld%d:q%d:b%d", call.type, kc, yk_ld(call.d),
+           yk_lg(call.nq), yk_lg(call.nb));
+  if (yk_tune_get(key, cfg)) {
+    if (*cfg < 0 || *cfg >= YK_NCFG) *cfg = 0;
+    return YK_OK;
+  }
+  const int ncand = call.k > YK_REG_R ? YK_NCFG : 3;  // the stride matters for k > 16 only
+  YkF32Call c = call;
+  c.cfg = 0;
+  YK_TRY(yk_f32_call(&c));  // warm-up
+  float best = 1e30f;
+  for (int i = 0; i < ncand; i++) {
+    c.cfg = i;
+    float ms = 0;
+    YK_TRY(yk_time_ms(yk_f32_call, &c, &ms));
+    if (ms < best) {
+      best = ms;
+      *cfg = i;
+    }
+  }
+  yk_tune_put(key, *cfg);
+  *ran = 1;
+  return YK_OK;
 }
 
 extern "C" int yk_knn_f32(int type, long nq, long nb, int d, int k, const float *q,
@@ -882,7 +960,11 @@ extern "C" int yk_knn_f32(int type, long nq, long nb, int d, int k, const float 
   YkGuard guard;
   const int c = yk_knn_f32_check(&type, nq, nb, k);
   if (c != YK_OK) return c > 0 ? YK_OK : c;
-  return yk_knn_f32_run(type, nq, nb, d, k, q, qn, b, bn, w, assign, dis, nullptr);
+  const YkF32Call call = {type, nq, nb, d, k, q, qn, b, bn, w, assign, dis, 0};
+  int cfg = 0, ran = 0;
+  YK_TRY(yk_f32_plan(call, &cfg, &ran));
+  if (ran) return YK_OK;
+  return yk_knn_f32_run(type, nq, nb, d, k, q, qn, b, bn, w, assign, dis, nullptr, cfg);
 }
 
 // ---------------------------------------------------------------------------
