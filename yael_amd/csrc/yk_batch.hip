@@ -127,14 +127,17 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
       for (long p = p0 + pl; p < p1; p += 64) {
         int a[8];
         float4 x[8];
+// loads at an index clamped into the chunk, unconditional: the step's
+        // 16 loads issue back to back, with one wait before the first use
 #pragma unroll
         for (int s = 0; s < 8; s++) {
-          const long q = p + 8 * s;
-          a[s] = q < p1 ? slot(b.assign[q]) : -1;
-          x[s] = a[s] >= 0 ? *(const float4 *)(b.v + q * ld + j0) : make_float4(0.f, 0.f, 0.f, 0.f);
+          const long q = p + 8 * s < p1 ? p + 8 * s : p1 - 1;
+          a[s] = b.assign[q];
+          x[s] = *(const float4 *)(b.v + q * ld + j0);
         }
 #pragma unroll
         for (int s = 0; s < 8; s++) {
+          a[s] = p + 8 * s < p1 ? slot(a[s]) : -1;
           if (a[s] < 0) continue;
           double *row = lsum + a[s] * sd + j0;
           __hip_atomic_fetch_add(row, (double)x[s].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -159,11 +162,12 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
         float x[4];
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-          a[s] = slot(b.assign[p + s]);
-          x[s] = a[s] >= 0 ? b.v[(p + s) * ld + t] : 0.f;
+          a[s] = b.assign[p + s];
+          x[s] = b.v[(p + s) * ld + t];
         }
 #pragma unroll
         for (int s = 0; s < 4; s++) {
+          a[s] = slot(a[s]);
           if (a[s] < 0) continue;
           lsum[a[s] * d + t] += (double)x[s];
           if (t == 0) lcnt[a[s]]++;
@@ -185,12 +189,13 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
         float4 x[4];
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-          const long q = p + 64 * s;
-          a[s] = q < p1 ? slot(b.assign[q]) : -1;
-          x[s] = a[s] >= 0 ? *(const float4 *)(b.v + q * ld + j0) : make_float4(0.f, 0.f, 0.f, 0.f);
+          const long q = p + 64 * s < p1 ? p + 64 * s : p1 - 1;  // clamped, as above
+          a[s] = b.assign[q];
+          x[s] = *(const float4 *)(b.v + q * ld + j0);
         }
 #pragma unroll
         for (int s = 0; s < 4; s++) {
+          a[s] = p + 64 * s < p1 ? slot(a[s]) : -1;
           if (a[s] < 0) continue;
           double *row = lsum + a[s] * sd + j0;
           __hip_atomic_fetch_add(row, (double)x[s].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);

@@ -1436,6 +1436,18 @@ static const int yk_bstride_scale[3][2] = {{1, 1}, {1, 2}, {2, 1}};
 #define YK_BNCFG 9
 static int g_bcfg = 0;
 
+// the sketch's sampling stride for k' > 16 (same reasoning as the FP32
+// form in yk_select.hip), scaled by the current plan
+static long yk_bsketch_stride(long nb, int kp) {
+  const int *sc = yk_bstride_scale[(g_bcfg / 3) % 3];
+  long stride = 600L * sc[0] / sc[1] / kp;
+  if (stride > 8) stride = 8;
+  if (stride < 1) stride = 1;
+  const long nblocks = (nb + 31) / 32;
+  while (stride > 1 && nblocks / stride < 2L * kp) stride >>= 1;
+  return stride;
+}
+
 // bbytes: bytes of one 32-row block of the base copy (32 rows x ldh BF16)
 static BPlan yk_bplan(long nq, long nb, int nblk, long sample, long min_splits, long max_split,
                       long bbytes) {
@@ -1596,12 +1608,7 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   int *cnt = (int *)vcnt, *flag = cnt + nq + 32;
   // sketch: per-lane minima over a 1-in-stride sample, >= kp splits (same
   // reasoning as the FP32 form in yk_select.hip)
-  const int *sc = yk_bstride_scale[(g_bcfg / 3) % 3];
-  long stride = 600L * sc[0] / sc[1] / kp;
-  if (stride > 8) stride = 8;
-  if (stride < 1) stride = 1;
-  const long nblocks = (a.nb + 31) / 32;
-  while (stride > 1 && nblocks / stride < 2L * kp) stride >>= 1;
+  const long stride = yk_bsketch_stride(a.nb, kp);
   BPlan pa = yk_bplan(nq, a.nb, yk_bnb(1, a.ldh >> 4), stride, kp, kp > 256 ? kp : 256,
                       64L * a.ldh);
   // per query and split: 2 lane halves x 2 block parities of minima, and
@@ -1700,9 +1707,40 @@ extern "C" int yk_knn_bf16(int type, long nq, long nb, int d, int k, int kp, con
       return yk_bf16_call(&call);
     }
     const int ncand = kp > YK_REG_R ? YK_BNCFG : 3;  // the stride matters for k' > 16 only
+    // candidates whose plans come out the same as an earlier one's are not
+    // timed again
+    bool dup[YK_BNCFG] = {};
+    long sig[YK_BNCFG][3];
+    int distinct = 0;
+    const int ldh = yk_bf16_ld(d, type == 12 ? YK_L2 : type), nch = ldh >> 4;
+    const long nqb = kp <= YK_REG_R ? nq : (nq < 16384 ? nq : 16384);
+    for (int i = 0; i < ncand; i++) {
+      g_bcfg = i;
+      if (kp <= YK_REG_R) {
+        const int R = kp <= 8 ? 8 : YK_REG_R, nbq = yk_bnb(2, nch, R);
+        BPlan p = yk_bplan(nqb, nb, nbq, 1, 1, 256, 64L * ldh);
+        if (p.tiles * nbq >= 4096) p = yk_bplan(nqb, nb, nbq, 1, 1, 1, 64L * ldh);
+        sig[i][0] = p.nsplit, sig[i][1] = p.brange, sig[i][2] = 0;
+      } else {
+        const long st = yk_bsketch_stride(nb, kp);
+        const BPlan pa =
+            yk_bplan(nqb, nb, yk_bnb(1, nch), st, kp, kp > 256 ? kp : 256, 64L * ldh);
+        const BPlan pb = yk_bplan(nqb, nb, yk_bnb(3, nch), 1, 1, 4096, 64L * ldh);
+        sig[i][0] = st, sig[i][1] = pa.nsplit, sig[i][2] = pb.nsplit;
+      }
+      for (int j = 0; j < i && !dup[i]; j++)
+        dup[i] = !dup[j] && sig[j][0] == sig[i][0] && sig[j][1] == sig[i][1] &&
+                 sig[j][2] == sig[i][2];
+      distinct += !dup[i];
+    }
+    g_bcfg = 0;
+    if (distinct < 2)  // one plan: nothing to choose
+      return yk_knn_bf16_run(type, nq, nb, d, k, kp, q, qn, qh, qstats, b, bn, bh, bstats_max,
+                             assign, dis, n_fallback);
     YK_TRY(yk_bf16_call(&call));  // warm-up, default plan
     float best = 1e30f;
     for (int i = 0; i < ncand; i++) {
+      if (dup[i]) continue;
       call.cfg = i;
       float ms = 0;
       YK_TRY(yk_time_ms(yk_bf16_call, &call, &ms));

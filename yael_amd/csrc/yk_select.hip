@@ -935,11 +935,37 @@ static int yk_f32_plan(const YkF32Call &call, int *cfg, int *ran) {
     return YK_OK;
   }
   const int ncand = call.k > YK_REG_R ? YK_NCFG : 3;  // the stride matters for k > 16 only
+  // candidates whose plans come out the same as an earlier one's are not
+  // timed again (e.g. queries enough to fill the chip without any split)
+  long sig[YK_NCFG][3];
+  int distinct = 0;
+  bool dup[YK_NCFG] = {};
+  for (int i = 0; i < ncand; i++) {
+    const long nqb = call.nq < YK_LARGE_K_QB ? call.nq : YK_LARGE_K_QB;
+    if (call.k == 1) {
+      const YkPlan p = yk_plan(call.nq, call.nb, 1, 1, 4096, i);
+      sig[i][0] = p.nsplit, sig[i][1] = p.brange, sig[i][2] = 0;
+    } else if (call.k <= YK_REG_R) {
+      const YkPlan p = yk_plan(call.nq, call.nb, 1, 1, 256, i);
+      sig[i][0] = p.nsplit, sig[i][1] = p.brange, sig[i][2] = 0;
+    } else {
+      const long st = yk_sketch_stride(call.nb, call.k, i);
+      const YkPlan pa = yk_plan(nqb, call.nb, st, call.k, call.k > 256 ? call.k : 256, i);
+      const YkPlan pb = yk_plan(nqb, call.nb, 1, 1, 4096, i);
+      sig[i][0] = st, sig[i][1] = pa.nsplit, sig[i][2] = pb.nsplit;
+    }
+    for (int j = 0; j < i && !dup[i]; j++)
+      dup[i] = !dup[j] && sig[j][0] == sig[i][0] && sig[j][1] == sig[i][1] &&
+               sig[j][2] == sig[i][2];
+    distinct += !dup[i];
+  }
+  if (distinct < 2) return YK_OK;  // one plan: nothing to choose
   YkF32Call c = call;
   c.cfg = 0;
   YK_TRY(yk_f32_call(&c));  // warm-up
   float best = 1e30f;
   for (int i = 0; i < ncand; i++) {
+    if (dup[i]) continue;
     c.cfg = i;
     float ms = 0;
     YK_TRY(yk_time_ms(yk_f32_call, &c, &ms));
