@@ -52,6 +52,7 @@ typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 #define YK_CAND_CAP 4096
 #define YK_REG_R 16
 #define YK_CSTAGE 8  // MODE 3: keys staged in LDS per (lane, query tile)
+#define YK_RING 128   // MODE 3 (register-resident queries): parked tiles per wave
 
 // ---------------------------------------------------------------------------
 // conversion
@@ -361,7 +362,7 @@ __device__ __forceinline__ void yk_reg_insert16(uint64_t (&t)[R], uint64_t x) {
 
 // Per-wave selection state over NB 32-query column tiles: lane (r, h) owns
 // query r of each tile and, per 32-row block, rows 8g + 4h + e (g, e < 4).
-template <int NB, int MODE, int R, int CST = YK_CSTAGE>
+template <int NB, int MODE, int R, int CST = YK_CSTAGE, int RING = 0>
 struct BfSel {
   long qrow[NB];
   bool qok[NB];
@@ -376,12 +377,23 @@ struct BfSel {
   int ncs[NB];
   int lane;
   float *vs;  // MODES 2, 3: this lane's 16-value LDS slot
+  // MODE 3 with RING > 0: a tile whose lane minimum passes the bound is
+  // parked in the wave's ring of RING entries (its 16 values, query, first
+  // row and row limit: five float4 each); the ring is drained 64 entries at
+  // a time, one entry per lane, so the per-value work of the rare passing
+  // tiles runs on full waves instead of once per tile for one or two lanes.
+  // rhead, rtail: wave-uniform entry counters.
+  float4 *ring;
+  int rhead, rtail;
 
   __device__ __forceinline__ void init(const Bf16Args &a, long qbase, int r, int lane_,
-                                       uint64_t *cs_wave, float *vs_lane) {
+                                       uint64_t *cs_wave, float *vs_lane,
+                                       float4 *ring_wave = nullptr) {
     lane = lane_;
     cs = cs_wave;
     vs = vs_lane;
+    ring = ring_wave;
+    rhead = rtail = 0;
 #pragma unroll
     for (int j = 0; j < NB; j++) {
       ncs[j] = 0;
@@ -434,11 +446,16 @@ struct BfSel {
       if (MODE == 1) {
         // sketch: the lane's minimum value only, kept apart for the even and
         // odd blocks of the item (sub, a constant after inlining; see finish)
-        float m = bv[j][MODE == 1 ? sub : 0];
+        // (selects, not an index: a run-time index would place bv in
+        // scratch memory)
+        float m = sub ? bv[j][MODE == 1 ? 1 : 0] : bv[j][0];
 #pragma unroll
         for (int i = 0; i < 16; i++)
           if (FULL || 8 * (i >> 2) + (i & 3) < rl) m = fminf(m, acc[j][i]);
-        bv[j][MODE == 1 ? sub : 0] = m;
+        if (sub)
+          bv[j][MODE == 1 ? 1 : 0] = m;
+        else
+          bv[j][0] = m;
         continue;
       }
       float m = acc[j][0];
@@ -448,6 +465,29 @@ struct BfSel {
                                          ? INFINITY
                                          : yk_fkey_inv((uint32_t)(top[j][R - 1] >> 32)))
                                   : Tv[j];
+      if (MODE == 3 && RING > 0) {
+        // park the passing lanes' tiles (the row limit goes with them: the
+        // minimum may come from a pad row of the last block)
+        const bool pass = qok[j] && m <= lim;
+        const unsigned long long bal = __ballot(pass);
+        if (bal) {
+          if (pass) {
+            const int rank = __builtin_amdgcn_mbcnt_hi(
+                (unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+            float4 *e = ring + ((rhead + rank) & (RING - 1)) * 5;
+#pragma unroll
+            for (int q4 = 0; q4 < 4; q4++)
+              e[q4] = make_float4(acc[j][4 * q4], acc[j][4 * q4 + 1], acc[j][4 * q4 + 2],
+                                  acc[j][4 * q4 + 3]);
+            e[4] = make_float4(__int_as_float((int)qrow[j]),
+                               __int_as_float((int)bb + 4 * h), __int_as_float(FULL ? 64 : rl),
+                               0.f);
+          }
+          rhead += __popcll(bal);
+          if (rhead - rtail >= 64) drain(a, 64);
+        }
+        continue;
+      }
       if (!(m <= lim)) continue;
       // A lane that may take a value: a 16-bit mask of its passing values
       // (branch-free), its 16 values parked in its LDS slot, then one
@@ -486,6 +526,39 @@ struct BfSel {
     }
   }
 
+  // the n oldest ring entries, lane i taking entry rtail + i: every value
+  // within the bound and the row limit is keyed (value, row), and the keys
+  // not above the query's bound key are appended to its list
+  __device__ __forceinline__ void drain(const Bf16Args &a, int n) {
+    if (lane < n) {
+      const float4 *e = ring + ((rtail + lane) & (RING - 1)) * 5;
+      const float4 md = e[4];
+      const int q = __float_as_int(md.x), row0 = __float_as_int(md.y), rl = __float_as_int(md.z);
+      const uint64_t T = a.thr[q];
+      const float lim = T == YK_KEY_NONE ? INFINITY : yk_fkey_inv((uint32_t)(T >> 32));
+      unsigned msk = 0;
+#pragma unroll
+      for (int q4 = 0; q4 < 4; q4++) {
+        const float4 v = e[q4];
+        const float w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+          msk |= (w[t] <= lim && 8 * q4 + t < rl) ? (1u << (4 * q4 + t)) : 0u;
+      }
+      const float *ev = (const float *)e;
+      while (msk) {
+        const int i = __builtin_ctz(msk);
+        msk &= msk - 1;
+        const uint64_t key = yk_key(ev[i], row0 + 8 * (i >> 2) + (i & 3));
+        if (key <= T) {
+          const int b = atomicAdd(&a.cnt[q], 1);
+          if (b < YK_CAND_CAP) a.cand[(long)q * YK_CAND_CAP + b] = key;
+        }
+      }
+    }
+    rtail += n;
+  }
+
   // append n staged keys of tile j to the query's list (the count keeps
   // growing past YK_CAND_CAP: the select kernel detects the overflow)
   __device__ __forceinline__ void flush(const Bf16Args &a, int j, int n) {
@@ -499,7 +572,9 @@ struct BfSel {
   // (or, with kp_direct, the pair-merged kp smallest keys), MODE 3 the keys
   // still staged
   __device__ __forceinline__ void finish(const Bf16Args &a, long qbase, int split, int r, int h) {
-    if (MODE == 3) {
+    if (MODE == 3 && RING > 0) {
+      while (rtail < rhead) drain(a, min(64, rhead - rtail));
+    } else if (MODE == 3) {
 #pragma unroll
       for (int j = 0; j < NB; j++)
         if (ncs[j] > 0) flush(a, j, ncs[j]);
@@ -580,16 +655,20 @@ __device__ __forceinline__ void yk_scan_bf16_body(const Bf16Args &a) {
   const int ibeg = xcd * a.per_xcd, iend = min(a.total, ibeg + a.per_xcd);
   const int nch = a.ldh >> 4;
   const int ldh = a.ldh;
-  __shared__ uint64_t cstage[MODE == 3 ? 4 * NB * 64 * YK_CSTAGE : 1];
-  __shared__ float vstage[MODE >= 2 ? 4 * 64 * 16 : 1];
+  // MODE 3 parks passing tiles in a per-wave ring (YK_RING entries)
+  constexpr int RING = MODE == 3 ? YK_RING : 0;
+  __shared__ uint64_t cstage[MODE == 3 && RING == 0 ? 4 * NB * 64 * YK_CSTAGE : 1];
+  __shared__ float vstage[MODE == 2 || (MODE == 3 && RING == 0) ? 4 * 64 * 16 : 1];
+  __shared__ float4 rstage[RING > 0 ? 4 * RING * 5 : 1];
   for (int item = ibeg + local; item < iend; item += wpx) {
     const int tile = item % a.tiles_x, split = item / a.tiles_x;
     const long qbase = ((long)tile * nwave + wave) * (32 * NB);
     const long bbeg = (long)split * a.brange;
     const long bend = min(a.nb, bbeg + a.brange);
-    BfSel<NB, MODE, R> sel;
-    sel.init(a, qbase, r, lane, cstage + wave * (MODE == 3 ? NB * 64 * YK_CSTAGE : 0),
-             vstage + (MODE >= 2 ? (wave * 64 + lane) * 16 : 0));
+    BfSel<NB, MODE, R, YK_CSTAGE, RING> sel;
+    sel.init(a, qbase, r, lane, cstage + wave * (MODE == 3 && RING == 0 ? NB * 64 * YK_CSTAGE : 0),
+             vstage + (MODE == 2 || (MODE == 3 && RING == 0) ? (wave * 64 + lane) * 16 : 0),
+             rstage + wave * RING * 5);
     if (MODE == 3 && !__any(sel.active())) continue;
 
     bf16x8 qreg[NB][QRES ? NCH : 1];
