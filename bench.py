@@ -496,6 +496,23 @@ def bench_precision_d960(ya, dev, nb=1_000_000, nq=10_000, d=960, k=10, kp=64, s
     return r
 
 
+def tuned_plans():
+    """The launch plans the library's autotuner recorded (shape class -> plan
+    index), read from its tuning cache file."""
+    path = os.environ.get("YAEL_MI355X_TUNE_CACHE") or os.path.join(
+        os.path.expanduser("~"), ".cache", "yael_mi355x_tune.txt")
+    out = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                kv = line.split()
+                if len(kv) == 2:
+                    out[kv[0]] = int(kv[1])
+    except OSError:
+        pass
+    return out
+
+
 def bench_batched(ya, dev, ninst=256, total=2_000_000, d=128, k=64, niter=15, env=None):
     """hkm level-2 pattern: 256 independent k-means instances of Zipf sizes,
     one batched run (grouped launches, graph-replayed iteration); with env,
@@ -777,6 +794,7 @@ def main():
                      "sketch_kernel_ms_per_launch": sk_ms / max(sk_n, 1)},
         "accuracy": {"max_rel_err_vs_fp64": r["max_rel_err_f64"], "sorted": r["sorted_ok"],
                      "fallback_queries": r["nfb"]},
+        "tuned_plans": tuned_plans(),
     }
     print("# bf16 filtered pass: %.3f ms/launch -> %.1f TFLOP/s; sketch %.3f ms/launch"
           % (bf_ms_launch, ach16, sk_ms / max(sk_n, 1)), file=sys.stderr, flush=True)

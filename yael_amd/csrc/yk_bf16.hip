@@ -1511,14 +1511,14 @@ struct BPlan {
 // default plan.  The plan of the search in progress (calls are serialised
 // by the library lock; the nested escalation search keeps its caller's).
 static const long yk_bitem_targets[3] = {4096, 2048, 8192};
-static const int yk_bstride_scale[3][2] = {{1, 1}, {1, 2}, {2, 1}};
-#define YK_BNCFG 9
+static const int yk_bstride_scale[4][2] = {{1, 1}, {1, 2}, {2, 1}, {3, 2}};
+#define YK_BNCFG 12
 static int g_bcfg = 0;
 
 // the sketch's sampling stride for k' > 16 (same reasoning as the FP32
 // form in yk_select.hip), scaled by the current plan
 static long yk_bsketch_stride(long nb, int kp) {
-  const int *sc = yk_bstride_scale[(g_bcfg / 3) % 3];
+  const int *sc = yk_bstride_scale[(g_bcfg / 3) % 4];
   long stride = 600L * sc[0] / sc[1] / kp;
   if (stride > 8) stride = 8;
   if (stride < 1) stride = 1;
