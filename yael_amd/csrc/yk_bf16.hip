@@ -348,6 +348,7 @@ struct Bf16Args {
   uint64_t *cand;       // MODE 3: candidate lists; MODE 2 with kp_direct: [nq][kp]
   int kp_direct;        // MODE 2, nsplit == 1: the lane pair's kp <= R smallest keys
   const uint16_t *qf;   // long vectors: the queries fragment-major (as the base)
+  int *tick;            // per XCD, zero at launch: item tickets (nullptr: static order)
 };
 
 template <int R>
@@ -660,7 +661,21 @@ __device__ __forceinline__ void yk_scan_bf16_body(const Bf16Args &a) {
   __shared__ uint64_t cstage[MODE == 3 && RING == 0 ? 4 * NB * 64 * YK_CSTAGE : 1];
   __shared__ float vstage[MODE == 2 || (MODE == 3 && RING == 0) ? 4 * 64 * 16 : 1];
   __shared__ float4 rstage[RING > 0 ? 4 * RING * 5 : 1];
-  for (int item = ibeg + local; item < iend; item += wpx) {
+  // The first item of each workgroup is fixed, the later ones are drawn in
+  // order from the XCD's ticket counter: the workgroups of an XCD then keep
+  // to a window of consecutive items (items are split-major, so one or two
+  // base ranges), which its L2 holds while every query tile reads it --
+  // with a fixed stride between a workgroup's items they drift apart and
+  // the ranges are fetched from HBM several times over.
+  __shared__ int s_item;
+  auto next_item = [&](int it) {
+    if (!a.tick) return it + wpx;
+    __syncthreads();  // every wave has read the previous ticket
+    if (threadIdx.x == 0) s_item = ibeg + wpx + atomicAdd(&a.tick[xcd], 1);
+    __syncthreads();
+    return s_item;
+  };
+  for (int item = ibeg + local; item < iend; item = next_item(item)) {
     const int tile = item % a.tiles_x, split = item / a.tiles_x;
     const long qbase = ((long)tile * nwave + wave) * (32 * NB);
     const long bbeg = (long)split * a.brange;
@@ -1625,6 +1640,18 @@ static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
   const int nthr = (MODE != 2 && (a.ldh >> 4) > 16) ? 64 * YK_TW : 256;
   const int per_res = yk_resident_wgs((const void *)kern, nthr) / 8;
   const dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), blk(nthr);
+  // item tickets for the scans built on yk_scan_bf16_body (one small
+  // array, zeroed in stream order before each launch)
+  a.tick = nullptr;
+  if (MODE == 2 || (a.ldh >> 4) <= 16) {
+    static int *g_btick = nullptr;
+    if (!g_btick) {
+      const hipError_t e = hipMalloc((void **)&g_btick, 64 * sizeof(int));
+      if (e != hipSuccess) return yk_hip_fail(e, "hipMalloc(item tickets)");
+    }
+    hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), g_btick, 8L);
+    a.tick = g_btick;
+  }
   int tok = yk_prof_begin(MODE == 1 ? YK_PROF_BF16_SKETCH : YK_PROF_BF16);
   hipLaunchKernelGGL(kern, grid, blk, 0, yk_stream(), a);
   yk_prof_end(tok);
