@@ -1640,18 +1640,9 @@ static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
   const int nthr = (MODE != 2 && (a.ldh >> 4) > 16) ? 64 * YK_TW : 256;
   const int per_res = yk_resident_wgs((const void *)kern, nthr) / 8;
   const dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), blk(nthr);
-  // item tickets for the scans built on yk_scan_bf16_body (one small
-  // array, zeroed in stream order before each launch)
+  // item tickets for the scans built on yk_scan_bf16_body
   a.tick = nullptr;
-  if (MODE == 2 || (a.ldh >> 4) <= 16) {
-    static int *g_btick = nullptr;
-    if (!g_btick) {
-      const hipError_t e = hipMalloc((void **)&g_btick, 64 * sizeof(int));
-      if (e != hipSuccess) return yk_hip_fail(e, "hipMalloc(item tickets)");
-    }
-    hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), g_btick, 8L);
-    a.tick = g_btick;
-  }
+  if (MODE == 2 || (a.ldh >> 4) <= 16) YK_TRY(yk_item_tickets(&a.tick));
   int tok = yk_prof_begin(MODE == 1 ? YK_PROF_BF16_SKETCH : YK_PROF_BF16);
   hipLaunchKernelGGL(kern, grid, blk, 0, yk_stream(), a);
   yk_prof_end(tok);

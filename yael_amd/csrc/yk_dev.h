@@ -45,6 +45,10 @@ int yk_knn_bigk(int type, long nq, long nb, int d, int k, const float *q, const 
 // Milliseconds of run(arg) on the library stream (HIP events around it).
 int yk_time_ms(int (*run)(void *), void *arg, float *ms);
 
+// The persistent scans' per-XCD work-item ticket counters (8 ints used),
+// zeroed on the library stream before the caller's next launch.
+int yk_item_tickets(int **out);
+
 // profiling slots
 enum { YK_PROF_FUSED = 0, YK_PROF_NORM = 1, YK_PROF_UPDATE = 2, YK_PROF_MERGE = 3,
        YK_PROF_RERANK = 4, YK_PROF_BATCH = 5, YK_PROF_BF16 = 6, YK_PROF_BF16_SKETCH = 7,

@@ -48,6 +48,22 @@ extern "C" int yk_set_error_c(int code, const char *fmt, ...) {
 
 hipStream_t yk_stream(void) { return g_stream; }
 
+// Work-item tickets of the persistent scans: per-XCD counters, zeroed in
+// stream order before each launch that uses them.  A static device array,
+// so that taking them allocates nothing (its address is resolved once, at
+// initialisation).
+static __device__ __align__(256) int yk_tick_store[64];
+static int *g_tick = nullptr;
+
+__global__ void yk_tick_zero_kernel(int *t) { t[threadIdx.x] = 0; }
+
+int yk_item_tickets(int **out) {
+  hipLaunchKernelGGL(yk_tick_zero_kernel, dim3(1), dim3(64), 0, g_stream, g_tick);
+  YK_HIP(hipGetLastError());
+  *out = g_tick;
+  return YK_OK;
+}
+
 extern "C" int yk_init(void) {
   if (g_ready) return YK_OK;
   pthread_mutex_lock(&g_init_mu);
@@ -65,6 +81,9 @@ extern "C" int yk_init(void) {
     } else if ((e = hipEventCreate(&g_t0)) != hipSuccess ||
                (e = hipEventCreate(&g_t1)) != hipSuccess) {
       rc = yk_hip_fail(e, "hipEventCreate");
+    } else if ((e = hipGetSymbolAddress((void **)&g_tick, HIP_SYMBOL(yk_tick_store))) !=
+               hipSuccess) {
+      rc = yk_hip_fail(e, "hipGetSymbolAddress(item tickets)");
     } else {
       rc = yk_prof_setup();
       if (rc == YK_OK) g_ready = 1;

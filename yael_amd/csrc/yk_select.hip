@@ -91,6 +91,7 @@ struct ScanArgs {
   int kc, kpad;
   const int *done;  // instances already finished (skipped)
   const int *qidx;  // optional: point j of the concatenation is row qidx[j] of q / qn
+  int *tick;        // per XCD, zero at launch: item tickets (nullptr: static order)
 };
 
 // The hardware hands consecutive block ids to the 8 XCDs in turn; give each
@@ -181,7 +182,18 @@ __global__ __launch_bounds__(256) void yk_scan_f32(ScanArgs a0) {
   // YK_CAND_CAP so the select kernel sees an overflow)
   __shared__ uint64_t cstage[MODE == 3 ? 4 * NB * 64 * YK_CSTAGE : 1];
   uint64_t *const cs = cstage + (MODE == 3 ? wave * NB * 64 * YK_CSTAGE : 0);
-  for (int item = ibeg + local; item < iend; item += wpx) {
+  // a fixed first item, then tickets from the XCD's counter: the XCD's
+  // workgroups keep to a window of consecutive items (one or two base ranges
+  // in its L2) instead of drifting apart
+  __shared__ int s_item;
+  auto next_item = [&](int it) {
+    if (!a0.tick) return it + wpx;
+    __syncthreads();  // every wave has read the previous ticket
+    if (threadIdx.x == 0) s_item = ibeg + wpx + atomicAdd(&a0.tick[xcd], 1);
+    __syncthreads();
+    return s_item;
+  };
+  for (int item = ibeg + local; item < iend; item = next_item(item)) {
   ScanArgs a = a0;
   int tile, split;
   if (a0.item_inst) {
@@ -725,6 +737,7 @@ static int yk_launch_scan(const YkPlan &p, ScanArgs a, bool qres) {
   a.per_xcd = (a.total + 7) / 8;
   (void)qres;
   auto kern = yk_f32_pick<MODE, R>(a.ld, a.w != nullptr);
+  YK_TRY(yk_item_tickets(&a.tick));
   const int per_xcd_res = yk_resident_wgs((const void *)kern, 64 * p.waves) / 8;
   dim3 grid(8 * (a.per_xcd < per_xcd_res ? a.per_xcd : per_xcd_res)), block(64 * p.waves);
   int tok = yk_prof_begin(MODE == 1 ? YK_PROF_F32_SKETCH : YK_PROF_FUSED);
@@ -1121,6 +1134,7 @@ int yk_batch_assign_idx(long nitems, const int *item_inst, const int *item_tile,
   a.total = (int)nitems;
   a.per_xcd = (a.total + 7) / 8;
   auto kern = yk_f32_pick<0, 1>(a.ld, false);
+  YK_TRY(yk_item_tickets(&a.tick));
   const int per_res = yk_resident_wgs((const void *)kern, 256) / 8;
   dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), block(256);
   hipLaunchKernelGGL(kern, grid, block, 0, yk_stream(), a);
