@@ -475,8 +475,14 @@ static int yk_batch_enqueue(yk_batch *bt) {
   BatchDev &b = bt->dev;
   hipStream_t st = yk_stream();
   YK_TRY(yk_norms(b.cent, (long)b.ninst * b.kpad, b.d, b.cn, nullptr));
-  YK_TRY(yk_batch_assign(b.nitems, b.item_inst, b.item_tile, b.inst_off, b.d, b.k, b.kpad, b.v,
-                         b.vn, b.cent, b.cn, b.done, b.assign, b.dis));
+  // small centroid sets: centroids in registers, points streamed per chunk
+  const int rc = yk_batch_assign_res(b.ninst, b.d, b.k, b.kpad, b.chunk, b.v, b.vn, b.cent, b.cn,
+                                     b.inst_off, b.nuitems, b.uitem_inst, b.uitem_first, b.done,
+                                     b.assign, b.dis);
+  if (rc < 0) return rc;
+  if (rc == 1)
+    YK_TRY(yk_batch_assign(b.nitems, b.item_inst, b.item_tile, b.inst_off, b.d, b.k, b.kpad, b.v,
+                           b.vn, b.cent, b.cn, b.done, b.assign, b.dis));
   // d >= 64 a multiple of 32: a wave per 32-dimension group (at most 4);
   // d < 64: a wave per float4 column (at most 4), or one wave when d is not
   // a multiple of 4; else one thread per dimension (at most 256)

@@ -37,6 +37,14 @@ int yk_batch_assign_idx(long nitems, const int *item_inst, const int *item_tile,
                         const double *vn, const int *qidx, const float *cent, const float *cn,
                         const int *done, int *assign, float *dis);
 
+// The batched assign over the update chunks (chunk points of one instance
+// per item, uitem_* as in yk_batch_create) with the centroids held in
+// registers; 1 when the shape (d, kc) has no such kernel.
+int yk_batch_assign_res(int ninst, int d, int kc, int kpad, int chunk, const float *v,
+                        const double *vn, const float *cent, const float *cn,
+                        const long *inst_off, long nuitems, const int *uitem_inst,
+                        const int *uitem_first, const int *done, int *assign, float *dis);
+
 // Exact kNN for k > YK_KMAX (yk_bigk.hip): distance rows per query batch,
 // per-query radix select and segmented sort; arguments as yk_knn_f32.
 int yk_knn_bigk(int type, long nq, long nb, int d, int k, const float *q, const double *qn,

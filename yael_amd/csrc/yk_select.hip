@@ -1097,6 +1097,225 @@ extern "C" int yk_knn_f32_noalloc(int type, long nq, long nb, int d, int k, cons
 // ---------------------------------------------------------------------------
 // batched form: the assign step of many k-means instances in one launch
 
+// Small centroid sets (NBLK 32-row blocks, d <= 16 NCH): one workgroup per
+// update chunk (up to `chunk` consecutive points of one instance) holds the
+// instance's centroid fragments in registers and streams the chunk's points
+// through in 32-point blocks, wave w taking blocks w, w + W, ...; a block's
+// fragments are reloaded chunk by chunk right after the last centroid
+// block's MFMAs have read them, so every load has the next block's MFMAs to
+// arrive in.  (The general scan holds the points and streams the
+// centroids, which for k <= 64 leaves each 32-point tile two centroid blocks
+// of work against the full latency of loading its points.)  Same MFMA
+// operands, same order and the same epilogue arithmetic as the general
+// scan's MODE 0: the outputs are its bits.
+struct BassignArgs {
+  const float *v;
+  const double *vn;
+  const float *cent, *cn;
+  int ld, kc, kpad, chunk;
+  const long *inst_off;
+  const int *uitem_inst, *uitem_first;
+  int nuitems, per_xcd;
+  const int *done;
+  int *assign;
+  float *dis;
+  int *tick;
+};
+
+template <int NCH, int NBLK>
+__global__ __launch_bounds__(256) void yk_bassign_res_kernel(BassignArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int r = lane & 31, h = lane >> 5;
+  const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, wpx = gridDim.x >> 3;
+  const int ibeg = xcd * a.per_xcd, iend = min(a.nuitems, ibeg + a.per_xcd);
+  const int ld = a.ld;
+  __shared__ int s_item;
+  auto next_item = [&](int it) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = ibeg + wpx + atomicAdd(&a.tick[xcd], 1);
+    __syncthreads();
+    return s_item;
+  };
+  for (int u = ibeg + local; u < iend; u = next_item(u)) {
+    const int inst = a.uitem_inst[u];
+    if (a.done[inst]) continue;
+    const long off = a.inst_off[inst], n = a.inst_off[inst + 1] - off;
+    const long p0 = (long)(u - a.uitem_first[inst]) * a.chunk;
+    const long p1 = p0 + a.chunk < n ? p0 + a.chunk : n;
+    const float *cb0 = a.cent + (long)inst * a.kpad * ld;
+    const float *cnb = a.cn + (long)inst * a.kpad;
+    // centroid fragments (rows past kc clamped to the last, masked below)
+    // and norms (rows 32 cb + 8 g + 4 h .. + 3)
+    float4 cf[NBLK][2 * NCH];
+    float nn[NBLK][16];
+#pragma unroll
+    for (int cb = 0; cb < NBLK; cb++) {
+      const float *row = cb0 + (long)min(32 * cb + r, a.kc - 1) * ld + h * 8;
+#pragma unroll
+      for (int c = 0; c < NCH; c++) {
+        cf[cb][2 * c] = *(const float4 *)(row + 16 * c);
+        cf[cb][2 * c + 1] = *(const float4 *)(row + 16 * c + 4);
+      }
+#pragma unroll
+      for (int g = 0; g < 4; g++) {
+        const float4 n4 = *(const float4 *)(cnb + 32 * cb + 8 * g + 4 * h);
+        nn[cb][4 * g] = n4.x; nn[cb][4 * g + 1] = n4.y;
+        nn[cb][4 * g + 2] = n4.z; nn[cb][4 * g + 3] = n4.w;
+      }
+    }
+    const long step = 32L * nwave;
+    long pb = p0 + 32L * wave;
+    if (pb >= p1) continue;
+    // the block's point fragments and norm (rows past the chunk clamped to
+    // its last point, not written)
+    float4 bq[2 * NCH];
+    auto loadq = [&](long blk, int c) {
+      const float *pr = a.v + (off + min(blk + r, p1 - 1)) * ld + h * 8 + 16 * c;
+      bq[2 * c] = *(const float4 *)pr;
+      bq[2 * c + 1] = *(const float4 *)(pr + 4);
+    };
+#pragma unroll
+    for (int c = 0; c < NCH; c++) loadq(pb, c);
+    double qd = a.vn[off + min(pb + r, p1 - 1)];
+    for (; pb < p1; pb += step) {
+      const long pn = pb + step < p1 ? pb + step : pb;  // past the chunk: reload this block
+      const double qdn = a.vn[off + min(pn + r, p1 - 1)];
+      float bv = 1e30f;
+      int bi = -1;
+#pragma unroll
+      for (int cb = 0; cb < NBLK; cb++) {
+        f32x16 acc;
+#pragma unroll
+        for (int e = 0; e < 16; e++) acc[e] = 0.f;
+#pragma unroll
+        for (int c = 0; c < NCH; c++) {
+          const float4 a0 = cf[cb][2 * c], a1 = cf[cb][2 * c + 1];
+          const float4 b0 = bq[2 * c], b1 = bq[2 * c + 1];
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc, 0, 0, 0);
+          if (cb == NBLK - 1) {
+            __builtin_amdgcn_sched_barrier(0);
+            loadq(pn, c);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+        if (32 * cb + 32 <= a.kc) {
+          yk_best1_block<false>(acc, nn[cb], qd, 32L * cb, h, false, bv, bi);
+        } else {
+#pragma unroll
+          for (int i = 0; i < 16; i++) {
+            const int bidx = 32 * cb + 8 * (i >> 2) + 4 * h + (i & 3);
+            const float v = (float)(qd + (double)nn[cb][i]) - 2.0f * acc[i];
+            if (bidx < a.kc && v < bv) {
+              bv = v;
+              bi = bidx;
+            }
+          }
+        }
+      }
+      const float ov = __shfl_xor(bv, 32);
+      const int oi = __shfl_xor(bi, 32);
+      if (ov < bv || (ov == bv && oi < bi)) {
+        bv = ov;
+        bi = oi;
+      }
+      if (h == 0 && pb + r < p1) {
+        a.dis[off + pb + r] = bv;
+        a.assign[off + pb + r] = bi;
+      }
+      qd = qdn;
+    }
+  }
+}
+
+template <int NCH, int NBLK>
+static void (*yk_bassign_pick2())(BassignArgs) {
+  return yk_bassign_res_kernel<NCH, NBLK>;
+}
+
+// The kernel for (nch, nblk), or null when the centroid set does not fit
+// (centroid fragments of nblk x nch x 8 registers, at most 128).
+static void (*yk_bassign_pick(int nch, int nblk))(BassignArgs) {
+  if (nblk == 1) {
+    switch (nch) {
+      case 1: return yk_bassign_pick2<1, 1>();
+      case 2: return yk_bassign_pick2<2, 1>();
+      case 3: return yk_bassign_pick2<3, 1>();
+      case 4: return yk_bassign_pick2<4, 1>();
+      case 5: return yk_bassign_pick2<5, 1>();
+      case 6: return yk_bassign_pick2<6, 1>();
+      case 7: return yk_bassign_pick2<7, 1>();
+      case 8: return yk_bassign_pick2<8, 1>();
+    }
+  } else if (nblk == 2) {
+    switch (nch) {
+      case 1: return yk_bassign_pick2<1, 2>();
+      case 2: return yk_bassign_pick2<2, 2>();
+      case 3: return yk_bassign_pick2<3, 2>();
+      case 4: return yk_bassign_pick2<4, 2>();
+      case 5: return yk_bassign_pick2<5, 2>();
+      case 6: return yk_bassign_pick2<6, 2>();
+      case 7: return yk_bassign_pick2<7, 2>();
+      case 8: return yk_bassign_pick2<8, 2>();
+    }
+  } else if (nblk <= 4 && nch <= 4) {
+    switch (nch * 8 + nblk) {
+      case 1 * 8 + 3: return yk_bassign_pick2<1, 3>();
+      case 2 * 8 + 3: return yk_bassign_pick2<2, 3>();
+      case 3 * 8 + 3: return yk_bassign_pick2<3, 3>();
+      case 4 * 8 + 3: return yk_bassign_pick2<4, 3>();
+      case 1 * 8 + 4: return yk_bassign_pick2<1, 4>();
+      case 2 * 8 + 4: return yk_bassign_pick2<2, 4>();
+      case 3 * 8 + 4: return yk_bassign_pick2<3, 4>();
+      case 4 * 8 + 4: return yk_bassign_pick2<4, 4>();
+    }
+  }
+  return nullptr;
+}
+
+// The batched assign over the update chunks with the centroids
+// register-resident; returns 1 when the shape has no such kernel (the
+// caller then uses yk_batch_assign).
+int yk_batch_assign_res(int ninst, int d, int kc, int kpad, int chunk, const float *v,
+                        const double *vn, const float *cent, const float *cn,
+                        const long *inst_off, long nuitems, const int *uitem_inst,
+                        const int *uitem_first, const int *done, int *assign, float *dis) {
+  YK_TRY(yk_init());
+  (void)ninst;
+  const int ld = yk_ld(d);
+  auto kern = yk_bassign_pick(ld >> 4, (kc + 31) / 32);
+  if (!kern || nuitems <= 0 || nuitems > 0x7fffffffL) return 1;
+  BassignArgs a = {};
+  a.v = v;
+  a.vn = vn;
+  a.cent = cent;
+  a.cn = cn;
+  a.ld = ld;
+  a.kc = kc;
+  a.kpad = kpad;
+  a.chunk = chunk;
+  a.inst_off = inst_off;
+  a.uitem_inst = uitem_inst;
+  a.uitem_first = uitem_first;
+  a.nuitems = (int)nuitems;
+  a.per_xcd = (a.nuitems + 7) / 8;
+  a.done = done;
+  a.assign = assign;
+  a.dis = dis;
+  YK_TRY(yk_item_tickets(&a.tick));
+  const int per_res = yk_resident_wgs((const void *)kern, 256) / 8;
+  const dim3 grid(8 * (a.per_xcd < per_res ? a.per_xcd : per_res)), block(256);
+  hipLaunchKernelGGL(kern, grid, block, 0, yk_stream(), a);
+  YK_HIP(hipGetLastError());
+  return YK_OK;
+}
+
 extern "C" int yk_batch_assign(long nitems, const int *item_inst, const int *item_tile,
                                const long *inst_off, int d, int kc, int kpad, const float *v,
                                const double *vn, const float *cent, const float *cn,
