@@ -121,31 +121,46 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
     if (lane == 0) wobj[wave] = obj;
   }
   if (d >= 64 && (d & 31) == 0) {
+    // Within each 32-dimension group the table's columns are permuted,
+    // dimension 4 c + e at column 8 e + c: the lanes of one ds_add_f64 (the
+    // e-th component of their float4) then hit 8 consecutive doubles of a
+    // row, not every fourth pair of banks.  Each step's loads are issued a
+    // step ahead (at indices clamped into the chunk: always in bounds), so
+    // the adds of one step overlap the next step's loads.
     const int nwave = nthr >> 6, pl = lane >> 3, cl = lane & 7;
     for (int g = wave; g < (d >> 5); g += nwave) {
       const int j0 = g * 32 + cl * 4;
-      for (long p = p0 + pl; p < p1; p += 64) {
-        int a[8];
-        float4 x[8];
-// loads at an index clamped into the chunk, unconditional: the step's
-        // 16 loads issue back to back, with one wait before the first use
+      int an[8];
+      float4 xn[8];
+      auto load = [&](long p) {
 #pragma unroll
         for (int s = 0; s < 8; s++) {
           const long q = p + 8 * s < p1 ? p + 8 * s : p1 - 1;
-          a[s] = b.assign[q];
-          x[s] = *(const float4 *)(b.v + q * ld + j0);
+          an[s] = b.assign[q];
+          xn[s] = *(const float4 *)(b.v + q * ld + j0);
         }
+      };
+      if (p0 + pl < p1) load(p0 + pl);
+      for (long p = p0 + pl; p < p1; p += 64) {
+        int a[8];
+        float4 x[8];
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+          a[s] = an[s];
+          x[s] = xn[s];
+        }
+        load(p + 64 < p1 ? p + 64 : p);
 #pragma unroll
         for (int s = 0; s < 8; s++) {
           a[s] = p + 8 * s < p1 ? slot(a[s]) : -1;
           if (a[s] < 0) continue;
-          double *row = lsum + a[s] * sd + j0;
+          double *row = lsum + a[s] * sd + g * 32 + cl;
           __hip_atomic_fetch_add(row, (double)x[s].x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_fetch_add(row + 1, (double)x[s].y, __ATOMIC_RELAXED,
+          __hip_atomic_fetch_add(row + 8, (double)x[s].y, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_fetch_add(row + 2, (double)x[s].z, __ATOMIC_RELAXED,
+          __hip_atomic_fetch_add(row + 16, (double)x[s].z, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-          __hip_atomic_fetch_add(row + 3, (double)x[s].w, __ATOMIC_RELAXED,
+          __hip_atomic_fetch_add(row + 24, (double)x[s].w, __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_WORKGROUP);
           if (g == 0 && cl == 0)
             __hip_atomic_fetch_add(&lcnt[a[s]], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -230,7 +245,11 @@ __global__ __launch_bounds__(256) void yk_bupd_partial_kernel(BatchDev b) {
     b.pobj[u] = obj;
   }
   double *ps = b.psum + (u * (size_t)b.k + kc0) * b.d;
-  for (int i = tid; i < ks * d; i += nthr) ps[i] = lsum[(i / d) * sd + i % d];
+  const bool perm = d >= 64 && (d & 31) == 0;  // the column order of that path
+  for (int i = tid; i < ks * d; i += nthr) {
+    const int j = i % d;
+    ps[i] = lsum[(i / d) * sd + (perm ? (j & ~31) | ((j & 3) << 3) | ((j >> 2) & 7) : j)];
+  }
   for (int i = tid; i < ks; i += nthr) b.pcnt[u * b.k + kc0 + i] = lcnt[i];
 }
 

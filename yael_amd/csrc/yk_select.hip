@@ -1180,38 +1180,44 @@ __global__ __launch_bounds__(256) void yk_bassign_res_kernel(BassignArgs a) {
     for (; pb < p1; pb += step) {
       const long pn = pb + step < p1 ? pb + step : pb;  // past the chunk: reload this block
       const double qdn = a.vn[off + min(pn + r, p1 - 1)];
+      // the centroid blocks' MFMA chains interleaved chunk by chunk (two
+      // independent accumulators); chunk c of the next point block loads
+      // once every chain has read it, a whole block of MFMAs ahead of use
+      f32x16 acc[NBLK];
+#pragma unroll
+      for (int cb = 0; cb < NBLK; cb++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) acc[cb][e] = 0.f;
+#pragma unroll
+      for (int c = 0; c < NCH; c++) {
+        const float4 b0 = bq[2 * c], b1 = bq[2 * c + 1];
+#pragma unroll
+        for (int cb = 0; cb < NBLK; cb++) {
+          const float4 a0 = cf[cb][2 * c], a1 = cf[cb][2 * c + 1];
+          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc[cb], 0, 0, 0);
+          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc[cb], 0, 0, 0);
+          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc[cb], 0, 0, 0);
+          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc[cb], 0, 0, 0);
+          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc[cb], 0, 0, 0);
+          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc[cb], 0, 0, 0);
+          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc[cb], 0, 0, 0);
+          acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc[cb], 0, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        loadq(pn, c);
+        __builtin_amdgcn_sched_barrier(0);
+      }
       float bv = 1e30f;
       int bi = -1;
 #pragma unroll
       for (int cb = 0; cb < NBLK; cb++) {
-        f32x16 acc;
-#pragma unroll
-        for (int e = 0; e < 16; e++) acc[e] = 0.f;
-#pragma unroll
-        for (int c = 0; c < NCH; c++) {
-          const float4 a0 = cf[cb][2 * c], a1 = cf[cb][2 * c + 1];
-          const float4 b0 = bq[2 * c], b1 = bq[2 * c + 1];
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.w, b0.w, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.z, b1.z, acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.w, b1.w, acc, 0, 0, 0);
-          if (cb == NBLK - 1) {
-            __builtin_amdgcn_sched_barrier(0);
-            loadq(pn, c);
-            __builtin_amdgcn_sched_barrier(0);
-          }
-        }
         if (32 * cb + 32 <= a.kc) {
-          yk_best1_block<false>(acc, nn[cb], qd, 32L * cb, h, false, bv, bi);
+          yk_best1_block<false>(acc[cb], nn[cb], qd, 32L * cb, h, false, bv, bi);
         } else {
 #pragma unroll
           for (int i = 0; i < 16; i++) {
             const int bidx = 32 * cb + 8 * (i >> 2) + 4 * h + (i & 3);
-            const float v = (float)(qd + (double)nn[cb][i]) - 2.0f * acc[i];
+            const float v = (float)(qd + (double)nn[cb][i]) - 2.0f * acc[cb][i];
             if (bidx < a.kc && v < bv) {
               bv = v;
               bi = bidx;
