@@ -1098,11 +1098,11 @@ extern "C" int yk_knn_f32_noalloc(int type, long nq, long nb, int d, int k, cons
 // batched form: the assign step of many k-means instances in one launch
 
 // Small centroid sets (NBLK 32-row blocks, d <= 16 NCH): one workgroup per
-// update chunk (up to `chunk` consecutive points of one instance) holds the
-// instance's centroid fragments in registers and streams the chunk's points
+// update chunk (up to `chunk` consecutive points of one instance) stages the
+// instance's centroid fragments in LDS and streams the chunk's points
 // through in 32-point blocks, wave w taking blocks w, w + W, ...; a block's
-// fragments are reloaded chunk by chunk right after the last centroid
-// block's MFMAs have read them, so every load has the next block's MFMAs to
+// fragments are reloaded chunk by chunk right after the centroid blocks'
+// MFMAs have read them, so every load has the next block's MFMAs to
 // arrive in.  (The general scan holds the points and streams the
 // centroids, which for k <= 64 leaves each 32-point tile two centroid blocks
 // of work against the full latency of loading its points.)  Same MFMA
@@ -1123,15 +1123,18 @@ struct BassignArgs {
 };
 
 template <int NCH, int NBLK>
-__global__ __launch_bounds__(256) void yk_bassign_res_kernel(BassignArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+yk_bassign_res_kernel(BassignArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   const int r = lane & 31, h = lane >> 5;
   const int xcd = blockIdx.x & 7, local = blockIdx.x >> 3, wpx = gridDim.x >> 3;
   const int ibeg = xcd * a.per_xcd, iend = min(a.nuitems, ibeg + a.per_xcd);
   const int ld = a.ld;
+  __shared__ float4 cfs[NBLK * 2 * NCH * 64];
+  __shared__ float4 nns[NBLK * 4 * 64];
   __shared__ int s_item;
   auto next_item = [&](int it) {
-    __syncthreads();
+    __syncthreads();  // also: every wave is done with the staged centroids
     if (threadIdx.x == 0) s_item = ibeg + wpx + atomicAdd(&a.tick[xcd], 1);
     __syncthreads();
     return s_item;
@@ -1144,25 +1147,21 @@ __global__ __launch_bounds__(256) void yk_bassign_res_kernel(BassignArgs a) {
     const long p1 = p0 + a.chunk < n ? p0 + a.chunk : n;
     const float *cb0 = a.cent + (long)inst * a.kpad * ld;
     const float *cnb = a.cn + (long)inst * a.kpad;
-    // centroid fragments (rows past kc clamped to the last, masked below)
-    // and norms (rows 32 cb + 8 g + 4 h .. + 3)
-    float4 cf[NBLK][2 * NCH];
-    float nn[NBLK][16];
-#pragma unroll
-    for (int cb = 0; cb < NBLK; cb++) {
-      const float *row = cb0 + (long)min(32 * cb + r, a.kc - 1) * ld + h * 8;
-#pragma unroll
-      for (int c = 0; c < NCH; c++) {
-        cf[cb][2 * c] = *(const float4 *)(row + 16 * c);
-        cf[cb][2 * c + 1] = *(const float4 *)(row + 16 * c + 4);
-      }
-#pragma unroll
-      for (int g = 0; g < 4; g++) {
-        const float4 n4 = *(const float4 *)(cnb + 32 * cb + 8 * g + 4 * h);
-        nn[cb][4 * g] = n4.x; nn[cb][4 * g + 1] = n4.y;
-        nn[cb][4 * g + 2] = n4.z; nn[cb][4 * g + 3] = n4.w;
-      }
+    // centroid fragments and norms, staged in LDS once for the workgroup
+    // in the lanes' order: fragment entry (cb, half-chunk q, lane) holds row
+    // 32 cb + r, dimensions 16 (q / 2) + 8 h + 4 (q % 2) .. + 3 (rows past kc
+    // clamped to the last, masked below); norm entry (cb, g, lane) rows
+    // 32 cb + 8 g + 4 h .. + 3
+    for (int i = threadIdx.x; i < NBLK * 2 * NCH * 64; i += blockDim.x) {
+      const int ln = i & 63, q = (i >> 6) % (2 * NCH), cb = (i >> 6) / (2 * NCH);
+      const float *row = cb0 + (long)min(32 * cb + (ln & 31), a.kc - 1) * ld + (ln >> 5) * 8;
+      cfs[i] = *(const float4 *)(row + 16 * (q >> 1) + 4 * (q & 1));
     }
+    for (int i = threadIdx.x; i < NBLK * 4 * 64; i += blockDim.x) {
+      const int ln = i & 63, g = (i >> 6) & 3, cb = i >> 8;
+      nns[i] = *(const float4 *)(cnb + 32 * cb + 8 * g + 4 * (ln >> 5));
+    }
+    __syncthreads();
     const long step = 32L * nwave;
     long pb = p0 + 32L * wave;
     if (pb >= p1) continue;
@@ -1188,12 +1187,27 @@ __global__ __launch_bounds__(256) void yk_bassign_res_kernel(BassignArgs a) {
       for (int cb = 0; cb < NBLK; cb++)
 #pragma unroll
         for (int e = 0; e < 16; e++) acc[cb][e] = 0.f;
+      // the centroid operand of chunk c + 1 is read from LDS while chunk
+      // c's MFMAs run
+      float4 av[2][NBLK][2];
+#pragma unroll
+      for (int cb = 0; cb < NBLK; cb++) {
+        av[0][cb][0] = cfs[(cb * 2 * NCH) * 64 + lane];
+        av[0][cb][1] = cfs[(cb * 2 * NCH + 1) * 64 + lane];
+      }
 #pragma unroll
       for (int c = 0; c < NCH; c++) {
         const float4 b0 = bq[2 * c], b1 = bq[2 * c + 1];
+        if (c + 1 < NCH) {
+#pragma unroll
+          for (int cb = 0; cb < NBLK; cb++) {
+            av[(c + 1) & 1][cb][0] = cfs[(cb * 2 * NCH + 2 * c + 2) * 64 + lane];
+            av[(c + 1) & 1][cb][1] = cfs[(cb * 2 * NCH + 2 * c + 3) * 64 + lane];
+          }
+        }
 #pragma unroll
         for (int cb = 0; cb < NBLK; cb++) {
-          const float4 a0 = cf[cb][2 * c], a1 = cf[cb][2 * c + 1];
+          const float4 a0 = av[c & 1][cb][0], a1 = av[c & 1][cb][1];
           acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc[cb], 0, 0, 0);
           acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc[cb], 0, 0, 0);
           acc[cb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.z, b0.z, acc[cb], 0, 0, 0);
@@ -1211,13 +1225,19 @@ __global__ __launch_bounds__(256) void yk_bassign_res_kernel(BassignArgs a) {
       int bi = -1;
 #pragma unroll
       for (int cb = 0; cb < NBLK; cb++) {
+        float nn[16];
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          const float4 n4 = nns[(cb * 4 + g) * 64 + lane];
+          nn[4 * g] = n4.x; nn[4 * g + 1] = n4.y; nn[4 * g + 2] = n4.z; nn[4 * g + 3] = n4.w;
+        }
         if (32 * cb + 32 <= a.kc) {
-          yk_best1_block<false>(acc[cb], nn[cb], qd, 32L * cb, h, false, bv, bi);
+          yk_best1_block<false>(acc[cb], nn, qd, 32L * cb, h, false, bv, bi);
         } else {
 #pragma unroll
           for (int i = 0; i < 16; i++) {
             const int bidx = 32 * cb + 8 * (i >> 2) + 4 * h + (i & 3);
-            const float v = (float)(qd + (double)nn[cb][i]) - 2.0f * acc[cb][i];
+            const float v = (float)(qd + (double)nn[i]) - 2.0f * acc[cb][i];
             if (bidx < a.kc && v < bv) {
               bv = v;
               bi = bidx;
@@ -1246,7 +1266,7 @@ static void (*yk_bassign_pick2())(BassignArgs) {
 }
 
 // The kernel for (nch, nblk), or null when the centroid set does not fit
-// (centroid fragments of nblk x nch x 8 registers, at most 128).
+// (centroid fragments of nblk x nch x 2 KB of LDS, at most 32 KB).
 static void (*yk_bassign_pick(int nch, int nblk))(BassignArgs) {
   if (nblk == 1) {
     switch (nch) {
