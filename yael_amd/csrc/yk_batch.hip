@@ -309,6 +309,14 @@ __global__ void yk_bsplit_kernel(BatchDev b) {
   const int it = min(max(*b.it - *b.w0, 0), b.win - 1);
   const int k = b.k, d = b.d;
   const int *na = b.nassign + (long)inst * k;
+  // the split probabilities are needed only when a cluster is empty (the
+  // draws are taken per empty cluster)
+  bool any_empty = false;
+  for (int c = 0; c < k; c++) any_empty |= na[c] == 0;
+  if (!any_empty) {
+    b.nre[inst] = 0;
+    return;
+  }
   float *p = b.pbuf + (long)inst * k;
   for (int c = 0; c < k; c++) {
     const unsigned u = (unsigned)na[c];
