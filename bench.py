@@ -496,9 +496,10 @@ def bench_precision_d960(ya, dev, nb=1_000_000, nq=10_000, d=960, k=10, kp=64, s
     return r
 
 
-def tuned_plans():
+def tuned_plans(tag=None):
     """The launch plans the library's autotuner recorded (shape class -> plan
-    index), read from its tuning cache file."""
+    index), read from its tuning cache file; with tag, only the shape
+    classes whose key contains it."""
     path = os.environ.get("YAEL_MI355X_TUNE_CACHE") or os.path.join(
         os.path.expanduser("~"), ".cache", "yael_mi355x_tune.txt")
     out = {}
@@ -506,7 +507,7 @@ def tuned_plans():
         with open(path) as f:
             for line in f:
                 kv = line.split()
-                if len(kv) == 2:
+                if len(kv) == 2 and (tag is None or tag in kv[0]):
                     out[kv[0]] = int(kv[1])
     except OSError:
         pass
@@ -794,7 +795,9 @@ def main():
                      "sketch_kernel_ms_per_launch": sk_ms / max(sk_n, 1)},
         "accuracy": {"max_rel_err_vs_fp64": r["max_rel_err_f64"], "sorted": r["sorted_ok"],
                      "fallback_queries": r["nfb"]},
-        "tuned_plans": tuned_plans(),
+        # the plans of this search's shape class (queries and base sizes as
+        # floor(log2)) in the library's tuning cache
+        "tuned_plans": tuned_plans(":q%d:b%d" % (int(np.log2(args.nq)), int(np.log2(args.nb)))),
     }
     print("# bf16 filtered pass: %.3f ms/launch -> %.1f TFLOP/s; sketch %.3f ms/launch"
           % (bf_ms_launch, ach16, sk_ms / max(sk_n, 1)), file=sys.stderr, flush=True)
