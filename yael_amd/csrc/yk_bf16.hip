@@ -1096,6 +1096,159 @@ __device__ __forceinline__ void yk_wave_select(const uint64_t *src, int m, int k
   }
 }
 
+// ---------------------------------------------------------------------------
+// Selection without a full sort: one wave per query holds the query's m keys
+// E per lane (element i * 64 + lane, so the loads are coalesced; padding
+// YK_KEY_NONE) and finds the kp-th smallest key bit by bit, top bit first:
+// with X the bits fixed so far, the candidate C = X with the current bit 0
+// and every lower bit 1 counts the keys <= C by one compare per held key
+// and a ballot popcount; fewer than kp means the bit is 1.  After the last
+// bit X is the least value with at least kp keys <= it: the kp-th smallest
+// key itself.  With lo_const (the sketch's keys, whose low word is always
+// 0xffffffff) only the 32 high bits are searched.
+
+template <int E>
+__device__ __forceinline__ void yk_wave_load(const uint64_t *src, int m, int lane,
+                                             uint64_t (&v)[E]) {
+#pragma unroll
+  for (int i = 0; i < E; i++) {
+    const int e = i * 64 + lane;
+    v[i] = e < m ? src[e] : YK_KEY_NONE;
+  }
+}
+
+template <int E>
+__device__ __forceinline__ uint64_t yk_wave_kth(const uint64_t (&v)[E], int kp, bool lo_const) {
+  uint64_t x = 0;
+  const int blo = lo_const ? 32 : 0;
+  for (int bit = 63; bit >= blo; bit--) {
+    const uint64_t c = x | ((1ull << bit) - 1);
+    int n = 0;
+#pragma unroll
+    for (int i = 0; i < E; i++) n += __popcll(__builtin_amdgcn_ballot_w64(v[i] <= c));
+    if (n < kp) x |= 1ull << bit;
+  }
+  return lo_const ? (x | 0xffffffffull) : x;
+}
+
+// the kp (<= 64 E2) smallest of the keys held in v (keys are distinct;
+// fewer than kp real keys: padded with YK_KEY_NONE), ascending, to dst:
+// the keys <= the kp-th are compacted into the wave's LDS slice (ballot
+// ranks), then sorted in registers
+template <int E, int E2>
+__device__ __forceinline__ void yk_wave_topk(const uint64_t (&v)[E], int kp, int lane,
+                                             uint64_t *lds, uint64_t *dst) {
+  const uint64_t kth = yk_wave_kth<E>(v, kp, false);
+  int base = 0;
+#pragma unroll
+  for (int i = 0; i < E; i++) {
+    const bool take = v[i] <= kth && v[i] != YK_KEY_NONE;
+    const uint64_t m = __builtin_amdgcn_ballot_w64(take);
+    const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    if (take) lds[base + rank] = v[i];
+    base += __popcll(m);
+  }
+  // (base <= kp <= 64 E2: the LDS slice holds 64 E2 keys)
+  for (int e = base + lane; e < 64 * E2; e += 64) lds[e] = YK_KEY_NONE;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  uint64_t w[E2];
+#pragma unroll
+  for (int i = 0; i < E2; i++) w[i] = lds[lane * E2 + i];
+  yk_wave_sort<E2>(w, lane);
+#pragma unroll
+  for (int i = 0; i < E2; i++) {
+    const int e = lane * E2 + i;
+    if (e < kp) dst[e] = w[i];
+  }
+}
+
+template <int E>
+__device__ __forceinline__ void yk_rsel_thr(const uint64_t *p, int n, int kp, int lane,
+                                            uint64_t *kth) {
+  uint64_t v[E];
+  yk_wave_load<E>(p, n, lane, v);
+  const uint64_t x = yk_wave_kth<E>(v, kp, true);
+  if (lane == 0) *kth = x;
+}
+
+// the sketch's bound: per query the kp-th smallest of its n partial keys
+// (low words 0xffffffff), or YK_KEY_NONE when n < kp; one wave per query
+__global__ __launch_bounds__(256) void yk_bf_rsel_thr_kernel(const uint64_t *__restrict__ src,
+                                                             long nq, int n, int kp,
+                                                             uint64_t *thr) {
+  const long q = blockIdx.x * 4L + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (q >= nq) return;
+  const uint64_t *p = src + q * (long)n;
+  if (kp > n) {
+    if (lane == 0) thr[q] = YK_KEY_NONE;
+    return;
+  }
+  if (n <= 256)
+    yk_rsel_thr<4>(p, n, kp, lane, thr + q);
+  else if (n <= 512)
+    yk_rsel_thr<8>(p, n, kp, lane, thr + q);
+  else if (n <= 1024)
+    yk_rsel_thr<16>(p, n, kp, lane, thr + q);
+  else if (n <= 2048)
+    yk_rsel_thr<32>(p, n, kp, lane, thr + q);
+  else
+    yk_rsel_thr<64>(p, n, kp, lane, thr + q);
+}
+
+template <int E, int E2>
+__device__ __forceinline__ void yk_rsel_cand(const uint64_t *p, int m, int n, int kp, int lane,
+                                             uint64_t *lds, uint64_t *dst, uint64_t *thr,
+                                             int *cnt, int *flag) {
+  uint64_t v[E];
+  yk_wave_load<E>(p, m, lane, v);
+  if (n > YK_CAND_CAP) {
+    // the list overflowed: the kp-th smallest captured key is a tighter
+    // bound for the query's next pass
+    const uint64_t x = yk_wave_kth<E>(v, kp, false);
+    if (lane == 0) {
+      *thr = x;
+      *cnt = 0;
+      atomicOr(flag, 1);
+    }
+    return;
+  }
+  yk_wave_topk<E, E2>(v, kp, lane, lds, dst);
+  if (lane == 0) *thr = 0;
+}
+
+// The filtered pass's candidate lists: per query (thr != 0) the kp smallest
+// of its captured keys, sorted, to cand_out, and thr cleared; a list that
+// overflowed YK_CAND_CAP gets a tighter bound and the flag.  One wave per
+// query; kp <= 64 E2 (E2 = 16 covers YK_KMAX).
+template <int E2>
+__global__ __launch_bounds__(256) void yk_bf_rsel_cand_kernel(const uint64_t *__restrict__ cand,
+                                                              long nq, int *cnt, uint64_t *thr,
+                                                              int kp, uint64_t *cand_out,
+                                                              int *flag) {
+  __shared__ uint64_t lds[4][64 * E2];
+  const long q = blockIdx.x * 4L + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (q >= nq || thr[q] == 0) return;
+  const int n = cnt[q];
+  const int m = n < YK_CAND_CAP ? n : YK_CAND_CAP;
+  const uint64_t *p = cand + q * (long)YK_CAND_CAP;
+  uint64_t *dst = cand_out + q * kp;
+  if (m <= 256)
+    yk_rsel_cand<4, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, cnt + q, flag);
+  else if (m <= 512)
+    yk_rsel_cand<8, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, cnt + q, flag);
+  else if (m <= 1024)
+    yk_rsel_cand<16, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, cnt + q, flag);
+  else if (m <= 2048)
+    yk_rsel_cand<32, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, cnt + q, flag);
+  else
+    yk_rsel_cand<64, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, cnt + q, flag);
+}
+
 // yk_bf_sortsel_kernel for n <= 1024 keys per query: four queries per
 // workgroup, one per wave
 __global__ __launch_bounds__(256) void yk_bf_wsortsel_kernel(const uint64_t *__restrict__ src,
@@ -1654,7 +1807,10 @@ static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
 // the kp-th smallest to thr
 static int yk_bf_sortsel(const uint64_t *src, long nq, int n, int kp, uint64_t *cand_out,
                          uint64_t *thr) {
-  if (n <= 1024) {
+  if (thr && n <= 4096) {  // the sketch's bound: a select, no sort
+    hipLaunchKernelGGL(yk_bf_rsel_thr_kernel, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0,
+                       yk_stream(), src, nq, n, kp, thr);
+  } else if (n <= 1024) {
     hipLaunchKernelGGL(yk_bf_wsortsel_kernel, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0,
                        yk_stream(), src, nq, n, kp, cand_out, thr);
   } else {
@@ -1724,10 +1880,16 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   a.cand = (uint64_t *)vcand;
   for (int tries = 0;; tries++) {
     YK_TRY((yk_launch_bscan<3, 1>(pb, a)));
-    hipLaunchKernelGGL(yk_bf_wselect_kernel, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0,
-                       yk_stream(), a.cand, nq, cnt, thr, kp, cand_out);
-    hipLaunchKernelGGL(yk_bf_select_kernel, dim3((unsigned)nq), dim3(256), 0, yk_stream(), a.cand,
-                       cnt, thr, kp, cand_out, flag);
+    const dim3 g4((unsigned)((nq + 3) / 4));
+    if (kp <= 256)
+      hipLaunchKernelGGL(yk_bf_rsel_cand_kernel<4>, g4, dim3(256), 0, yk_stream(), a.cand, nq, cnt,
+                         thr, kp, cand_out, flag);
+    else if (kp <= 512)
+      hipLaunchKernelGGL(yk_bf_rsel_cand_kernel<8>, g4, dim3(256), 0, yk_stream(), a.cand, nq, cnt,
+                         thr, kp, cand_out, flag);
+    else
+      hipLaunchKernelGGL(yk_bf_rsel_cand_kernel<16>, g4, dim3(256), 0, yk_stream(), a.cand, nq,
+                         cnt, thr, kp, cand_out, flag);
     YK_HIP(hipGetLastError());
     int hflag = 0;
     YK_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
