@@ -98,6 +98,22 @@ def test_forced_fallback(dev):
     assert nfb > 0
 
 
+@pytest.mark.parametrize("kp", [64, 300])
+def test_massive_ties_overflow_the_candidate_lists(dev, kp):
+    """16 distinct integer vectors repeated over 80,000 base rows (~5,000
+    copies each): the filtered pass's bound value is shared by every copy of
+    a query's nearest vector, so each query captures more than its
+    4,096-key list; the overflow path (the k'-th captured key as a tighter
+    bound, another pass) must still give the FP32 form's lists, exact ties
+    ordered by index."""
+    rng = np.random.default_rng(12)
+    distinct = mixture(rng, 16, 64, 16, 30)
+    b = distinct[rng.integers(0, 16, 80000)]
+    q = mixture(rng, 300, 64, 16, 30)
+    print("massive ties, k'", kp, flush=True)
+    same_as_fp32(dev, q, b, 10, kp)
+
+
 def test_small_base(dev):
     rng = np.random.default_rng(10)
     b = mixture(rng, 50, 16, 4, 10)
