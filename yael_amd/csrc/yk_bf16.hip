@@ -370,7 +370,7 @@ struct BfSel {
   uint64_t top[NB][MODE == 2 ? R : 1];
   float bv[NB][MODE == 1 ? 2 : 1];  // MODE 1: minima of the even and odd blocks
   uint64_t T[NB];
-  float Tv[NB];
+  float Tv[NB];  // MODE 3: the bound's value; MODE 2: the value of top[j][R - 1] (or INF)
   // MODE 3: candidate keys are staged in LDS, CST per (query tile,
   // lane), and appended to the query's global list a region at a time (one
   // atomic per region instead of one per key)
@@ -408,6 +408,7 @@ struct BfSel {
       T[j] = MODE == 3 ? a.thr[qrow[j]] : 0;
       Tv[j] = (T[j] == 0) ? -INFINITY
                           : (T[j] == YK_KEY_NONE ? INFINITY : yk_fkey_inv((uint32_t)(T[j] >> 32)));
+      if (MODE == 2) Tv[j] = INFINITY;  // the empty list's bound
     }
   }
 
@@ -462,10 +463,8 @@ struct BfSel {
       float m = acc[j][0];
 #pragma unroll
       for (int i = 1; i < 16; i++) m = fminf(m, acc[j][i]);
-      const float lim = MODE == 2 ? (top[j][R - 1] == YK_KEY_NONE
-                                         ? INFINITY
-                                         : yk_fkey_inv((uint32_t)(top[j][R - 1] >> 32)))
-                                  : Tv[j];
+      // (MODE 2: the list bound's value is kept in Tv, updated on insertion)
+      const float lim = Tv[j];
       if (MODE == 3 && RING > 0) {
         // park the passing lanes' tiles (the row limit goes with them: the
         // minimum may come from a pad row of the last block)
@@ -512,7 +511,11 @@ struct BfSel {
         msk &= msk - 1;
         const uint64_t key = yk_key(vs[i], (int)bb + 8 * (i >> 2) + (i & 3) + 4 * h);
         if (MODE == 2) {
-          if (key < top[j][R - 1]) yk_reg_insert16<R>(top[j], key);
+          if (key < top[j][R - 1]) {
+            yk_reg_insert16<R>(top[j], key);
+            Tv[j] = top[j][R - 1] == YK_KEY_NONE ? INFINITY
+                                                 : yk_fkey_inv((uint32_t)(top[j][R - 1] >> 32));
+          }
         } else if (key <= T[j]) {
           const int n = ncs[j];
           cs[(j * 64 + lane) * CST + n] = key;
