@@ -1005,35 +1005,6 @@ __global__ __launch_bounds__(256) void yk_bf_sortsel_kernel(const uint64_t *__re
   for (int i = threadIdx.x; i < kp; i += blockDim.x) cand_out[q * kp + i] = i < p2 ? s[i] : YK_KEY_NONE;
 }
 
-// candidates of a filtered pass: fitted -> the kp smallest into cand_out,
-// thr = 0; overflow -> thr = kp-th captured key, cnt = 0, flag set
-__global__ __launch_bounds__(256) void yk_bf_select_kernel(const uint64_t *__restrict__ cand,
-                                                           int *cnt, uint64_t *thr, int kp,
-                                                           uint64_t *cand_out, int *flag) {
-  __shared__ uint64_t s[YK_CAND_CAP];
-  const long q = blockIdx.x;
-  if (thr[q] == 0) return;
-  const int n = cnt[q];
-  const int m = n < YK_CAND_CAP ? n : YK_CAND_CAP;
-  int p2 = 1;
-  while (p2 < m) p2 <<= 1;
-  const uint64_t *p = cand + q * (long)YK_CAND_CAP;
-  for (int i = threadIdx.x; i < p2; i += blockDim.x) s[i] = i < m ? p[i] : YK_KEY_NONE;
-  __syncthreads();
-  yk_bitonic_b(s, p2);
-  if (n > YK_CAND_CAP) {
-    if (threadIdx.x == 0) {
-      thr[q] = s[kp - 1];
-      cnt[q] = 0;
-      atomicOr(flag, 1);
-    }
-    return;
-  }
-  for (int i = threadIdx.x; i < kp; i += blockDim.x) cand_out[q * kp + i] = i < p2 ? s[i] : YK_KEY_NONE;
-  __syncthreads();
-  if (threadIdx.x == 0) thr[q] = 0;
-}
-
 // ---------------------------------------------------------------------------
 // One wave per query, keys in registers: E keys per lane (64 E keys), the
 // lane's keys being elements lane * E .. lane * E + E - 1 of the sequence.
@@ -1273,28 +1244,6 @@ __global__ __launch_bounds__(256) void yk_bf_wsortsel_kernel(const uint64_t *__r
     yk_wave_select<8>(p, n, kp, lane, dst, kth);
   else
     yk_wave_select<16>(p, n, kp, lane, dst, kth);
-}
-
-// yk_bf_select_kernel for the queries whose captured keys fit 1024 (the
-// rest are left to yk_bf_select_kernel, which skips these: thr = 0 here)
-__global__ __launch_bounds__(256) void yk_bf_wselect_kernel(const uint64_t *__restrict__ cand,
-                                                            long nq, const int *__restrict__ cnt,
-                                                            uint64_t *thr, int kp,
-                                                            uint64_t *cand_out) {
-  const long q = blockIdx.x * 4L + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (q >= nq || thr[q] == 0) return;
-  const int n = cnt[q];
-  if (n > 1024) return;
-  const uint64_t *p = cand + q * (long)YK_CAND_CAP;
-  uint64_t *dst = cand_out + q * kp;
-  if (kp <= 256 && n <= 256)
-    yk_wave_select<4>(p, n, kp, lane, dst, nullptr);
-  else if (kp <= 512 && n <= 512)
-    yk_wave_select<8>(p, n, kp, lane, dst, nullptr);
-  else
-    yk_wave_select<16>(p, n, kp, lane, dst, nullptr);
-  if (lane == 0) thr[q] = 0;
 }
 
 __global__ void yk_bf_zero_kernel(int *p, long n) {
