@@ -1181,8 +1181,10 @@ __device__ __forceinline__ void yk_rsel_cand(const uint64_t *p, int m, int n, in
   yk_wave_load<E>(p, m, lane, v);
   if (n > YK_CAND_CAP) {
     // the list overflowed: the kp-th smallest captured key is a tighter
-    // bound for the query's next pass
+    // bound for the query's next pass; the query's list is written empty
+    // (every reader of it stays in bounds until that pass replaces it)
     const uint64_t x = yk_wave_kth<E>(v, kp, false);
+    for (int i = lane; i < kp; i += 64) dst[i] = YK_KEY_NONE;
     if (lane == 0) {
       *thr = x;
       *cnt = 0;
@@ -1775,8 +1777,14 @@ static int yk_bf_sortsel(const uint64_t *src, long nq, int n, int kp, uint64_t *
   return YK_OK;
 }
 
-// kp smallest first-pass keys per query -> cand [nq][kp]
-static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
+// kp smallest first-pass keys per query -> cand [nq][kp].  For kp > YK_REG_R
+// the filtered pass captures at most YK_CAND_CAP keys per query; a query
+// whose capture overflowed gets an empty list, a tighter bound and the
+// device flag `ovf` set (the caller zeroes it first).  Without resume the
+// call enqueues one filtered pass and returns without waiting; with resume
+// (after the caller read the flag set) it repeats the filtered pass for the
+// overflowed queries with their tighter bounds, waiting on the flag each time.
+static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out, int *ovf, bool resume) {
   const long nq = a.nq;
   void *vpart;
   if (kp <= YK_REG_R) {
@@ -1810,46 +1818,48 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out) {
   YK_TRY(yk_bscratch(2, sizeof(int) * (nq + 64), &vcnt));
   YK_TRY(yk_bscratch(3, sizeof(uint64_t) * nq * YK_CAND_CAP, &vcand));
   uint64_t *thr = (uint64_t *)vthr;
-  int *cnt = (int *)vcnt, *flag = cnt + nq + 32;
-  // sketch: per-lane minima over a 1-in-stride sample, >= kp splits (same
-  // reasoning as the FP32 form in yk_select.hip)
-  const long stride = yk_bsketch_stride(a.nb, kp);
-  BPlan pa = yk_bplan(nq, a.nb, yk_bnb(1, a.ldh >> 4), stride, kp, kp > 256 ? kp : 256,
-                      64L * a.ldh);
-  // per query and split: 2 lane halves x 2 block parities of minima, and
-  // for long vectors that per wave of the tiled kernel's workgroup
-  const int nsub = (a.ldh >> 4) > 16 ? 4 * YK_TW : 4;
-  YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * pa.nsplit * nsub, &vpart));
-  a.part = (uint64_t *)vpart;
-  YK_TRY((yk_launch_bscan<1, 1>(pa, a)));
-  YK_TRY(yk_bf_sortsel(a.part, nq, pa.nsplit * nsub, kp, nullptr, thr));
-  hipLaunchKernelGGL(yk_bf_zero_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
-                     yk_stream(), cnt, nq);
-  hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), flag, 1L);
+  int *cnt = (int *)vcnt;
+  if (!resume) {
+    // sketch: per-lane minima over a 1-in-stride sample, >= kp splits (same
+    // reasoning as the FP32 form in yk_select.hip)
+    const long stride = yk_bsketch_stride(a.nb, kp);
+    BPlan pa = yk_bplan(nq, a.nb, yk_bnb(1, a.ldh >> 4), stride, kp, kp > 256 ? kp : 256,
+                        64L * a.ldh);
+    // per query and split: 2 lane halves x 2 block parities of minima, and
+    // for long vectors that per wave of the tiled kernel's workgroup
+    const int nsub = (a.ldh >> 4) > 16 ? 4 * YK_TW : 4;
+    YK_TRY(yk_bscratch(0, sizeof(uint64_t) * nq * pa.nsplit * nsub, &vpart));
+    a.part = (uint64_t *)vpart;
+    YK_TRY((yk_launch_bscan<1, 1>(pa, a)));
+    YK_TRY(yk_bf_sortsel(a.part, nq, pa.nsplit * nsub, kp, nullptr, thr));
+    hipLaunchKernelGGL(yk_bf_zero_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
+                       yk_stream(), cnt, nq);
+  }
   BPlan pb = yk_bplan(nq, a.nb, yk_bnb(3, a.ldh >> 4), 1, 1, 4096, 64L * a.ldh);
   a.thr = thr;
   a.cnt = cnt;
   a.cand = (uint64_t *)vcand;
-  for (int tries = 0;; tries++) {
+  for (int tries = resume ? 1 : 0;; tries++) {
+    if (resume) hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), ovf, 1L);
     YK_TRY((yk_launch_bscan<3, 1>(pb, a)));
     const dim3 g4((unsigned)((nq + 3) / 4));
     if (kp <= 256)
       hipLaunchKernelGGL(yk_bf_rsel_cand_kernel<4>, g4, dim3(256), 0, yk_stream(), a.cand, nq, cnt,
-                         thr, kp, cand_out, flag);
+                         thr, kp, cand_out, ovf);
     else if (kp <= 512)
       hipLaunchKernelGGL(yk_bf_rsel_cand_kernel<8>, g4, dim3(256), 0, yk_stream(), a.cand, nq, cnt,
-                         thr, kp, cand_out, flag);
+                         thr, kp, cand_out, ovf);
     else
       hipLaunchKernelGGL(yk_bf_rsel_cand_kernel<16>, g4, dim3(256), 0, yk_stream(), a.cand, nq,
-                         cnt, thr, kp, cand_out, flag);
+                         cnt, thr, kp, cand_out, ovf);
     YK_HIP(hipGetLastError());
+    if (!resume) break;
     int hflag = 0;
-    YK_HIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
+    YK_HIP(hipMemcpyAsync(&hflag, ovf, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
     YK_HIP(hipStreamSynchronize(yk_stream()));
     if (!hflag) break;
     if (tries == 5)
       return yk_set_error(YK_EINVAL, "k'=%d: too many tied first-pass distances", kp);
-    hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), flag, 1L);
   }
   return YK_OK;
 }
@@ -1986,7 +1996,9 @@ static int yk_knn_bf16_run(int type, long nq, long nb, int d, int k, int kp, con
   const long qb = nq < QB ? nq : QB;
   YK_TRY(yk_bscratch(4, sizeof(uint64_t) * qb * kp, &vcand));
   YK_TRY(yk_bscratch(5, sizeof(int) * (qb + 64), &vunc));
-  int *unc = (int *)vunc, *nunc = unc + qb + 32;
+  // the uncertified count and the candidate-overflow flag side by side: one
+  // read after the re-rank brings both back
+  int *unc = (int *)vunc, *nunc = unc + qb + 32, *ovf = nunc + 1;
   const int ld = yk_ld(d), ldh = yk_bf16_ld(d, type);
   const float gam_bf = 2.0f * (ldh + 8) * 0x1p-23f, gam_32 = 2.0f * (ld + 8) * 0x1p-23f;
   int total_fb = 0;
@@ -2010,61 +2022,76 @@ static int yk_knn_bf16_run(int type, long nq, long nb, int d, int k, int kp, con
       YK_HIP(hipGetLastError());
       a.qf = (const uint16_t *)vqf;
     }
-    YK_TRY(yk_bf16_candidates(a, kp, (uint64_t *)vcand));
-    hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), nunc, 1L);
-    int tok = yk_prof_begin(YK_PROF_RERANK);
-    if (kp <= 8) {
-      void (*rk4)(const float *, const double *, const float *, const float *, long, int, int,
-                  const uint64_t *, long, int, int, const float4 *, const float4 *, float, float,
-                  int *, float *, int *, int *);
-      switch (ld >> 4) {
-        case 1: rk4 = yk_rerank4_kernel<1>; break;
-        case 2: rk4 = yk_rerank4_kernel<2>; break;
-        case 3: rk4 = yk_rerank4_kernel<3>; break;
-        case 4: rk4 = yk_rerank4_kernel<4>; break;
-        case 5: rk4 = yk_rerank4_kernel<5>; break;
-        case 6: rk4 = yk_rerank4_kernel<6>; break;
-        case 7: rk4 = yk_rerank4_kernel<7>; break;
-        case 8: rk4 = yk_rerank4_kernel<8>; break;
-        default: rk4 = yk_rerank4_kernel<0>; break;
+    hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), nunc, 2L);
+    YK_TRY(yk_bf16_candidates(a, kp, (uint64_t *)vcand, ovf, false));
+    auto rerank = [&]() -> int {
+      int tok = yk_prof_begin(YK_PROF_RERANK);
+      if (kp <= 8) {
+        void (*rk4)(const float *, const double *, const float *, const float *, long, int, int,
+                    const uint64_t *, long, int, int, const float4 *, const float4 *, float, float,
+                    int *, float *, int *, int *);
+        switch (ld >> 4) {
+          case 1: rk4 = yk_rerank4_kernel<1>; break;
+          case 2: rk4 = yk_rerank4_kernel<2>; break;
+          case 3: rk4 = yk_rerank4_kernel<3>; break;
+          case 4: rk4 = yk_rerank4_kernel<4>; break;
+          case 5: rk4 = yk_rerank4_kernel<5>; break;
+          case 6: rk4 = yk_rerank4_kernel<6>; break;
+          case 7: rk4 = yk_rerank4_kernel<7>; break;
+          case 8: rk4 = yk_rerank4_kernel<8>; break;
+          default: rk4 = yk_rerank4_kernel<0>; break;
+        }
+        hipLaunchKernelGGL(rk4, dim3((unsigned)((n + 3) / 4)), dim3(64), 0, yk_stream(),
+                           q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, nb, ld, type,
+                           (const uint64_t *)vcand, n, kp, k, (const float4 *)qstats + q0,
+                           (const float4 *)bstats_max, gam_bf, gam_32, assign + q0 * k,
+                           dis + q0 * k, unc, nunc);
+      } else {
+        void (*rt)(const float *, const double *, const float *, const float *, int, int,
+                   const uint64_t *, int, int, const float4 *, const float4 *, float, float,
+                   uint64_t *);
+        switch (ld >> 4) {
+          case 1: rt = yk_rerank_tile_kernel<1>; break;
+          case 2: rt = yk_rerank_tile_kernel<2>; break;
+          case 3: rt = yk_rerank_tile_kernel<3>; break;
+          case 4: rt = yk_rerank_tile_kernel<4>; break;
+          case 5: rt = yk_rerank_tile_kernel<5>; break;
+          case 6: rt = yk_rerank_tile_kernel<6>; break;
+          case 7: rt = yk_rerank_tile_kernel<7>; break;
+          case 8: rt = yk_rerank_tile_kernel<8>; break;
+          default: rt = yk_rerank_tile_kernel<0>; break;
+        }
+        void *vex;
+        YK_TRY(yk_bscratch(8, sizeof(uint64_t) * (size_t)n * kp, &vex));
+        hipLaunchKernelGGL(rt, dim3((unsigned)((kp + 31) / 32), (unsigned)n), dim3(64), 0,
+                           yk_stream(), q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, ld, type,
+                           (const uint64_t *)vcand, kp, k, (const float4 *)qstats + q0,
+                           (const float4 *)bstats_max, gam_bf, gam_32, (uint64_t *)vex);
+        hipLaunchKernelGGL(yk_rerank_fin_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0,
+                           yk_stream(), (const uint64_t *)vcand, n, nb, type, kp, k,
+                           (const float4 *)qstats + q0, (const float4 *)bstats_max, gam_bf,
+                           gam_32, (const uint64_t *)vex, assign + q0 * k, dis + q0 * k, unc,
+                           nunc);
       }
-      hipLaunchKernelGGL(rk4, dim3((unsigned)((n + 3) / 4)), dim3(64), 0, yk_stream(),
-                         q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, nb, ld, type,
-                         (const uint64_t *)vcand, n, kp, k, (const float4 *)qstats + q0,
-                         (const float4 *)bstats_max, gam_bf, gam_32, assign + q0 * k,
-                         dis + q0 * k, unc, nunc);
-    } else {
-      void (*rt)(const float *, const double *, const float *, const float *, int, int,
-                 const uint64_t *, int, int, const float4 *, const float4 *, float, float,
-                 uint64_t *);
-      switch (ld >> 4) {
-        case 1: rt = yk_rerank_tile_kernel<1>; break;
-        case 2: rt = yk_rerank_tile_kernel<2>; break;
-        case 3: rt = yk_rerank_tile_kernel<3>; break;
-        case 4: rt = yk_rerank_tile_kernel<4>; break;
-        case 5: rt = yk_rerank_tile_kernel<5>; break;
-        case 6: rt = yk_rerank_tile_kernel<6>; break;
-        case 7: rt = yk_rerank_tile_kernel<7>; break;
-        case 8: rt = yk_rerank_tile_kernel<8>; break;
-        default: rt = yk_rerank_tile_kernel<0>; break;
-      }
-      void *vex;
-      YK_TRY(yk_bscratch(8, sizeof(uint64_t) * (size_t)n * kp, &vex));
-      hipLaunchKernelGGL(rt, dim3((unsigned)((kp + 31) / 32), (unsigned)n), dim3(64), 0,
-                         yk_stream(), q + q0 * ld, qn ? qn + q0 : nullptr, b, bn, ld, type,
-                         (const uint64_t *)vcand, kp, k, (const float4 *)qstats + q0,
-                         (const float4 *)bstats_max, gam_bf, gam_32, (uint64_t *)vex);
-      hipLaunchKernelGGL(yk_rerank_fin_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0,
-                         yk_stream(), (const uint64_t *)vcand, n, nb, type, kp, k,
-                         (const float4 *)qstats + q0, (const float4 *)bstats_max, gam_bf,
-                         gam_32, (const uint64_t *)vex, assign + q0 * k, dis + q0 * k, unc,
-                         nunc);
-    }
-    yk_prof_end(tok);
-    YK_HIP(hipGetLastError());
-    int hn = 0;
-    YK_HIP(hipMemcpyAsync(&hn, nunc, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
+      yk_prof_end(tok);
+      YK_HIP(hipGetLastError());
+      return YK_OK;
+    };
+    YK_TRY(rerank());
+    int hv[2] = {0, 0};
+    YK_HIP(hipMemcpyAsync(hv, nunc, sizeof(hv), hipMemcpyDeviceToHost, yk_stream()));
     YK_HIP(hipStreamSynchronize(yk_stream()));
+    if (hv[1]) {
+      // a query's capture overflowed (massive ties): its list was empty in
+      // this re-rank; the filtered pass again for such queries, then the
+      // re-rank of the batch again
+      YK_TRY(yk_bf16_candidates(a, kp, (uint64_t *)vcand, ovf, true));
+      hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), nunc, 1L);
+      YK_TRY(rerank());
+      YK_HIP(hipMemcpyAsync(hv, nunc, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
+      YK_HIP(hipStreamSynchronize(yk_stream()));
+    }
+    const int hn = hv[0];
     // Uncertified queries, first level: the precision form again on just
     // those queries with 8x the candidates (the certificate's margin grows
     // with the k'-th first-pass value); what that level cannot certify goes
