@@ -379,13 +379,15 @@ struct BfSel {
   int lane;
   float *vs;  // MODES 2, 3: this lane's 16-value LDS slot
   // MODE 3 with RING > 0: a tile whose lane minimum passes the bound is
-  // parked in the wave's ring of RING entries (its 16 values, query, first
-  // row and row limit: five float4 each); the ring is drained 64 entries at
-  // a time, one entry per lane, so the per-value work of the rare passing
-  // tiles runs on full waves instead of once per tile for one or two lanes.
-  // rhead, rtail: wave-uniform entry counters.
+  // parked in the wave's ring of RING entries (its 16 values, then the
+  // query, the lane's first row and the query's bound key as its value and
+  // row: five float4 each); the ring is drained 64 entries at a time, one
+  // entry per lane, so the per-value work of the rare passing tiles runs on
+  // full waves instead of once per tile for one or two lanes.  rhead, rtail:
+  // wave-uniform entry counters; rend: the end of the item's row range.
   float4 *ring;
   int rhead, rtail;
+  long rend;
 
   __device__ __forceinline__ void init(const Bf16Args &a, long qbase, int r, int lane_,
                                        uint64_t *cs_wave, float *vs_lane,
@@ -406,8 +408,12 @@ struct BfSel {
 #pragma unroll
       for (int s = 0; s < (MODE == 2 ? R : 1); s++) top[j][s] = YK_KEY_NONE;
       T[j] = MODE == 3 ? a.thr[qrow[j]] : 0;
-      Tv[j] = (T[j] == 0) ? -INFINITY
-                          : (T[j] == YK_KEY_NONE ? INFINITY : yk_fkey_inv((uint32_t)(T[j] >> 32)));
+      // MODE 3: a NaN bound for a query past the end or already done (thr
+      // 0), which no value passes, whatever it is: the pass test is then the
+      // bare compare
+      Tv[j] = (T[j] == 0 || !qok[j])
+                  ? __builtin_nanf("")
+                  : (T[j] == YK_KEY_NONE ? INFINITY : yk_fkey_inv((uint32_t)(T[j] >> 32)));
       if (MODE == 2) Tv[j] = INFINITY;  // the empty list's bound
     }
   }
@@ -460,16 +466,18 @@ struct BfSel {
           bv[j][0] = m;
         continue;
       }
-      float m = acc[j][0];
+      // (a chain started from +inf compiles to eight v_min3 with no operand
+      // canonicalisation; NaN is still dropped)
+      float m = INFINITY;
 #pragma unroll
-      for (int i = 1; i < 16; i++) m = fminf(m, acc[j][i]);
+      for (int i = 0; i < 16; i++) m = fminf(m, acc[j][i]);
       // (MODE 2: the list bound's value is kept in Tv, updated on insertion)
       const float lim = Tv[j];
       if (MODE == 3 && RING > 0) {
         // park the passing lanes' tiles (the row limit goes with them: the
         // minimum may come from a pad row of the last block)
-        const bool pass = qok[j] && m <= lim;
-        const unsigned long long bal = __ballot(pass);
+        const bool pass = m <= lim;
+        const unsigned long long bal = __builtin_amdgcn_ballot_w64(pass);
         if (bal) {
           if (pass) {
             const int rank = __builtin_amdgcn_mbcnt_hi(
@@ -479,9 +487,8 @@ struct BfSel {
             for (int q4 = 0; q4 < 4; q4++)
               e[q4] = make_float4(acc[j][4 * q4], acc[j][4 * q4 + 1], acc[j][4 * q4 + 2],
                                   acc[j][4 * q4 + 3]);
-            e[4] = make_float4(__int_as_float((int)qrow[j]),
-                               __int_as_float((int)bb + 4 * h), __int_as_float(FULL ? 64 : rl),
-                               0.f);
+            e[4] = make_float4(__int_as_float((int)qrow[j]), __int_as_float((int)bb + 4 * h), lim,
+                               __uint_as_float((uint32_t)T[j]));
           }
           rhead += __popcll(bal);
           if (rhead - rtail >= 64) drain(a, 64);
@@ -530,34 +537,50 @@ struct BfSel {
     }
   }
 
-  // the n oldest ring entries, lane i taking entry rtail + i: every value
-  // within the bound and the row limit is keyed (value, row), and the keys
-  // not above the query's bound key are appended to its list
+  // the n oldest ring entries, lane i taking entry rtail + i: the values of
+  // rows in the item's range whose keys (value, row) are not above the
+  // query's bound key are appended to its list -- counted first, so that one
+  // atomic per lane reserves their places (an atomic with return per key,
+  // and a load of the bound, made the wave wait out a round trip to the L2
+  // for each)
   __device__ __forceinline__ void drain(const Bf16Args &a, int n) {
     if (lane < n) {
       const float4 *e = ring + ((rtail + lane) & (RING - 1)) * 5;
       const float4 md = e[4];
-      const int q = __float_as_int(md.x), row0 = __float_as_int(md.y), rl = __float_as_int(md.z);
-      const uint64_t T = a.thr[q];
-      const float lim = T == YK_KEY_NONE ? INFINITY : yk_fkey_inv((uint32_t)(T >> 32));
-      unsigned msk = 0;
+      const int q = __float_as_int(md.x), row0 = __float_as_int(md.y);
+      const float lim = md.z;  // the bound key's value (INFINITY for no bound)
+      const uint32_t trow = __float_as_uint(md.w);  // and its row
+      const int rl = (int)min(rend - row0, 64L);
+      unsigned msk = 0, eq = 0;
 #pragma unroll
       for (int q4 = 0; q4 < 4; q4++) {
         const float4 v = e[q4];
         const float w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int t = 0; t < 4; t++)
-          msk |= (w[t] <= lim && 8 * q4 + t < rl) ? (1u << (4 * q4 + t)) : 0u;
-      }
-      const float *ev = (const float *)e;
-      while (msk) {
-        const int i = __builtin_ctz(msk);
-        msk &= msk - 1;
-        const uint64_t key = yk_key(ev[i], row0 + 8 * (i >> 2) + (i & 3));
-        if (key <= T) {
-          const int b = atomicAdd(&a.cnt[q], 1);
-          if (b < YK_CAND_CAP) a.cand[(long)q * YK_CAND_CAP + b] = key;
+        for (int t = 0; t < 4; t++) {
+          const bool in = 8 * q4 + t < rl;
+          msk |= (w[t] <= lim && in) ? (1u << (4 * q4 + t)) : 0u;
+          eq |= (w[t] == lim && in) ? (1u << (4 * q4 + t)) : 0u;
         }
+      }
+      // below the bound's value the key is below the bound key; at equal
+      // values (the same order-preserving image: -0 is taken as +0) the row
+      // decides
+      for (unsigned m = eq; m;) {
+        const int i = __builtin_ctz(m);
+        m &= m - 1;
+        if ((uint32_t)(row0 + 8 * (i >> 2) + (i & 3)) > trow) msk &= ~(1u << i);
+      }
+      if (msk) {
+        const float *ev = (const float *)e;
+        const long base = (long)q * YK_CAND_CAP;
+        int b = atomicAdd(&a.cnt[q], __popc(msk));
+        do {
+          const int i = __builtin_ctz(msk);
+          msk &= msk - 1;
+          if (b < YK_CAND_CAP) a.cand[base + b] = yk_key(ev[i], row0 + 8 * (i >> 2) + (i & 3));
+          b++;
+        } while (msk);
       }
     }
     rtail += n;
@@ -687,6 +710,7 @@ __device__ __forceinline__ void yk_scan_bf16_body(const Bf16Args &a) {
     sel.init(a, qbase, r, lane, cstage + wave * (MODE == 3 && RING == 0 ? NB * 64 * YK_CSTAGE : 0),
              vstage + (MODE == 2 || (MODE == 3 && RING == 0) ? (wave * 64 + lane) * 16 : 0),
              rstage + wave * RING * 5);
+    sel.rend = bend;
     if (MODE == 3 && !__any(sel.active())) continue;
 
     bf16x8 qreg[NB][QRES ? NCH : 1];
