@@ -114,6 +114,42 @@ def test_massive_ties_overflow_the_candidate_lists(dev, kp):
     same_as_fp32(dev, q, b, 10, kp)
 
 
+def test_starting_bound_underflow(dev):
+    """The filtered pass starts from a bound estimated from the sketch's
+    sample (an m-th smallest sampled minimum, m < k'); a query left with
+    fewer than k' keys within it takes the proven bound (the k'-th smallest
+    sampled minimum) in a second pass.  Here the only rows near the queries
+    are the first rows of every 32nd block: all of them in blocks the
+    stride-8 sketch samples, one per base range, so the sample
+    over-represents them and the estimated bound lets through fewer than k'
+    rows.  The second pass must run (two filtered-pass launches, one sketch,
+    so no escalation) and the lists must be the FP32 form's."""
+    import ctypes
+    import yael_amd
+    rng = np.random.default_rng(21)
+    nb, d, k, kp = 65536, 32, 10, 64
+    # far rows not so far that the base's norms widen the certificate's
+    # margin past the gap between the 10th and the 64th near row
+    b = (12.0 + rng.standard_normal((nb, d))).astype(np.float32)
+    near = np.arange(0, nb, 32 * 32)
+    b[near] = rng.standard_normal((len(near), d)).astype(np.float32)
+    q = rng.standard_normal((200, d)).astype(np.float32)
+    lib = yael_amd.lib()
+    lib.yk_prof_enable(1)
+    lib.yk_prof_reset()
+    try:
+        same_as_fp32(dev, q, b, k, kp)
+        counts = []
+        for slot in (6, 7):  # filtered-pass scans, sketch scans
+            tot, cnt = ctypes.c_double(), ctypes.c_long()
+            lib.yk_prof_get(slot, ctypes.byref(tot), ctypes.byref(cnt))
+            counts.append(cnt.value)
+    finally:
+        lib.yk_prof_enable(0)
+    print("filtered-pass and sketch launches", counts, flush=True)
+    assert counts == [2, 1]
+
+
 def test_small_base(dev):
     rng = np.random.default_rng(10)
     b = mixture(rng, 50, 16, 4, 10)
