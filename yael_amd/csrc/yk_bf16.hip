@@ -1680,7 +1680,7 @@ struct BPlan {
 // default plan.  The plan of the search in progress (calls are serialised
 // by the library lock; the nested escalation search keeps its caller's).
 static const long yk_bitem_targets[3] = {4096, 2048, 8192};
-static const int yk_bstride_scale[4][2] = {{1, 1}, {1, 2}, {2, 1}, {3, 2}};
+static const int yk_bstride_scale[4][2] = {{1, 1}, {8, 3}, {2, 1}, {3, 2}};
 #define YK_BNCFG 12
 static int g_bcfg = 0;
 
