@@ -1103,7 +1103,11 @@ __device__ __forceinline__ void yk_wave_select(const uint64_t *src, int m, int k
 // and a ballot popcount; fewer than kp means the bit is 1.  After the last
 // bit X is the least value with at least kp keys <= it: the kp-th smallest
 // key itself.  With lo_const (the sketch's keys, whose low word is always
-// 0xffffffff) only the 32 high bits are searched.
+// 0xffffffff) only the 32 high bits are searched.  The search starts below
+// the bits every real key shares (the highest bit where the least and the
+// greatest real key differ: distances of one magnitude share sign and
+// exponent), and once the high word is fixed, a high word held by a single
+// key makes that key the answer without searching the index bits.
 
 template <int E>
 __device__ __forceinline__ void yk_wave_load(const uint64_t *src, int m, int lane,
@@ -1115,18 +1119,68 @@ __device__ __forceinline__ void yk_wave_load(const uint64_t *src, int m, int lan
   }
 }
 
+__device__ __forceinline__ uint64_t yk_wave_min64(uint64_t a) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t lo = __shfl_xor((uint32_t)a, o), hi = __shfl_xor((uint32_t)(a >> 32), o);
+    const uint64_t b = ((uint64_t)hi << 32) | lo;
+    a = b < a ? b : a;
+  }
+  return a;
+}
+
+// keys <= c among the held keys
+template <int E>
+__device__ __forceinline__ int yk_wave_count_le(const uint64_t (&v)[E], uint64_t c) {
+  int n = 0;
+#pragma unroll
+  for (int i = 0; i < E; i++) n += __popcll(__builtin_amdgcn_ballot_w64(v[i] <= c));
+  return n;
+}
+
 template <int E>
 __device__ __forceinline__ uint64_t yk_wave_kth(const uint64_t (&v)[E], int kp, bool lo_const) {
   uint64_t x = 0;
-  const int blo = lo_const ? 32 : 0;
-  for (int bit = 63; bit >= blo; bit--) {
-    const uint64_t c = x | ((1ull << bit) - 1);
-    int n = 0;
+  int top = 63;
+  if (E <= 16) {  // (with more keys per lane the bookkeeping costs occupancy)
+    // the least and the greatest real key (the greatest as the least of the
+    // complements), and the real key count
+    uint64_t mn = YK_KEY_NONE, cmx = YK_KEY_NONE;
+    int nreal = 0;
 #pragma unroll
-    for (int i = 0; i < E; i++) n += __popcll(__builtin_amdgcn_ballot_w64(v[i] <= c));
-    if (n < kp) x |= 1ull << bit;
+    for (int i = 0; i < E; i++) {
+      const bool real = v[i] != YK_KEY_NONE;
+      nreal += __popcll(__builtin_amdgcn_ballot_w64(real));
+      mn = v[i] < mn ? v[i] : mn;
+      cmx = real && ~v[i] < cmx ? ~v[i] : cmx;
+    }
+    if (nreal < kp) return YK_KEY_NONE;
+    mn = yk_wave_min64(mn);
+    const uint64_t diff = mn ^ ~yk_wave_min64(cmx);
+    if (diff == 0) return mn;
+    top = 63 - __builtin_clzll(diff);  // the bits above it: every real key's
+    x = top == 63 ? 0 : (mn >> (top + 1)) << (top + 1);
   }
-  return lo_const ? (x | 0xffffffffull) : x;
+  // the high word
+  for (int bit = top; bit >= 32; bit--)
+    if (yk_wave_count_le<E>(v, x | ((1ull << bit) - 1)) < kp) x |= 1ull << bit;
+  if (lo_const) return x | 0xffffffffull;
+  if (E <= 16 && top >= 32) {
+    // the high word is fixed: if one key holds it, that key is the answer
+    const uint64_t xh = x >> 32;
+    int eq = 0;
+    uint64_t keq = YK_KEY_NONE;
+#pragma unroll
+    for (int i = 0; i < E; i++) {
+      const bool in = (v[i] >> 32) == xh && v[i] != YK_KEY_NONE;
+      eq += __popcll(__builtin_amdgcn_ballot_w64(in));
+      keq = in && v[i] < keq ? v[i] : keq;
+    }
+    if (eq == 1) return yk_wave_min64(keq);
+  }
+  // the index bits
+  for (int bit = top < 31 ? top : 31; bit >= 0; bit--)
+    if (yk_wave_count_le<E>(v, x | ((1ull << bit) - 1)) < kp) x |= 1ull << bit;
+  return x;
 }
 
 // the kp (<= 64 E2) smallest of the keys held in v (keys are distinct;
