@@ -117,13 +117,14 @@ def test_massive_ties_overflow_the_candidate_lists(dev, kp):
 def test_starting_bound_underflow(dev):
     """The filtered pass starts from a bound estimated from the sketch's
     sample (an m-th smallest sampled minimum, m < k'); a query left with
-    fewer than k' keys within it takes the proven bound (the k'-th smallest
-    sampled minimum) in a second pass.  Here the only rows near the queries
-    are the first rows of every 32nd block: all of them in blocks the
-    stride-8 sketch samples, one per base range, so the sample
+    fewer than k' keys within it is scanned again without a bound, which
+    overflows its list, and then with the bound the overflow gives (the
+    k'-th smallest of the keys captured).  Here the only rows near the
+    queries are the first rows of every 32nd block: all of them in blocks
+    the stride-8 sketch samples, one per base range, so the sample
     over-represents them and the estimated bound lets through fewer than k'
-    rows.  The second pass must run (two filtered-pass launches, one sketch,
-    so no escalation) and the lists must be the FP32 form's."""
+    rows.  Both further passes must run (three filtered-pass launches, one
+    sketch, so no escalation) and the lists must be the FP32 form's."""
     import ctypes
     import yael_amd
     rng = np.random.default_rng(21)
@@ -147,7 +148,7 @@ def test_starting_bound_underflow(dev):
     finally:
         lib.yk_prof_enable(0)
     print("filtered-pass and sketch launches", counts, flush=True)
-    assert counts == [2, 1]
+    assert counts == [3, 1]
 
 
 def test_small_base(dev):

@@ -1218,56 +1218,53 @@ __device__ __forceinline__ void yk_wave_topk(const uint64_t (&v)[E], int kp, int
 }
 
 template <int E>
-__device__ __forceinline__ void yk_rsel_thr(const uint64_t *p, int n, int kp, int m, int lane,
-                                            uint64_t *kth, uint64_t *mth) {
+__device__ __forceinline__ void yk_rsel_thr(const uint64_t *p, int n, int m, int lane,
+                                            uint64_t *mth) {
   uint64_t v[E];
   yk_wave_load<E>(p, n, lane, v);
-  const uint64_t x = yk_wave_kth<E>(v, kp, true);
-  const uint64_t y = m < kp ? yk_wave_kth<E>(v, m, true) : x;
-  if (lane == 0) {
-    *kth = x;
-    *mth = y;
-  }
+  const uint64_t x = yk_wave_kth<E>(v, m, true);
+  if (lane == 0) *mth = x;
 }
 
-// the sketch's bounds: per query the kp-th smallest of its n partial keys
-// (low words 0xffffffff) to thr_valid, an upper bound of the kp-th smallest
-// first-pass key, and the m-th smallest (m <= kp) to thr, the bound the
-// filtered pass starts with; YK_KEY_NONE for both when n < kp.  One wave per
-// query.
+// the sketch's bound: per query the m-th smallest of its n partial keys (low
+// words 0xffffffff), m <= kp, the bound the filtered pass starts with (for
+// m = kp an upper bound of the kp-th smallest first-pass key);
+// YK_KEY_NONE when n < kp.  One wave per query.
 __global__ __launch_bounds__(256) void yk_bf_rsel_thr_kernel(const uint64_t *__restrict__ src,
                                                              long nq, int n, int kp, int m,
-                                                             uint64_t *thr, uint64_t *thr_valid) {
+                                                             uint64_t *thr) {
   const long q = blockIdx.x * 4L + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (q >= nq) return;
   const uint64_t *p = src + q * (long)n;
   if (kp > n) {
-    if (lane == 0) thr[q] = thr_valid[q] = YK_KEY_NONE;
+    if (lane == 0) thr[q] = YK_KEY_NONE;
     return;
   }
   if (n <= 256)
-    yk_rsel_thr<4>(p, n, kp, m, lane, thr_valid + q, thr + q);
+    yk_rsel_thr<4>(p, n, m, lane, thr + q);
   else if (n <= 512)
-    yk_rsel_thr<8>(p, n, kp, m, lane, thr_valid + q, thr + q);
+    yk_rsel_thr<8>(p, n, m, lane, thr + q);
   else if (n <= 1024)
-    yk_rsel_thr<16>(p, n, kp, m, lane, thr_valid + q, thr + q);
+    yk_rsel_thr<16>(p, n, m, lane, thr + q);
   else if (n <= 2048)
-    yk_rsel_thr<32>(p, n, kp, m, lane, thr_valid + q, thr + q);
+    yk_rsel_thr<32>(p, n, m, lane, thr + q);
   else
-    yk_rsel_thr<64>(p, n, kp, m, lane, thr_valid + q, thr + q);
+    yk_rsel_thr<64>(p, n, m, lane, thr + q);
 }
 
 template <int E, int E2>
 __device__ __forceinline__ void yk_rsel_cand(const uint64_t *p, int m, int n, int kp, int lane,
                                              uint64_t *lds, uint64_t *dst, uint64_t *thr,
-                                             uint64_t tvalid, int *cnt, int *flag) {
-  if (n < kp && *thr != tvalid) {
-    // fewer than kp keys within the starting bound: the query's next pass
-    // takes the proven bound (kp keys or more are within it)
+                                             int *cnt, int *flag) {
+  if (n < kp) {
+    // fewer than kp keys within the starting bound (an estimated one: under
+    // a proven bound, or under an overflow's, kp keys or more are within
+    // it): the query's next pass takes no bound; its list then overflows
+    // and the overflow's bound below is a proven one
     for (int i = lane; i < kp; i += 64) dst[i] = YK_KEY_NONE;
     if (lane == 0) {
-      *thr = tvalid;
+      *thr = YK_KEY_NONE;
       *cnt = 0;
       atomicOr(flag, 1);
     }
@@ -1295,14 +1292,12 @@ __device__ __forceinline__ void yk_rsel_cand(const uint64_t *p, int m, int n, in
 // The filtered pass's candidate lists: per query (thr != 0) the kp smallest
 // of its captured keys, sorted, to cand_out, and thr cleared.  A list that
 // overflowed YK_CAND_CAP gets a tighter bound and the flag; one with fewer
-// than kp keys, captured under a starting bound other than the proven one
-// (thr_valid), gets the proven bound and the flag.  Either way the query's
-// list is written empty until its next pass.  One wave per query; kp <= 64
-// E2 (E2 = 16 covers YK_KMAX).
+// than kp keys (under an estimated starting bound) gets no bound and the
+// flag.  Either way the query's list is written empty until its next pass.
+// One wave per query; kp <= 64 E2 (E2 = 16 covers YK_KMAX).
 template <int E2>
 __global__ __launch_bounds__(256) void yk_bf_rsel_cand_kernel(const uint64_t *__restrict__ cand,
                                                               long nq, int *cnt, uint64_t *thr,
-                                                              const uint64_t *__restrict__ thr_valid,
                                                               int kp, uint64_t *cand_out,
                                                               int *flag) {
   __shared__ uint64_t lds[4][64 * E2];
@@ -1313,17 +1308,16 @@ __global__ __launch_bounds__(256) void yk_bf_rsel_cand_kernel(const uint64_t *__
   const int m = n < YK_CAND_CAP ? n : YK_CAND_CAP;
   const uint64_t *p = cand + q * (long)YK_CAND_CAP;
   uint64_t *dst = cand_out + q * kp;
-  const uint64_t tv = thr_valid[q];
   if (m <= 256)
-    yk_rsel_cand<4, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, tv, cnt + q, flag);
+    yk_rsel_cand<4, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, cnt + q, flag);
   else if (m <= 512)
-    yk_rsel_cand<8, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, tv, cnt + q, flag);
+    yk_rsel_cand<8, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, cnt + q, flag);
   else if (m <= 1024)
-    yk_rsel_cand<16, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, tv, cnt + q, flag);
+    yk_rsel_cand<16, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, cnt + q, flag);
   else if (m <= 2048)
-    yk_rsel_cand<32, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, tv, cnt + q, flag);
+    yk_rsel_cand<32, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, cnt + q, flag);
   else
-    yk_rsel_cand<64, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, tv, cnt + q, flag);
+    yk_rsel_cand<64, E2>(p, m, n, kp, lane, lds[wave], dst, thr + q, cnt + q, flag);
 }
 
 // yk_bf_sortsel_kernel for n <= 1024 keys per query: four queries per
@@ -1859,13 +1853,13 @@ static int yk_launch_bscan(const BPlan &p, Bf16Args a) {
 }
 
 // per query, the n keys at src[q * n ..]: the kp smallest to cand_out, or
-// the kp-th smallest to thr_valid and the m-th smallest (m <= kp; the kp-th
-// where the keys are too many for the select) to thr
+// the m-th smallest (m <= kp; the kp-th where the keys are too many for the
+// select) to thr
 static int yk_bf_sortsel(const uint64_t *src, long nq, int n, int kp, uint64_t *cand_out,
-                         uint64_t *thr, int m = 0, uint64_t *thr_valid = nullptr) {
-  if (thr && n <= 4096) {  // the sketch's bounds: a select, no sort
+                         uint64_t *thr, int m = 0) {
+  if (thr && n <= 4096) {  // the sketch's bound: a select, no sort
     hipLaunchKernelGGL(yk_bf_rsel_thr_kernel, dim3((unsigned)((nq + 3) / 4)), dim3(256), 0,
-                       yk_stream(), src, nq, n, kp, m, thr, thr_valid);
+                       yk_stream(), src, nq, n, kp, m > 0 && m < kp ? m : kp, thr);
     YK_HIP(hipGetLastError());
     return YK_OK;
   }
@@ -1879,9 +1873,6 @@ static int yk_bf_sortsel(const uint64_t *src, long nq, int n, int kp, uint64_t *
                        yk_stream(), src, n, kp, p2, cand_out, thr);
   }
   YK_HIP(hipGetLastError());
-  if (thr && thr_valid)
-    YK_HIP(hipMemcpyAsync(thr_valid, thr, sizeof(uint64_t) * nq, hipMemcpyDeviceToDevice,
-                          yk_stream()));
   return YK_OK;
 }
 
@@ -1891,8 +1882,9 @@ static int yk_bf_sortsel(const uint64_t *src, long nq, int n, int kp, uint64_t *
 // rows of the whole base below it, give or take stride * sqrt(m): five
 // standard deviations above the kp needed, for about 2 kp candidates per
 // query where the kp-th minimum (the proven bound, about 4.5 kp at stride 4)
-// lets through more.  A query left with fewer than kp keys takes the proven
-// bound in a second pass, so the results do not depend on m.
+// lets through more.  A query left with fewer than kp keys is scanned again
+// without a bound (then with the bound its overflow gives), so the results
+// do not depend on m.
 static int yk_bound_rank(int kp, long stride) {
   const double t = (double)kp / (double)stride;
   const double r = 0.5 * (5.0 + sqrt(25.0 + 4.0 * t));
@@ -1937,10 +1929,10 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out, int *ovf, 
     return YK_OK;
   }
   void *vthr, *vcnt, *vcand;
-  YK_TRY(yk_bscratch(1, 2 * sizeof(uint64_t) * nq, &vthr));
+  YK_TRY(yk_bscratch(1, sizeof(uint64_t) * nq, &vthr));
   YK_TRY(yk_bscratch(2, sizeof(int) * (nq + 64), &vcnt));
   YK_TRY(yk_bscratch(3, sizeof(uint64_t) * nq * YK_CAND_CAP, &vcand));
-  uint64_t *thr = (uint64_t *)vthr, *thr_valid = thr + nq;
+  uint64_t *thr = (uint64_t *)vthr;
   int *cnt = (int *)vcnt;
   if (!resume) {
     // sketch: per-lane minima over a 1-in-stride sample, >= kp splits (same
@@ -1955,7 +1947,7 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out, int *ovf, 
     a.part = (uint64_t *)vpart;
     YK_TRY((yk_launch_bscan<1, 1>(pa, a)));
     YK_TRY(yk_bf_sortsel(a.part, nq, pa.nsplit * nsub, kp, nullptr, thr,
-                         yk_bound_rank(kp, stride), thr_valid));
+                         yk_bound_rank(kp, stride)));
     hipLaunchKernelGGL(yk_bf_zero_kernel, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0,
                        yk_stream(), cnt, nq);
   }
@@ -1969,13 +1961,13 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out, int *ovf, 
     const dim3 g4((unsigned)((nq + 3) / 4));
     if (kp <= 256)
       hipLaunchKernelGGL(yk_bf_rsel_cand_kernel<4>, g4, dim3(256), 0, yk_stream(), a.cand, nq, cnt,
-                         thr, thr_valid, kp, cand_out, ovf);
+                         thr, kp, cand_out, ovf);
     else if (kp <= 512)
       hipLaunchKernelGGL(yk_bf_rsel_cand_kernel<8>, g4, dim3(256), 0, yk_stream(), a.cand, nq, cnt,
-                         thr, thr_valid, kp, cand_out, ovf);
+                         thr, kp, cand_out, ovf);
     else
       hipLaunchKernelGGL(yk_bf_rsel_cand_kernel<16>, g4, dim3(256), 0, yk_stream(), a.cand, nq,
-                         cnt, thr, thr_valid, kp, cand_out, ovf);
+                         cnt, thr, kp, cand_out, ovf);
     YK_HIP(hipGetLastError());
     if (!resume) break;
     int hflag = 0;
