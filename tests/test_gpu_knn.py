@@ -116,6 +116,45 @@ def test_knn_split_merge_large_k(ya, oracle):
         parity.check_knn(2, q, b, k, gi, gd, oi, od, exact=True, label="split k%d" % k)
 
 
+def test_fp32_starting_bound_underflow(ya, oracle):
+    """The FP32 form (k > 16) starts its filtered pass from an estimated
+    bound (an m-th smallest sketch minimum, m < k); a query left with fewer
+    than k keys within it is scanned again without a bound, which overflows
+    its list, then with the overflow's bound.  At this size the sketch
+    samples one block in 8 and splits the base into 37 ranges of 1,792 rows;
+    the only rows near the queries are the first rows of those ranges, one
+    per sketch subset, so the m-th smallest minimum (m = 35 for k = 40) has
+    35 rows at or below it.  Integer data: the lists must equal the oracle's
+    bit for bit, after three filtered passes over one sketch."""
+    import yael_amd
+    from yael_amd import device as dev
+    rng = np.random.default_rng(31)
+    nb, d, k = 65536, 32, 40
+    b = np.rint(40.0 + 3.0 * rng.standard_normal((nb, d))).astype(np.float32)
+    near = np.arange(0, nb, 1792)
+    b[near] = np.rint(3.0 * rng.standard_normal((len(near), d))).astype(np.float32)
+    q = np.rint(3.0 * rng.standard_normal((100, d))).astype(np.float32)
+    lib = yael_amd.lib()
+    Q, B = dev.DeviceSet(q), dev.DeviceSet(b)
+    lib.yk_prof_enable(1)
+    lib.yk_prof_reset()
+    try:
+        gi, gd = dev.knn_fp32(Q, B, k)
+        counts = []
+        for slot in (0, 8):  # filtered-pass scans, sketch scans
+            tot, cnt = ctypes.c_double(), ctypes.c_long()
+            lib.yk_prof_get(slot, ctypes.byref(tot), ctypes.byref(cnt))
+            counts.append(cnt.value)
+    finally:
+        lib.yk_prof_enable(0)
+        Q.free()
+        B.free()
+    print("filtered-pass and sketch launches", counts, flush=True)
+    oi, od = oracle.knn_full(2, b, q, k)
+    parity.check_knn(2, q, b, k, gi, gd, oi, od, exact=True, label="fp32 underflow")
+    assert counts == [3, 1]
+
+
 def test_knn_edge_shapes(ya, oracle):
     rng = np.random.default_rng(8)
     for nb, nq, d, k in [(1, 1, 1, 1), (5, 3, 7, 5), (33, 65, 3, 33), (257, 1, 128, 256), (64, 129, 17, 1)]:

@@ -1257,11 +1257,11 @@ template <int E, int E2>
 __device__ __forceinline__ void yk_rsel_cand(const uint64_t *p, int m, int n, int kp, int lane,
                                              uint64_t *lds, uint64_t *dst, uint64_t *thr,
                                              int *cnt, int *flag) {
-  if (n < kp) {
+  if (n < kp && *thr != YK_KEY_NONE) {
     // fewer than kp keys within the starting bound (an estimated one: under
     // a proven bound, or under an overflow's, kp keys or more are within
-    // it): the query's next pass takes no bound; its list then overflows
-    // and the overflow's bound below is a proven one
+    // it, the data being finite): the query's next pass takes no bound; its
+    // list then overflows and the overflow's bound below is a proven one
     for (int i = lane; i < kp; i += 64) dst[i] = YK_KEY_NONE;
     if (lane == 0) {
       *thr = YK_KEY_NONE;
@@ -1876,21 +1876,6 @@ static int yk_bf_sortsel(const uint64_t *src, long nq, int n, int kp, uint64_t *
   return YK_OK;
 }
 
-// The filtered pass's starting bound: the m-th smallest sketch minimum, m
-// the least count with stride * (m - 5 sqrt(m)) >= kp.  Over a sample of one
-// block in `stride`, a bound with m sampled rows below it has about stride * m
-// rows of the whole base below it, give or take stride * sqrt(m): five
-// standard deviations above the kp needed, for about 2 kp candidates per
-// query where the kp-th minimum (the proven bound, about 4.5 kp at stride 4)
-// lets through more.  A query left with fewer than kp keys is scanned again
-// without a bound (then with the bound its overflow gives), so the results
-// do not depend on m.
-static int yk_bound_rank(int kp, long stride) {
-  const double t = (double)kp / (double)stride;
-  const double r = 0.5 * (5.0 + sqrt(25.0 + 4.0 * t));
-  const long m = (long)ceil(r * r);
-  return (int)(m < kp ? m : kp);
-}
 
 // kp smallest first-pass keys per query -> cand [nq][kp].  For kp > YK_REG_R
 // the filtered pass captures at most YK_CAND_CAP keys per query; a query

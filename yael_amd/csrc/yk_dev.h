@@ -57,6 +57,22 @@ int yk_time_ms(int (*run)(void *), void *arg, float *ms);
 // zeroed on the library stream before the caller's next launch.
 int yk_item_tickets(int **out);
 
+// The filtered pass's starting bound (both forms, k' > 16): the m-th
+// smallest sketch minimum, m the least count with stride * (m - 5 sqrt(m)) >=
+// kp.  Over a sample of one block in `stride`, a bound with m sampled rows
+// below it has about stride * m rows of the whole base below it, give or
+// take stride * sqrt(m): five standard deviations above the kp needed (about
+// 2 kp candidates per query where the kp-th minimum, the proven bound, lets
+// through about 4.5 kp at stride 4).  A query left with fewer than kp keys
+// is scanned again without a bound (then with the bound its overflow
+// gives), so no result depends on m.
+static inline int yk_bound_rank(int kp, long stride) {
+  const double t = (double)kp / (double)stride;
+  const double r = 0.5 * (5.0 + __builtin_sqrt(25.0 + 4.0 * t));
+  const long m = (long)__builtin_ceil(r * r);
+  return (int)(m < kp ? m : kp);
+}
+
 // profiling slots
 enum { YK_PROF_FUSED = 0, YK_PROF_NORM = 1, YK_PROF_UPDATE = 2, YK_PROF_MERGE = 3,
        YK_PROF_RERANK = 4, YK_PROF_BATCH = 5, YK_PROF_BF16 = 6, YK_PROF_BF16_SKETCH = 7,

@@ -578,6 +578,11 @@ __global__ __launch_bounds__(256) void yk_sortsel_kernel(const uint64_t *__restr
 // After a MODE 3 pass: queries whose list fitted get their k smallest and
 // become inactive (thr = 0); overflowing queries get the k-th smallest
 // captured key as a tighter threshold, their count is reset, *flag is set.
+// A query with fewer than k keys under a bound (the estimated starting
+// bound, or with NaN distances, which no bound admits, any bound) gets no
+// bound, its count reset and the flag: its next pass admits every value
+// (and overflows, the overflow's bound then being a proven one); a query
+// with fewer than k keys and no bound has no more to find.
 __global__ __launch_bounds__(256) void yk_select_kernel(const uint64_t *__restrict__ cand,
                                                         int *cnt, uint64_t *thr, int k,
                                                         int *out_i, float *out_d, int *flag) {
@@ -585,6 +590,14 @@ __global__ __launch_bounds__(256) void yk_select_kernel(const uint64_t *__restri
   const long q = blockIdx.x;
   if (thr[q] == 0) return;
   const int n = cnt[q];
+  if (n < k && thr[q] != YK_KEY_NONE) {
+    if (threadIdx.x == 0) {
+      thr[q] = YK_KEY_NONE;
+      cnt[q] = 0;
+      atomicOr(flag, 1);
+    }
+    return;
+  }
   const int m = n < YK_CAND_CAP ? n : YK_CAND_CAP;
   int p2 = 1;
   while (p2 < m) p2 <<= 1;
@@ -808,7 +821,9 @@ static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *d
     YK_TRY(yk_scratch(ar, 3, sizeof(uint64_t) * nq * pa.nsplit * 2, &vpart));
     a.part = (uint64_t *)vpart;
     YK_TRY((yk_launch_scan<1, 1>(pa, a, qres)));
-    YK_TRY(yk_sortsel(a.part, nq, pa.nsplit * 2, k, nullptr, nullptr, thr));
+    // the starting bound: the m-th smallest minimum (yk_bound_rank)
+    YK_TRY(yk_sortsel(a.part, nq, pa.nsplit * 2, yk_bound_rank(k, stride), nullptr, nullptr,
+                      thr));
     // (B) filtered scan + per-query sort, retried for overflowing queries
     YkPlan pb = yk_plan(nq, a.nb, 1, 1, 4096, cfg);
     a.thr = thr;
