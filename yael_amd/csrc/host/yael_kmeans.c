@@ -216,13 +216,15 @@ static int yh_kmeans_core(int d, int n, int k, int niter, int flags, int verbose
     }
     g_km_assigned += n;
     g_km_iters++;
-    if ((rc = yk_kmeans_update(n, k, d, m->v, m->assign, m->c, m->counts,
-                               (flags & KMEANS_NORMALIZE_CENTS) != 0, m->ws)) != YK_OK)
+    if ((rc = yk_kmeans_update_async(n, k, d, m->v, m->assign, m->c, m->counts,
+                                     (flags & KMEANS_NORMALIZE_CENTS) != 0, m->ws)) != YK_OK)
       return rc;
-    /* the counts and this assign step's objective (kmeans.c:300-305: the
-       distances to the centroids before the update), one wait */
+    /* the counts, this assign step's objective (kmeans.c:300-305: the
+       distances to the centroids before the update) and the update's check,
+       one wait */
     qerr_old = qerr;
-    if ((rc = yk_kmeans_readback(m->counts, k, hcounts, m->dis, n, &qerr, m->obj)) != YK_OK)
+    if ((rc = yk_kmeans_readback_checked(m->counts, k, hcounts, m->dis, n, &qerr, m->obj,
+                                         m->ws)) != YK_OK)
       return rc;
     int nempty = 0;
     for (int c = 0; c < k; c++) nempty += hcounts[c] == 0;

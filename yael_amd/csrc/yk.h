@@ -144,6 +144,16 @@ int yk_sum_f64(const float *x, long n, double *out_host, void *ws);
  * yk_sum_f64, ws its workspace), with one synchronisation */
 int yk_kmeans_readback(const int *counts, int k, int *counts_host, const float *dis, long n,
                        double *obj_host, void *ws);
+/* The same two steps with one synchronisation between them:
+ * yk_kmeans_update_async enqueues the update without waiting (its check
+ * that every point has a nearest centroid stays on the device), and
+ * yk_kmeans_readback_checked reads that check back with the counts and the
+ * objective (upd_ws: the update's workspace, for n points and k clusters),
+ * returning YK_EINVAL as yk_kmeans_update does when it failed. */
+int yk_kmeans_update_async(long n, int k, int d, const float *v, const int *assign,
+                           float *cent, int *counts, int normalize, void *ws);
+int yk_kmeans_readback_checked(const int *counts, int k, int *counts_host, const float *dis,
+                               long n, double *obj_host, void *ws, void *upd_ws);
 /* k-means++ step: best[j] = min(best[j], squared distance of point j to
  * point s) for j < n (first: best starts at HUGE_VAL); vn: the points'
  * double norms (yk_norms out_d); host_copy (optional, from yk_host_alloc)

@@ -275,8 +275,9 @@ extern "C" int yk_kmeans_readback(const int *counts, int k, int *counts_host, co
   return YK_OK;
 }
 
-extern "C" int yk_kmeans_update(long n, int k, int d, const float *v, const int *assign,
-                                float *cent, int *counts, int normalize, void *ws) {
+// the update's kernels, enqueued; the no-nearest-centroid flag left in w.err
+static int yk_kmeans_update_enqueue(long n, int k, int d, const float *v, const int *assign,
+                                    float *cent, int *counts, int normalize, void *ws) {
   YK_TRY(yk_init());
   KmWs w = yk_ws_layout(ws, n, k);
   hipStream_t st = yk_stream();
@@ -299,13 +300,44 @@ extern "C" int yk_kmeans_update(long n, int k, int d, const float *v, const int 
                        ld, k);
   yk_prof_end(tok);
   YK_HIP(hipGetLastError());
-  int err = 0;
-  YK_HIP(hipMemcpyAsync(&err, w.err, sizeof(int), hipMemcpyDeviceToHost, st));
-  YK_HIP(hipStreamSynchronize(st));
-  if (err)
-    return yk_set_error(YK_EINVAL,
-                        "a point has no nearest centroid (NaN in the input?), kmeans.c:281");
   return YK_OK;
+}
+
+static int yk_kmeans_nearest_error(void) {
+  return yk_set_error(YK_EINVAL,
+                      "a point has no nearest centroid (NaN in the input?), kmeans.c:281");
+}
+
+extern "C" int yk_kmeans_update(long n, int k, int d, const float *v, const int *assign,
+                                float *cent, int *counts, int normalize, void *ws) {
+  YK_TRY(yk_kmeans_update_enqueue(n, k, d, v, assign, cent, counts, normalize, ws));
+  int err = 0;
+  YK_HIP(hipMemcpyAsync(&err, yk_ws_layout(ws, n, k).err, sizeof(int), hipMemcpyDeviceToHost,
+                        yk_stream()));
+  YK_HIP(hipStreamSynchronize(yk_stream()));
+  return err ? yk_kmeans_nearest_error() : YK_OK;
+}
+
+extern "C" int yk_kmeans_update_async(long n, int k, int d, const float *v, const int *assign,
+                                      float *cent, int *counts, int normalize, void *ws) {
+  return yk_kmeans_update_enqueue(n, k, d, v, assign, cent, counts, normalize, ws);
+}
+
+extern "C" int yk_kmeans_readback_checked(const int *counts, int k, int *counts_host,
+                                          const float *dis, long n, double *obj_host, void *ws,
+                                          void *upd_ws) {
+  YK_TRY(yk_init());
+  double *part = (double *)ws;
+  int err = 0;
+  YK_TRY(yk_sum_f64_async(dis, n, part));
+  YK_HIP(hipMemcpyAsync(counts_host, counts, sizeof(int) * (size_t)k, hipMemcpyDeviceToHost,
+                        yk_stream()));
+  YK_HIP(hipMemcpyAsync(obj_host, part + YK_OBJ_BLOCKS, sizeof(double), hipMemcpyDeviceToHost,
+                        yk_stream()));
+  YK_HIP(hipMemcpyAsync(&err, yk_ws_layout(upd_ws, n, k).err, sizeof(int),
+                        hipMemcpyDeviceToHost, yk_stream()));
+  YK_HIP(hipStreamSynchronize(yk_stream()));
+  return err ? yk_kmeans_nearest_error() : YK_OK;
 }
 
 // workspace pointer of the objective partials (for callers that keep the
