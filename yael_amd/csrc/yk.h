@@ -140,16 +140,15 @@ size_t yk_kmeans_workspace_size(long n, int k, int d);
 int yk_kmeans_update(long n, int k, int d, const float *v, const int *assign,
                      float *cent, int *counts, int normalize, void *ws);
 int yk_sum_f64(const float *x, long n, double *out_host, void *ws);
-/* counts_host[k] = counts, *obj_host = the FP64 objective of dis (as
- * yk_sum_f64, ws its workspace), with one synchronisation */
-int yk_kmeans_readback(const int *counts, int k, int *counts_host, const float *dis, long n,
-                       double *obj_host, void *ws);
-/* The same two steps with one synchronisation between them:
+/* The update and the readback of an iteration with one synchronisation:
  * yk_kmeans_update_async enqueues the update without waiting (its check
  * that every point has a nearest centroid stays on the device), and
- * yk_kmeans_readback_checked reads that check back with the counts and the
- * objective (upd_ws: the update's workspace, for n points and k clusters),
- * returning YK_EINVAL as yk_kmeans_update does when it failed. */
+ * yk_kmeans_readback_checked reads that check back with counts_host[k] =
+ * counts and *obj_host = the FP64 objective of dis (as yk_sum_f64, ws its
+ * workspace; upd_ws: the update's workspace, for n points and k clusters),
+ * returning YK_EINVAL as yk_kmeans_update does when it failed.  The values
+ * pass through a pinned buffer of the library's (one per process, grown to
+ * the largest k, calls serialised by a lock). */
 int yk_kmeans_update_async(long n, int k, int d, const float *v, const int *assign,
                            float *cent, int *counts, int normalize, void *ws);
 int yk_kmeans_readback_checked(const int *counts, int k, int *counts_host, const float *dis,

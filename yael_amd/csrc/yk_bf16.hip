@@ -1956,8 +1956,7 @@ static int yk_bf16_candidates(Bf16Args a, int kp, uint64_t *cand_out, int *ovf, 
     YK_HIP(hipGetLastError());
     if (!resume) break;
     int hflag = 0;
-    YK_HIP(hipMemcpyAsync(&hflag, ovf, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
-    YK_HIP(hipStreamSynchronize(yk_stream()));
+    YK_TRY(yk_read_small(&hflag, ovf, sizeof(int)));
     if (!hflag) break;
     if (tries == 5)
       return yk_set_error(YK_EINVAL, "k'=%d: too many tied first-pass distances", kp);
@@ -2180,8 +2179,7 @@ static int yk_knn_bf16_run(int type, long nq, long nb, int d, int k, int kp, con
     };
     YK_TRY(rerank());
     int hv[2] = {0, 0};
-    YK_HIP(hipMemcpyAsync(hv, nunc, sizeof(hv), hipMemcpyDeviceToHost, yk_stream()));
-    YK_HIP(hipStreamSynchronize(yk_stream()));
+    YK_TRY(yk_read_small(hv, nunc, sizeof(hv)));
     if (hv[1]) {
       // a query's capture overflowed (massive ties): its list was empty in
       // this re-rank; the filtered pass again for such queries, then the
@@ -2189,8 +2187,7 @@ static int yk_knn_bf16_run(int type, long nq, long nb, int d, int k, int kp, con
       YK_TRY(yk_bf16_candidates(a, kp, (uint64_t *)vcand, ovf, true));
       hipLaunchKernelGGL(yk_bf_zero_kernel, dim3(1), dim3(64), 0, yk_stream(), nunc, 1L);
       YK_TRY(rerank());
-      YK_HIP(hipMemcpyAsync(hv, nunc, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
-      YK_HIP(hipStreamSynchronize(yk_stream()));
+      YK_TRY(yk_read_small(hv, nunc, sizeof(int)));
     }
     const int hn = hv[0];
     // Uncertified queries, first level: the precision form again on just

@@ -10,6 +10,9 @@
 int yk_set_error(int code, const char *fmt, ...);
 int yk_hip_fail(hipError_t e, const char *what);
 hipStream_t yk_stream(void);
+// dst = up to 256 bytes of device memory at src, after the stream's queued
+// work, with one wait
+int yk_read_small(void *dst, const void *src, size_t bytes);
 int yk_prof_setup(void);
 // Workgroups of `block` threads that fit on the whole device at once for
 // kernel `fn` (occupancy x CU count), at least 8 and a multiple of 8.

@@ -9,6 +9,9 @@
 // loop (kmeans.c:280-283), only in FP64 instead of FP32; the objective is a
 // fixed-shape tree reduction.  Two runs on the same inputs give the same bits.
 #include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "yk.h"
@@ -252,26 +255,7 @@ extern "C" int yk_sum_f64(const float *x, long n, double *out_host, void *ws) {
   YK_TRY(yk_init());
   double *part = (double *)ws;
   YK_TRY(yk_sum_f64_async(x, n, part));
-  YK_HIP(hipMemcpyAsync(out_host, part + YK_OBJ_BLOCKS, sizeof(double), hipMemcpyDeviceToHost,
-                        yk_stream()));
-  YK_HIP(hipStreamSynchronize(yk_stream()));
-  return YK_OK;
-}
-
-// The counts of an update and the objective of its assign step read back
-// with one wait: the objective's kernels are queued behind the update, then
-// both copies, then one synchronisation (ws: the objective's workspace, as
-// for yk_sum_f64).
-extern "C" int yk_kmeans_readback(const int *counts, int k, int *counts_host, const float *dis,
-                                  long n, double *obj_host, void *ws) {
-  YK_TRY(yk_init());
-  double *part = (double *)ws;
-  YK_TRY(yk_sum_f64_async(dis, n, part));
-  YK_HIP(hipMemcpyAsync(counts_host, counts, sizeof(int) * (size_t)k, hipMemcpyDeviceToHost,
-                        yk_stream()));
-  YK_HIP(hipMemcpyAsync(obj_host, part + YK_OBJ_BLOCKS, sizeof(double), hipMemcpyDeviceToHost,
-                        yk_stream()));
-  YK_HIP(hipStreamSynchronize(yk_stream()));
+  YK_TRY(yk_read_small(out_host, part + YK_OBJ_BLOCKS, sizeof(double)));
   return YK_OK;
 }
 
@@ -303,6 +287,12 @@ static int yk_kmeans_update_enqueue(long n, int k, int d, const float *v, const 
   return YK_OK;
 }
 
+// pinned landing buffer of the per-iteration readback (grown on demand,
+// kept until the process ends)
+static std::mutex g_rb_mu;
+static void *g_rb = nullptr;
+static size_t g_rb_bytes = 0;
+
 static int yk_kmeans_nearest_error(void) {
   return yk_set_error(YK_EINVAL,
                       "a point has no nearest centroid (NaN in the input?), kmeans.c:281");
@@ -312,9 +302,7 @@ extern "C" int yk_kmeans_update(long n, int k, int d, const float *v, const int 
                                 float *cent, int *counts, int normalize, void *ws) {
   YK_TRY(yk_kmeans_update_enqueue(n, k, d, v, assign, cent, counts, normalize, ws));
   int err = 0;
-  YK_HIP(hipMemcpyAsync(&err, yk_ws_layout(ws, n, k).err, sizeof(int), hipMemcpyDeviceToHost,
-                        yk_stream()));
-  YK_HIP(hipStreamSynchronize(yk_stream()));
+  YK_TRY(yk_read_small(&err, yk_ws_layout(ws, n, k).err, sizeof(int)));
   return err ? yk_kmeans_nearest_error() : YK_OK;
 }
 
@@ -328,15 +316,32 @@ extern "C" int yk_kmeans_readback_checked(const int *counts, int k, int *counts_
                                           void *upd_ws) {
   YK_TRY(yk_init());
   double *part = (double *)ws;
-  int err = 0;
   YK_TRY(yk_sum_f64_async(dis, n, part));
-  YK_HIP(hipMemcpyAsync(counts_host, counts, sizeof(int) * (size_t)k, hipMemcpyDeviceToHost,
+  // The three values land in pinned memory: copies into pageable memory go
+  // through a bounce buffer one at a time, each waiting for the stream.
+  // Layout: objective (8 bytes), error flag (4), pad (4), counts (4 k).
+  std::lock_guard<std::mutex> lock(g_rb_mu);
+  const size_t need = 16 + sizeof(int) * (size_t)k;
+  if (need > g_rb_bytes) {
+    if (g_rb) YK_HIP(hipHostFree(g_rb));
+    g_rb = nullptr;
+    g_rb_bytes = 0;
+    const size_t sz = need < 4096 ? 4096 : need;
+    YK_HIP(hipHostMalloc(&g_rb, sz, hipHostMallocDefault));
+    g_rb_bytes = sz;
+  }
+  char *rb = (char *)g_rb;
+  YK_HIP(hipMemcpyAsync(rb, part + YK_OBJ_BLOCKS, sizeof(double), hipMemcpyDeviceToHost,
                         yk_stream()));
-  YK_HIP(hipMemcpyAsync(obj_host, part + YK_OBJ_BLOCKS, sizeof(double), hipMemcpyDeviceToHost,
-                        yk_stream()));
-  YK_HIP(hipMemcpyAsync(&err, yk_ws_layout(upd_ws, n, k).err, sizeof(int),
+  YK_HIP(hipMemcpyAsync(rb + 8, yk_ws_layout(upd_ws, n, k).err, sizeof(int),
                         hipMemcpyDeviceToHost, yk_stream()));
+  YK_HIP(hipMemcpyAsync(rb + 16, counts, sizeof(int) * (size_t)k, hipMemcpyDeviceToHost,
+                        yk_stream()));
   YK_HIP(hipStreamSynchronize(yk_stream()));
+  int err;
+  memcpy(obj_host, rb, sizeof(double));
+  memcpy(&err, rb + 8, sizeof(int));
+  memcpy(counts_host, rb + 16, sizeof(int) * (size_t)k);
   return err ? yk_kmeans_nearest_error() : YK_OK;
 }
 

@@ -139,9 +139,28 @@ extern "C" int yk_host_free(void *p) {
 extern "C" int yk_memcpy_d2h(void *dst, const void *src, size_t bytes) {
   YK_TRY(yk_init());
   if (!bytes) return YK_OK;
+  if (bytes <= 256) return yk_read_small(dst, src, bytes);
   YK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
   YK_HIP(hipStreamSynchronize(g_stream));
   return YK_OK;
+}
+
+// A few device values (flags, counters: at most 256 bytes) read back with
+// one wait: the copy lands in a pinned buffer behind the queued work.  A copy
+// into pageable memory instead waits for the stream, then goes through a
+// bounce buffer with a wait of its own.
+int yk_read_small(void *dst, const void *src, size_t bytes) {
+  static pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+  static void *buf = nullptr;
+  if (bytes > 256) return yk_set_error(YK_EINVAL, "yk_read_small: %zu bytes > 256", bytes);
+  pthread_mutex_lock(&mu);
+  hipError_t e = hipSuccess;
+  if (!buf) e = hipHostMalloc(&buf, 256, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMemcpyAsync(buf, src, bytes, hipMemcpyDeviceToHost, g_stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(g_stream);
+  if (e == hipSuccess) memcpy(dst, buf, bytes);
+  pthread_mutex_unlock(&mu);
+  return e == hipSuccess ? YK_OK : yk_hip_fail(e, "small readback");
 }
 
 extern "C" int yk_memcpy_d2d(void *dst, const void *src, size_t bytes) {

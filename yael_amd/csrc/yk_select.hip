@@ -841,8 +841,7 @@ static int yk_knn_large_k(ScanArgs base, int k, bool qres, int *assign, float *d
       yk_prof_end(tok);
       YK_HIP(hipGetLastError());
       int hflag = 0;
-      YK_HIP(hipMemcpyAsync(&hflag, vflag, sizeof(int), hipMemcpyDeviceToHost, yk_stream()));
-      YK_HIP(hipStreamSynchronize(yk_stream()));
+      YK_TRY(yk_read_small(&hflag, vflag, sizeof(int)));
       if (!hflag) break;
       if (++tries == 6)
         return yk_set_error(YK_EINVAL,
