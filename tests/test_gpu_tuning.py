@@ -56,10 +56,10 @@ def test_tuned_plans_give_the_default_plans_bits(tmp_path):
     print("\n".join(lines), flush=True)
     # shape classes with more than one distinct candidate plan were tuned
     # (the others run their only plan) and recorded with a plan index
-    # (plan indices: the FP32 form has 3 work-item targets x 3 sketch-stride
-    # scales, the precision form 3 x 4)
+    # (plan indices: 3 work-item targets x 4 sketch-stride scales in both
+    # forms)
     rec = [x.split() for x in lines if x.strip()]
-    nplans = {"f32": 9, "bf16": 12}
-    assert all(len(r) == 2 and 0 <= int(r[1]) < nplans.get(r[0].split(":")[0], 9) for r in rec)
+    nplans = {"f32": 12, "bf16": 12}
+    assert all(len(r) == 2 and 0 <= int(r[1]) < nplans.get(r[0].split(":")[0], 12) for r in rec)
     assert sum(r[0].startswith("f32:") for r in rec) >= 2
     assert sum(r[0].startswith("bf16:") for r in rec) >= 1

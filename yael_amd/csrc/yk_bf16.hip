@@ -1728,7 +1728,7 @@ struct BPlan {
 // default plan.  The plan of the search in progress (calls are serialised
 // by the library lock; the nested escalation search keeps its caller's).
 static const long yk_bitem_targets[3] = {4096, 2048, 8192};
-static const int yk_bstride_scale[4][2] = {{1, 1}, {8, 3}, {2, 1}, {3, 2}};
+static const int yk_bstride_scale[4][2] = {{1, 1}, {8, 3}, {16, 3}, {3, 2}};
 #define YK_BNCFG 12
 static int g_bcfg = 0;
 
@@ -1737,7 +1737,8 @@ static int g_bcfg = 0;
 static long yk_bsketch_stride(long nb, int kp) {
   const int *sc = yk_bstride_scale[(g_bcfg / 3) % 4];
   long stride = 600L * sc[0] / sc[1] / kp;
-  if (stride > 8) stride = 8;
+  const long cap = sc[0] == 16 ? 16 : 8;  // the 16/3 scale may sample 1 in 16
+  if (stride > cap) stride = cap;
   if (stride < 1) stride = 1;
   const long nblocks = (nb + 31) / 32;
   while (stride > 1 && nblocks / stride < 2L * kp) stride >>= 1;

@@ -641,8 +641,8 @@ struct YkPlan {
 // scale of the k > 16 sketch's sampling stride.  cfg = target index + 3 x
 // stride index; 0 is the default plan.
 static const long yk_item_targets[3] = {4096, 2048, 8192};
-static const int yk_stride_scale[3][2] = {{1, 1}, {1, 2}, {2, 1}};  // numerator, denominator
-#define YK_NCFG 9
+static const int yk_stride_scale[4][2] = {{1, 1}, {1, 2}, {2, 1}, {16, 3}};  // num., denom.
+#define YK_NCFG 12
 
 // Split the base so that tiles * nsplit reaches the item target (~8 per
 // resident workgroup by default), with at least min_rows rows per split and
@@ -778,9 +778,10 @@ static int yk_sortsel(const uint64_t *src, long nq, int n, int k, int *oi, float
 // global rank 1.4 * stride * k, the expected candidate count; ~800 keeps
 // the filtered pass's append path rare and the list far from overflow.
 static long yk_sketch_stride(long nb, int k, int cfg = 0) {
-  const int *sc = yk_stride_scale[(cfg / 3) % 3];
+  const int *sc = yk_stride_scale[(cfg / 3) % 4];
   long stride = 600L * sc[0] / sc[1] / k;
-  if (stride > 8) stride = 8;
+  const long cap = sc[0] == 16 ? 16 : 8;  // the last scale may sample 1 in 16
+  if (stride > cap) stride = cap;
   if (stride < 1) stride = 1;
   const long nblocks = (nb + 31) / 32;
   while (stride > 1 && nblocks / stride < 2L * k) stride >>= 1;
