@@ -146,9 +146,9 @@ int yk_sum_f64(const float *x, long n, double *out_host, void *ws);
  * yk_kmeans_readback_checked reads that check back with counts_host[k] =
  * counts and *obj_host = the FP64 objective of dis (as yk_sum_f64, ws its
  * workspace; upd_ws: the update's workspace, for n points and k clusters),
- * returning YK_EINVAL as yk_kmeans_update does when it failed.  The values
- * pass through a pinned buffer of the library's (one per process, grown to
- * the largest k, calls serialised by a lock). */
+ * returning YK_EINVAL as yk_kmeans_update does when it failed.  One kernel
+ * writes the values into a mapped pinned buffer of the library's (one per
+ * process, grown to the largest k, calls serialised by a lock). */
 int yk_kmeans_update_async(long n, int k, int d, const float *v, const int *assign,
                            float *cent, int *counts, int normalize, void *ws);
 int yk_kmeans_readback_checked(const int *counts, int k, int *counts_host, const float *dis,

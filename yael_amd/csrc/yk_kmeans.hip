@@ -71,14 +71,13 @@ extern "C" size_t yk_kmeans_workspace_size(long n, int k, int d) {
          yk_align(sizeof(double) * (YK_OBJ_BLOCKS + 1)) + 256 + yk_align(yk_sort_tmp_bytes(n, k));
 }
 
-__global__ void yk_iota_kernel(int *p, long n) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) p[i] = (int)i;
-}
-
-__global__ void yk_zero_i32_kernel(int *p, long n) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+// the update's set-up in one launch: counts[0..k) = 0, *err = 0,
+// iota[i] = i for i < n (the sort's values)
+__global__ void yk_upd_prep_kernel(int *counts, int k, int *err, int *iota, long n) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < k) counts[i] = 0;
+  if (i < n) iota[i] = (int)i;
+  if (i == 0) *err = 0;
 }
 
 __global__ void yk_hist_kernel(const int *__restrict__ assign, long n, int k, int *counts,
@@ -267,10 +266,9 @@ static int yk_kmeans_update_enqueue(long n, int k, int d, const float *v, const 
   hipStream_t st = yk_stream();
   const int ld = yk_ld(d);
   int tok = yk_prof_begin(YK_PROF_UPDATE);
-  hipLaunchKernelGGL(yk_zero_i32_kernel, dim3((k + 255) / 256), dim3(256), 0, st, counts, (long)k);
-  hipLaunchKernelGGL(yk_zero_i32_kernel, dim3(1), dim3(64), 0, st, w.err, 1L);
-  hipLaunchKernelGGL(yk_iota_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, w.iota,
-                     n);
+  const long nprep = n > k ? n : k;
+  hipLaunchKernelGGL(yk_upd_prep_kernel, dim3((unsigned)((nprep + 255) / 256)), dim3(256), 0, st,
+                     counts, k, w.err, w.iota, n);
   hipLaunchKernelGGL(yk_hist_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, assign,
                      n, k, counts, w.err);
   hipLaunchKernelGGL(yk_scan_kernel, dim3(1), dim3(1024), 0, st, counts, k, w.offsets);
@@ -290,8 +288,22 @@ static int yk_kmeans_update_enqueue(long n, int k, int d, const float *v, const 
 // pinned landing buffer of the per-iteration readback (grown on demand,
 // kept until the process ends)
 static std::mutex g_rb_mu;
-static void *g_rb = nullptr;
+static void *g_rb = nullptr, *g_rb_dev = nullptr;  // host and device addresses
 static size_t g_rb_bytes = 0;
+
+// counts, the objective and the update's check into the readback buffer
+// (layout as in yk_kmeans_readback_checked)
+__global__ __launch_bounds__(256) void yk_rb_gather_kernel(const int *__restrict__ counts, int k,
+                                                           const double *__restrict__ obj,
+                                                           const int *__restrict__ err,
+                                                           char *__restrict__ rb) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < k) ((int *)(rb + 16))[i] = counts[i];
+  if (i == 0) {
+    *(double *)rb = *obj;
+    *(int *)(rb + 8) = *err;
+  }
+}
 
 static int yk_kmeans_nearest_error(void) {
   return yk_set_error(YK_EINVAL,
@@ -317,26 +329,26 @@ extern "C" int yk_kmeans_readback_checked(const int *counts, int k, int *counts_
   YK_TRY(yk_init());
   double *part = (double *)ws;
   YK_TRY(yk_sum_f64_async(dis, n, part));
-  // The three values land in pinned memory: copies into pageable memory go
-  // through a bounce buffer one at a time, each waiting for the stream.
-  // Layout: objective (8 bytes), error flag (4), pad (4), counts (4 k).
+  // The three values are written by one kernel straight into mapped pinned
+  // host memory (three copies would be three more launches, and copies into
+  // pageable memory go through a bounce buffer one at a time).  Layout:
+  // objective (8 bytes), error flag (4), pad (4), counts (4 k).
   std::lock_guard<std::mutex> lock(g_rb_mu);
   const size_t need = 16 + sizeof(int) * (size_t)k;
   if (need > g_rb_bytes) {
     if (g_rb) YK_HIP(hipHostFree(g_rb));
-    g_rb = nullptr;
+    g_rb = g_rb_dev = nullptr;
     g_rb_bytes = 0;
     const size_t sz = need < 4096 ? 4096 : need;
-    YK_HIP(hipHostMalloc(&g_rb, sz, hipHostMallocDefault));
+    YK_HIP(hipHostMalloc(&g_rb, sz, hipHostMallocCoherent | hipHostMallocMapped));
     g_rb_bytes = sz;
+    YK_HIP(hipHostGetDevicePointer(&g_rb_dev, g_rb, 0));
   }
   char *rb = (char *)g_rb;
-  YK_HIP(hipMemcpyAsync(rb, part + YK_OBJ_BLOCKS, sizeof(double), hipMemcpyDeviceToHost,
-                        yk_stream()));
-  YK_HIP(hipMemcpyAsync(rb + 8, yk_ws_layout(upd_ws, n, k).err, sizeof(int),
-                        hipMemcpyDeviceToHost, yk_stream()));
-  YK_HIP(hipMemcpyAsync(rb + 16, counts, sizeof(int) * (size_t)k, hipMemcpyDeviceToHost,
-                        yk_stream()));
+  hipLaunchKernelGGL(yk_rb_gather_kernel, dim3((unsigned)((k + 255) / 256)), dim3(256), 0,
+                     yk_stream(), counts, k, part + YK_OBJ_BLOCKS, yk_ws_layout(upd_ws, n, k).err,
+                     (char *)g_rb_dev);
+  YK_HIP(hipGetLastError());
   YK_HIP(hipStreamSynchronize(yk_stream()));
   int err;
   memcpy(obj_host, rb, sizeof(double));
