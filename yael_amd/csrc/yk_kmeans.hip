@@ -26,9 +26,19 @@ struct KmWs {
   int *iota;       // n
   double *obj;     // YK_OBJ_BLOCKS + 1
   int *err;        // 1
+  int *mat;        // counting sort: k x nseg segment counts (yk_cs_ok shapes only)
   void *sort_tmp;  // rocprim temp
   size_t sort_bytes;
 };
+
+// The update's cluster-major permutation by a stable counting sort where
+// the per-segment count matrix is small (k <= 2048 clusters, k x segments
+// <= 2^17 entries, segments of YK_CS_SEG points, one wave each), else by
+// rocprim's LSD radix sort of (cluster, point) pairs.  Both list each
+// cluster's points in ascending order.
+#define YK_CS_SEG 1024
+static long yk_cs_nseg(long n) { return (n + YK_CS_SEG - 1) / YK_CS_SEG; }
+static bool yk_cs_ok(long n, int k) { return k <= 2048 && (long)k * yk_cs_nseg(n) <= (1L << 17); }
 
 static size_t yk_align(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -60,6 +70,8 @@ static KmWs yk_ws_layout(void *base, long n, int k) {
   p += yk_align(sizeof(double) * (YK_OBJ_BLOCKS + 1));
   w.err = (int *)p;
   p += 256;
+  w.mat = (int *)p;
+  if (yk_cs_ok(n, k)) p += yk_align(sizeof(int) * (size_t)k * yk_cs_nseg(n));
   w.sort_bytes = yk_sort_tmp_bytes(n, k);
   w.sort_tmp = p;
   return w;
@@ -68,7 +80,159 @@ static KmWs yk_ws_layout(void *base, long n, int k) {
 extern "C" size_t yk_kmeans_workspace_size(long n, int k, int d) {
   (void)d;
   return yk_align(sizeof(int) * (k + 1)) + 3 * yk_align(sizeof(int) * n) +
-         yk_align(sizeof(double) * (YK_OBJ_BLOCKS + 1)) + 256 + yk_align(yk_sort_tmp_bytes(n, k));
+         yk_align(sizeof(double) * (YK_OBJ_BLOCKS + 1)) + 256 +
+         (yk_cs_ok(n, k) ? yk_align(sizeof(int) * (size_t)k * yk_cs_nseg(n)) : 0) +
+         yk_align(yk_sort_tmp_bytes(n, k));
+}
+
+// Counting sort, pass 1: wave w of the grid counts the clusters of segment
+// w (points [w YK_CS_SEG, +YK_CS_SEG), all its loads issued at once) in its
+// own LDS histogram and writes them segment-major, mat[w * k + c]; an
+// out-of-range cluster sets err.
+__global__ __launch_bounds__(256) void yk_cs_hist_kernel(const int *__restrict__ assign, long n,
+                                                         int k, long nseg, int *__restrict__ mat,
+                                                         int *err) {
+  extern __shared__ int lh[];  // 4 waves x k
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long seg = blockIdx.x * 4L + wave;
+  if (seg >= nseg) return;
+  int *h = lh + wave * k;
+  for (int c = lane; c < k; c += 64) h[c] = 0;
+  const long p0 = seg * YK_CS_SEG;
+  int a[YK_CS_SEG / 64];
+#pragma unroll
+  for (int j = 0; j < YK_CS_SEG / 64; j++) {
+    const long p = p0 + 64 * j + lane;
+    a[j] = p < n ? assign[p] : 0;
+  }
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < YK_CS_SEG / 64; j++) {
+    if (p0 + 64 * j + lane >= n) continue;
+    if ((unsigned)a[j] < (unsigned)k)
+      atomicAdd(&h[a[j]], 1);
+    else
+      bad = true;
+  }
+  if (__any(bad) && lane == 0) atomicOr(err, 1);
+  for (int c = lane; c < k; c += 64) mat[seg * k + c] = h[c];
+}
+
+// Pass 2, one workgroup: per cluster (thread c, and c + 1024) the prefix of
+// its counts over the segments, in place (loads eight segments at a time),
+// then the clusters' starts offsets[0..k] (exclusive scan of their totals,
+// in LDS) and sizes counts[0..k).
+__global__ __launch_bounds__(1024) void yk_cs_scan_kernel(int *__restrict__ mat, int k, long nseg,
+                                                          int *__restrict__ offsets,
+                                                          int *__restrict__ counts) {
+  __shared__ int part[2048];
+  const int t = threadIdx.x;
+  for (int c = t; c < 2048; c += 1024) {
+    int run = 0;
+    if (c < k) {
+      long s = 0;
+      for (; s + 8 <= nseg; s += 8) {
+        int v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = mat[(s + u) * k + c];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          mat[(s + u) * k + c] = run;
+          run += v[u];
+        }
+      }
+      for (; s < nseg; s++) {
+        const int v = mat[s * k + c];
+        mat[s * k + c] = run;
+        run += v;
+      }
+      counts[c] = run;
+    }
+    part[c] = run;
+  }
+  __syncthreads();
+  // exclusive scan of part[0..2048): each thread two entries
+  const int x0 = part[2 * t], x1 = part[2 * t + 1];
+  __syncthreads();
+  int v = x0 + x1;
+  part[t] = v;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int u = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += u;
+    __syncthreads();
+  }
+  const int before = part[t] - v;  // exclusive prefix of the pair
+  if (2 * t < k) offsets[2 * t] = before;
+  if (2 * t + 1 < k) offsets[2 * t + 1] = before + x0;
+  if (t == 1023) offsets[k] = part[1023];
+}
+
+// Pass 3: wave w places segment w's points.  Per 64-point chunk (all the
+// segment's clusters loaded at once), a bitonic sort of the keys (cluster
+// << 6 | lane) puts the chunk's points with equal clusters next to each
+// other in lane order; a point's place in its cluster is the cluster's
+// start, plus the earlier segments' counts of it, plus this segment's
+// running count of it (LDS, per wave, updated chunk by chunk in order),
+// plus its rank in its run: the sort is stable.  The chunks' ranks are
+// computed first and their gathers and stores issued together.
+__global__ __launch_bounds__(256) void yk_cs_scatter_kernel(const int *__restrict__ assign, long n,
+                                                            int k, long nseg,
+                                                            const int *__restrict__ mat,
+                                                            const int *__restrict__ offsets,
+                                                            int *__restrict__ perm) {
+  extern __shared__ int lr[];  // 4 waves x k
+  constexpr int NJ = YK_CS_SEG / 64;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long seg = blockIdx.x * 4L + wave;
+  if (seg >= nseg) return;
+  int *run = lr + wave * k;
+  for (int c = lane; c < k; c += 64) run[c] = 0;
+  const long p0 = seg * YK_CS_SEG;
+  unsigned key[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    const long p = p0 + 64 * j + lane;
+    const int a = p < n ? assign[p] : -1;
+    key[j] = (unsigned)a < (unsigned)k ? ((unsigned)a << 6) | (unsigned)lane : 0xffffffffu;
+  }
+  int pos[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    unsigned kk = key[j];
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        const unsigned other = __shfl_xor(kk, stride);
+        const bool up = (lane & size) == 0, lower = (lane & stride) == 0;
+        kk = (lower == up) ? min(kk, other) : max(kk, other);
+      }
+    }
+    key[j] = kk;
+    const bool valid = kk != 0xffffffffu;
+    const int c = (int)(kk >> 6);
+    const int prev = __shfl_up(c, 1), next = __shfl_down(c, 1);
+    const bool start = lane == 0 || prev != c, end = lane == 63 || next != c;
+    int st = start ? lane : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(st, o);
+      if (lane >= o) st = max(st, v);
+    }
+    const int rank = lane - st;
+    int base = 0;
+    if (valid) base = run[c];
+    if (valid && end) run[c] = base + rank + 1;
+    pos[j] = base + rank;
+  }
+#pragma unroll
+  for (int j = 0; j < NJ; j++) {
+    if (key[j] == 0xffffffffu) continue;
+    const int c = (int)(key[j] >> 6);
+    perm[offsets[c] + mat[seg * k + c] + pos[j]] = (int)(p0 + 64 * j + (key[j] & 63));
+  }
 }
 
 // the update's set-up in one launch: counts[0..k) = 0, *err = 0,
@@ -266,15 +430,31 @@ static int yk_kmeans_update_enqueue(long n, int k, int d, const float *v, const 
   hipStream_t st = yk_stream();
   const int ld = yk_ld(d);
   int tok = yk_prof_begin(YK_PROF_UPDATE);
-  const long nprep = n > k ? n : k;
+  const bool cs = yk_cs_ok(n, k);
+  const long niota = cs ? 0 : n;  // the counting sort needs no sort values
+  const long nprep = niota > k ? niota : k;
   hipLaunchKernelGGL(yk_upd_prep_kernel, dim3((unsigned)((nprep + 255) / 256)), dim3(256), 0, st,
-                     counts, k, w.err, w.iota, n);
-  hipLaunchKernelGGL(yk_hist_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, assign,
-                     n, k, counts, w.err);
-  hipLaunchKernelGGL(yk_scan_kernel, dim3(1), dim3(1024), 0, st, counts, k, w.offsets);
-  size_t bytes = w.sort_bytes;
-  YK_HIP(rocprim::radix_sort_pairs(w.sort_tmp, bytes, assign, w.keys_out, w.iota, w.perm,
-                                   (size_t)n, 0, yk_bits_for(k), st));
+                     counts, k, w.err, w.iota, niota);
+  if (cs) {
+    const long nseg = yk_cs_nseg(n);
+    const unsigned g = (unsigned)((nseg + 3) / 4);
+    const size_t lds = sizeof(int) * 4 * (size_t)k;
+    if (nseg > 0)
+      hipLaunchKernelGGL(yk_cs_hist_kernel, dim3(g), dim3(256), lds, st, assign, n, k, nseg, w.mat,
+                         w.err);
+    hipLaunchKernelGGL(yk_cs_scan_kernel, dim3(1), dim3(1024), 0, st, w.mat, k, nseg, w.offsets,
+                       counts);
+    if (nseg > 0)
+      hipLaunchKernelGGL(yk_cs_scatter_kernel, dim3(g), dim3(256), lds, st, assign, n, k, nseg,
+                         w.mat, w.offsets, w.perm);
+  } else {
+    hipLaunchKernelGGL(yk_hist_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, assign,
+                       n, k, counts, w.err);
+    hipLaunchKernelGGL(yk_scan_kernel, dim3(1), dim3(1024), 0, st, counts, k, w.offsets);
+    size_t bytes = w.sort_bytes;
+    YK_HIP(rocprim::radix_sort_pairs(w.sort_tmp, bytes, assign, w.keys_out, w.iota, w.perm,
+                                     (size_t)n, 0, yk_bits_for(k), st));
+  }
   hipLaunchKernelGGL(yk_segsum_kernel, dim3((k + 3) / 4), dim3(256), 0, st, v, d, ld, k,
                      w.offsets, w.perm, cent, normalize);
   if (normalize && d > 1024)
