@@ -3,8 +3,9 @@
 // segmented centroid sums in point order, scaling by the double reciprocal
 // of the count, optional L2 normalisation, and the FP64 objective.
 //
-// Determinism: the histogram uses integer atomics (order-free); the
-// permutation is a stable LSD radix sort of (cluster, point) pairs, so every
+// Determinism: the histograms use integer atomics (order-free); the
+// permutation is a stable sort of the points by cluster (a counting sort
+// for small k, else an LSD radix sort of (cluster, point) pairs), so every
 // cluster sums its points in ascending point order, like the reference's
 // loop (kmeans.c:280-283), only in FP64 instead of FP32; the objective is a
 // fixed-shape tree reduction.  Two runs on the same inputs give the same bits.
