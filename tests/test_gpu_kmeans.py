@@ -70,6 +70,25 @@ def test_first_step_bitexact_integer_data(ya, oracle):
     assert g[0] == o[0]
 
 
+@pytest.mark.parametrize("n,d,k", [(5000, 16, 2048), (5000, 16, 2049), (140000, 8, 900),
+                                   (140000, 8, 1000)])
+def test_update_permutation_paths_bitexact(ya, oracle, n, d, k):
+    """The update orders each cluster's points by a counting sort for k <=
+    2048 and k * ceil(n / 1024) <= 2^17, else by a radix sort: one
+    user-initialised step on integer data on each side of both limits is
+    bit-exact against the oracle (counts, centroids, empty-cluster splits)."""
+    rng = np.random.default_rng(n + k)
+    v = mixture(rng, n, d, 64, 20)
+    init = v[rng.choice(n, k, replace=False)].copy()
+    print("step n=%d d=%d k=%d" % (n, d, k), flush=True)
+    g = gpu_kmeans(ya, v, k, 1, init=init)
+    o = oracle.kmeans(v, k, 1, flags=QUIET | INIT_USER, seed=1, init=init)
+    assert np.array_equal(g[3], o[3]), "assign"
+    assert np.array_equal(g[4], o[4]), "nassign"
+    assert np.array_equal(g[1].view(np.int32), o[1].view(np.int32)), "centroids"
+    assert g[0] == o[0]
+
+
 def test_stepwise_parity_float_data(ya, oracle):
     """Each GPU iteration vs the oracle's iteration from the same centroids."""
     rng = np.random.default_rng(2)
