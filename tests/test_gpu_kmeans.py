@@ -115,12 +115,11 @@ def test_stepwise_parity_config0_size(ya, oracle):
     sigma 20): four GPU iterations, each against the oracle's iteration from
     the same centroids (assignment equal up to near-ties, counts equal when
     there are none, centroids within FP32-vs-FP64 rounding), then a whole
-    seeded 10-iteration run against the oracle's run.  Whole runs diverge:
-    a split's twins are near-ties, one assignment that flips changes a
-    count, and the next split draws another cluster, so the two runs reach
-    different local optima (at this seed the GPU's objective is 3.3% lower).
-    The bound: the GPU run is not worse than the oracle's by more than 1%,
-    and within 5% of it either way."""
+    seeded 10-iteration run against the oracle's run from the same seed
+    (seed 1: seed 0 would draw each side's seed from its own lrand48
+    state).  Whole runs may still diverge: a split's twins are near-ties,
+    one assignment that flips changes a count, and a later split then draws
+    another cluster.  The bound: the objectives agree within 1%."""
     rng = np.random.default_rng(5)
     v = mixture(rng, 100000, 128, 1024, 20)
     cent = v[rng.choice(len(v), 1024, replace=False)].copy()
@@ -135,11 +134,10 @@ def test_stepwise_parity_config0_size(ya, oracle):
             np.testing.assert_allclose(g[1], o[1], rtol=2e-6, atol=1e-4)
         cent = g[1]
     print("config-0 whole run", flush=True)
-    g = gpu_kmeans(ya, v, 1024, 10, seed=0)
-    o = oracle.kmeans(v, 1024, 10, flags=QUIET, seed=0)
+    g = gpu_kmeans(ya, v, 1024, 10, seed=1)
+    o = oracle.kmeans(v, 1024, 10, flags=QUIET, seed=1)
     print("objective: GPU %.1f, oracle %.1f" % (g[0], o[0]), flush=True)
-    assert g[0] <= 1.01 * o[0], (g[0], o[0])
-    assert abs(g[0] - o[0]) <= 0.05 * abs(o[0]), (g[0], o[0])
+    assert abs(g[0] - o[0]) <= 0.01 * abs(o[0]), (g[0], o[0])
 
 
 @pytest.mark.parametrize("case", ["km_plain", "km_redo", "km_norm"])
