@@ -714,7 +714,7 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="profile runs: first copy this many MiB three times with a plain "
                          "copy kernel (the reference point of the HBM traffic counters)")
-    ap.add_argument("--only", choices=["kmeans-large", "d960", "d960-bf16", "batched"],
+    ap.add_argument("--only", choices=["kmeans", "kmeans-large", "d960", "d960-bf16", "batched"],
                     default=None,
                     help="profile runs: run one extra workload alone and print its record")
     ap.add_argument("--nb", type=int, default=1_000_000)
@@ -744,7 +744,9 @@ def main():
         dst.free()
 
     if args.only:
-        only = {"kmeans-large": bench_kmeans_large, "d960": bench_precision_d960,
+        only = {"kmeans": lambda ya_, dev_: bench_kmeans(
+                    ya_, dev_, mixture(3, 100_000, 128, 1024, 20, centres_seed=0), 1024, 10),
+                "kmeans-large": bench_kmeans_large, "d960": bench_precision_d960,
                 "d960-bf16": lambda ya_, dev_: bench_precision_d960(ya_, dev_, fp32=False),
                 "batched": bench_batched}[args.only]
         rec = only(ya, dev)
