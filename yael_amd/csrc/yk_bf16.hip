@@ -1405,9 +1405,16 @@ __global__ __launch_bounds__(64) void yk_rerank_tile_kernel(
   double ebf, e32, bound;
   yk_rerank_bound(dot, qstats[qi], *bmax, gam_bf, gam_32, cq[k - 1], &ebf, &e32, &bound);
   if (!yk_rerank_needed(cq, kp, t0, bound)) return;
+  // A candidate whose first-pass value is above the bound cannot be among
+  // the k nearest (its FP32-form value exceeds the k-th's, as for a whole
+  // tile above): its row is not gathered (the lane reads row 0, shared) and
+  // its value is not written.
+  auto beyond = [&](uint64_t kk) {
+    return kk == YK_KEY_NONE || (double)yk_fkey_inv((uint32_t)(kk >> 32)) > bound;
+  };
   const int ci = t0 + r;
   const uint64_t key = ci < kp ? cq[ci] : YK_KEY_NONE;
-  const long row = key == YK_KEY_NONE ? 0 : (long)(uint32_t)key;
+  const long row = beyond(key) ? 0 : (long)(uint32_t)key;
   const float *bp = b + row * ld + h * 8;
   const float *qp = q + qi * ld + h * 8;
   const int nch = NCH > 0 ? NCH : ld >> 4;
@@ -1463,7 +1470,7 @@ __global__ __launch_bounds__(64) void yk_rerank_tile_kernel(
       if (cidx >= kp) continue;
       const uint64_t ck = cq[cidx];
       uint64_t xk = YK_KEY_NONE;
-      if (ck != YK_KEY_NONE) {
+      if (!beyond(ck)) {
         const int bi = (int)(uint32_t)ck;
         float v = dot ? acc[i] : (float)(qnd + (double)bn[bi]) - 2.0f * acc[i];
         // k = 1 follows nn.c:404-440: only values below 1e30 are taken
