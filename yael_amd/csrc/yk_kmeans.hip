@@ -356,6 +356,41 @@ __global__ __launch_bounds__(256) void yk_segsum_kernel(const float *__restrict_
   }
 }
 
+// The same sums with a cluster's columns split over waves, 64 per wave
+// (one per lane; nsl = ceil(d / 64) waves per cluster): at d = 128 twice the
+// waves of yk_segsum_kernel for the same rows, each column still summed in
+// point order, so the bits are that kernel's.  No normalisation (that needs
+// the whole row in one wave).
+__global__ __launch_bounds__(256) void yk_segsum_cols_kernel(const float *__restrict__ v, int d,
+                                                             int ld, int k, int nsl,
+                                                             const int *__restrict__ offsets,
+                                                             const int *__restrict__ perm,
+                                                             float *__restrict__ cent) {
+  const int lane = threadIdx.x & 63;
+  const long g = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= (long)k * nsl) return;
+  const long c = g / nsl;
+  const int j = (int)(g % nsl) * 64 + lane;
+  const int jl = j < d ? j : 0;  // lanes past d load column 0 and discard it
+  const int beg = offsets[c], end = offsets[c + 1];
+  const double inv = 1.0 / (double)(end - beg);
+  double s = 0.0;
+  for (int pb = beg; pb < end; pb += 64) {
+    const int nb = end - pb < 64 ? end - pb : 64;
+    const int pidx = lane < nb ? perm[pb + lane] : 0;
+    int t = 0;
+    for (; t + 8 <= nb; t += 8) {
+      float x[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) x[u] = v[(long)__shfl(pidx, t + u) * ld + jl];
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += (double)x[u];
+    }
+    for (; t < nb; t++) s += (double)v[(long)__shfl(pidx, t) * ld + jl];
+  }
+  if (j < d) cent[c * ld + j] = (float)(s * inv);
+}
+
 // L2 normalisation of k rows of d > 1024 floats (the segmented-sum kernel
 // keeps at most 1024 columns of a row in LDS): one wave per row, 64 columns
 // per coalesced load, the squares then added by lane 0 in column order
@@ -456,8 +491,14 @@ static int yk_kmeans_update_enqueue(long n, int k, int d, const float *v, const 
     YK_HIP(rocprim::radix_sort_pairs(w.sort_tmp, bytes, assign, w.keys_out, w.iota, w.perm,
                                      (size_t)n, 0, yk_bits_for(k), st));
   }
-  hipLaunchKernelGGL(yk_segsum_kernel, dim3((k + 3) / 4), dim3(256), 0, st, v, d, ld, k,
-                     w.offsets, w.perm, cent, normalize);
+  if (normalize) {
+    hipLaunchKernelGGL(yk_segsum_kernel, dim3((k + 3) / 4), dim3(256), 0, st, v, d, ld, k,
+                       w.offsets, w.perm, cent, normalize);
+  } else {
+    const int nsl = (d + 63) / 64;
+    hipLaunchKernelGGL(yk_segsum_cols_kernel, dim3((unsigned)(((long)k * nsl + 3) / 4)),
+                       dim3(256), 0, st, v, d, ld, k, nsl, w.offsets, w.perm, cent);
+  }
   if (normalize && d > 1024)
     hipLaunchKernelGGL(yk_normalize_rows_kernel, dim3((k + 3) / 4), dim3(256), 0, st, cent, d,
                        ld, k);
